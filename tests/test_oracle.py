@@ -1,0 +1,231 @@
+"""The CPU oracle against the reference's golden vectors, published known
+answers, the compiled reference headers (oracle/_ref) and the invariants the
+mathematics fixes.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+P = O.P
+
+
+def test_field_constants(golden):
+    f = golden["field"]
+    assert P == int(f["P"])
+    assert P == 2**62 - 3 * 2**25 + 1
+    assert (P - 1) == 2**25 * 47189 * 2912521
+    assert O.lib().orc_primitive_root() == f["primitive_root"]
+    psi = O.ntt_psi(1 << 20)
+    assert psi == int(f["psi_2_20"])
+    assert pow(psi, 1 << 20, P) == P - 1          # psi^N = -1
+    assert O.lib().orc_finv(12345) * 12345 % P == 1
+
+
+def test_mt19937_64_standard_value():
+    # The C++ standard fixes the 10000th output of a default-seeded engine.
+    assert int(O.MT64(5489).take(10000)[-1]) == 9981545732273789042
+
+
+def test_chacha8_kat(golden):
+    ks = O.chacha8_block(bytes(32))
+    assert ks.hex() == golden["chacha8_zero_key"]["keystream_hex_prefix"]
+
+
+def test_aes_fips197(golden):
+    for case in golden["aes128_fips197"]["cases"]:
+        rk = O.aes128_expand_key(bytes.fromhex(case["key"]))
+        assert O.aes128_encrypt(rk, bytes.fromhex(case["pt"])).hex() == case["ct"]
+
+
+def test_aes_key_schedule_fips197_a1():
+    # FIPS-197 A.1: last round key of 2b7e1516... is d014f9a8c9ee2589e13f0cc8b6630ca6
+    rk = O.aes128_expand_key(O.FIPS197_MASTER)
+    assert rk[160:176].hex() == "d014f9a8c9ee2589e13f0cc8b6630ca6"
+
+
+@pytest.mark.parametrize("logN", [1, 2, 3, 5, 8])
+def test_ntt_matches_schoolbook(logN):
+    rng = np.random.default_rng(logN)
+    N = 1 << logN
+    a = rng.integers(0, P, N, dtype=np.uint64)
+    b = rng.integers(0, P, N, dtype=np.uint64)
+    assert np.array_equal(O.poly_mul(a, b), O.poly_mul_schoolbook(a, b))
+    # forward is evaluation at psi^(2k+1) in natural order
+    psi = O.ntt_psi(N)
+    fa = O.ntt_forward(a)
+    for k in (0, 1, N - 1):
+        x = pow(psi, 2 * k + 1, P)
+        assert int(fa[k]) == sum(int(a[i]) * pow(x, i, P) for i in range(N)) % P
+    assert np.array_equal(O.ntt_inverse(fa), a)
+
+
+def test_poly_mul_patterns():
+    # zero / one / impulse / max patterns (GPU_baseline/test/poly_mul_cuda.cpp:56-71)
+    N = 64
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, P, N, dtype=np.uint64)
+    one = np.zeros(N, np.uint64); one[0] = 1
+    zero = np.zeros(N, np.uint64)
+    assert np.array_equal(O.poly_mul(a, one), a)
+    assert np.array_equal(O.poly_mul(a, zero), zero)
+    imp = np.zeros(N, np.uint64); imp[N - 1] = 1       # X^(N-1): negacyclic shift
+    got = O.poly_mul(one, imp)
+    assert int(got[N - 1]) == 1
+    x = np.zeros(N, np.uint64); x[1] = 1
+    assert int(O.poly_mul(imp, x)[0]) == P - 1          # X^N = -1
+    mx = np.full(N, P - 1, np.uint64)
+    assert np.array_equal(O.poly_mul(mx, mx), O.poly_mul_schoolbook(mx, mx))
+
+
+def test_ntt_naive_recipe_checksum(golden):
+    N, batch = 1 << 20, golden["ntt_naive_fnv"]["batch"]
+    m = O.MT64(11)
+    a = m.take(batch * N, P).reshape(batch, N)
+    b = m.take(batch * N, P).reshape(batch, N)
+    c = O.poly_mul_batch(a, b, threads=8)
+    assert "%016x" % O.fnv1a64(c) == golden["ntt_naive_fnv"]["by_logN"]["20"]
+
+
+def test_ntt_4step_recipe_checksum(golden):
+    N, batch = 1 << 20, golden["ntt_4step_xor"]["batch"]
+    j = np.arange(batch * N, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        a = (np.uint64(0x9E3779B97F4A7C15) * (j + np.uint64(1))) % np.uint64(P)
+        b = (np.uint64(0xC2B2AE3D27D4EB4F) * (j + np.uint64(7))) % np.uint64(P)
+    c = O.poly_mul_batch(a.reshape(batch, N), b.reshape(batch, N), threads=8)
+    assert "%016x" % int(np.bitwise_xor.reduce(c.ravel())) == \
+        golden["ntt_4step_xor"]["by_logN"]["20"]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_dpf_plateau_checksum(golden, case):
+    c = golden["dpf_plateau"]["cases"][case]
+    got = O.dpf_real_bench_checksum(c["n"], c["B"], c["t"], 1, 8)
+    assert "%016x" % got == c["checksum"]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("case", [3, 4])
+def test_dpf_plateau_checksum_large(golden, case):
+    c = golden["dpf_plateau"]["cases"][case]
+    got = O.dpf_real_bench_checksum(c["n"], c["B"], c["t"], 1, 8)
+    assert "%016x" % got == c["checksum"]
+
+
+def _dealt_instance(n, w, seed=5, B=3):
+    rng = np.random.default_rng(seed)
+    m, levels = 1 << w, n // w
+    nk = m - 1
+    alphas = rng.integers(0, 1 << n, B)
+    d0 = rng.integers(0, 2**63, (B, 2), dtype=np.uint64)
+    d1 = rng.integers(0, 2**63, (B, 2), dtype=np.uint64)
+    d0[:, 0] |= np.uint64(1)
+    d1[:, 0] &= ~np.uint64(1)
+    W = rng.integers(0, 2**63, 2, dtype=np.uint64)
+    delta = d0 ^ d1
+    dl0 = rng.integers(0, 2**63, (B, levels, nk, 2), dtype=np.uint64)
+    dl1 = dl0.copy()
+    for b in range(B):
+        for i in range(levels):
+            k = O.lib().orc_dpf_digit(n, w, i, int(alphas[b]))
+            if k < nk:
+                dl1[b, i, k] ^= delta[b]
+    return alphas, d0 ^ W, d1 ^ W, dl0, dl1, delta
+
+
+@pytest.mark.parametrize("w,prg", [(1, O.PRG_CHACHA8), (2, O.PRG_CHACHA8), (3, O.PRG_CHACHA8),
+                                   (1, O.PRG_AES), (2, O.PRG_AES)])
+def test_dpf_point_function_and_eval(w, prg):
+    n = 6 if w != 3 else 6
+    rks = O.aes_tree_round_keys((1 << w) - 1) if prg == O.PRG_AES else None
+    alphas, r0, r1, dl0, dl1, delta = _dealt_instance(n, w)
+    B = len(alphas)
+    l0, l1, cw = O.dpf_full_eval_both(n, w, prg, r0, r1, dl0, dl1, rks=rks, want_cw=True)
+    D = 1 << n
+    m, levels = 1 << w, n // w
+    x = (l0 ^ l1).reshape(B, D, 2)
+    for b in range(B):
+        for j in range(D):
+            exp = delta[b] if j == alphas[b] else np.zeros(2, np.uint64)
+            assert np.array_equal(x[b, j], exp)      # leaf0 ^ leaf1 = Delta*[j=alpha]
+        cwb = cw.reshape(B, levels * m, 2)[b]
+        # brute force: eval() of every point walks to the same leaves
+        for j in range(D):
+            assert O.dpf_eval(n, w, prg, r0[b], cwb, j, rks=rks) == tuple(int(v) for v in l0[b * D + j])
+            assert O.dpf_eval(n, w, prg, r1[b], cwb, j, rks=rks) == tuple(int(v) for v in l1[b * D + j])
+        # the fused protocol's CWs equal the path-only CWs (off-path nodes agree)
+        pcw = O.dpf_path_cw(n, w, prg, r0[b], r1[b], dl0[b], dl1[b], int(alphas[b]), rks=rks)
+        assert np.array_equal(pcw, cwb)
+
+
+def test_tree_prg_chacha8_is_keystream_chunks():
+    rng = np.random.default_rng(9)
+    par = rng.integers(0, 2**63, (4, 2), dtype=np.uint64)
+    h = O.tree_hash_chacha8(par, 7)
+    for j in range(4):
+        key = int(par[j, 0]).to_bytes(8, "little") + int(par[j, 1]).to_bytes(8, "little")
+        ks = O.chacha8_block(key, 0, 0) + O.chacha8_block(key, 0, 1)
+        for k in range(7):
+            chunk = ks[16 * k:16 * k + 16]
+            assert int(h[j * 7 + k, 0]) == int.from_bytes(chunk[:8], "little")
+            assert int(h[j * 7 + k, 1]) == int.from_bytes(chunk[8:], "little")
+    oh = O.out_hash(par)
+    key = int(par[0, 0]).to_bytes(8, "little") + int(par[0, 1]).to_bytes(8, "little")
+    assert int(oh[0, 0]) == int.from_bytes(O.chacha8_block(key, O.OUT_DOMAIN, 0)[:8], "little")
+
+
+def test_aes_tree_prg_is_ccr():
+    rks = O.aes_tree_round_keys(3)
+    mrk = O.aes128_expand_key(O.FIPS197_MASTER)
+    for k in range(3):
+        tweak = k.to_bytes(8, "little") + bytes(8)
+        sub = bytes(a ^ b for a, b in zip(O.aes128_encrypt(mrk, tweak), tweak))
+        assert bytes(rks[k * 176:(k + 1) * 176]) == O.aes128_expand_key(sub)
+
+
+@pytest.mark.parametrize("N,c,t,prg", [(1 << 12, 2, 16, O.PRG_CHACHA8), (1 << 10, 2, 4, O.PRG_AES),
+                                       (1 << 11, 3, 8, O.PRG_CHACHA8)])
+def test_ole_relation(N, c, t, prg):
+    x0, z0, x1, z1 = O.pcg_ole_expand(N, c, t, 1, prg, seed=7, threads=8)
+    prod = O.poly_mul(x0, x1)
+    s = (z0.astype(object) + z1.astype(object)) % P
+    assert all(int(p) == int(q) for p, q in zip(prod, s))
+    assert int((x0 != 0).sum()) > N // 2            # x is dense (a_i e_i)
+
+
+def test_ole_relation_fails_on_planted_error():
+    # a planted mistake in the oracle's output must be caught by the relation
+    N = 1 << 10
+    x0, z0, x1, z1 = O.pcg_ole_expand(N, 2, 4, 1, O.PRG_CHACHA8, seed=3, threads=4)
+    z0 = z0.copy()
+    z0[5] = (int(z0[5]) + 1) % P
+    prod = O.poly_mul(x0, x1)
+    s = (z0.astype(object) + z1.astype(object)) % P
+    assert any(int(p) != int(q) for p, q in zip(prod, s))
+
+
+def test_ref_headers_agree_with_oracle():
+    ref = O.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(1)
+    for logN in (4, 9, 12):
+        N = 1 << logN
+        a = rng.integers(0, P, N, dtype=np.uint64)
+        b = rng.integers(0, P, N, dtype=np.uint64)
+        ra = a.copy()
+        ref.ref_ntt_forward(O._p(ra), N)
+        assert np.array_equal(ra, O.ntt_forward(a))
+        ri = ra.copy()
+        ref.ref_ntt_inverse(O._p(ri), N)
+        assert np.array_equal(ri, a)
+        rc = np.empty_like(a)
+        ref.ref_poly_mul(O._p(rc), O._p(a), O._p(b), N)
+        assert np.array_equal(rc, O.poly_mul(a, b))
+        assert ref.ref_negacyclic_psi(N) == O.ntt_psi(N)
+    import ctypes as C
+    key = bytes(rng.integers(0, 256, 32, dtype=np.uint8))
+    out = (C.c_uint8 * 128)()
+    ref.ref_chacha8_blocks((C.c_uint8 * 32).from_buffer_copy(key), O.OUT_DOMAIN, 5, out, 2)
+    assert bytes(out)[:64] == O.chacha8_block(key, O.OUT_DOMAIN, 5)
+    assert bytes(out)[64:] == O.chacha8_block(key, O.OUT_DOMAIN, 6)
