@@ -1,0 +1,223 @@
+/*
+ * pcgsq.h -- C ABI of libpcgsq, the MI355X (gfx950) implementation of the
+ * PCG-OLE expansion path of tkgong/pcg-square.
+ *
+ * Every entry point replaces a function of the reference's device plugin seam
+ * (GPU_baseline/common/*.h, namespace pcg_cuda) or a local-compute entry of
+ * CPU_baseline, cited next to it.  Conventions:
+ *   - return value: PCGSQ_OK (0) or a PCGSQ_E* status; no C++ exception ever
+ *     crosses this boundary; pcgsq_last_error() holds the message (per thread);
+ *   - PCGSQ_EUNSUPPORTED is the reference's "exit 77" case (shape or prime
+ *     the device path does not cover);
+ *   - all buffers are plain pointers; "host" arguments are host memory, "dev"
+ *     arguments are device memory of the context's GPU;
+ *   - a context owns its device scratch (no hidden process-global pools);
+ *     calls on one context are serialized by the context; different contexts
+ *     are independent.
+ * Field: P = 2^62 - 3*2^25 + 1 (the only prime accepted, as in
+ * GPU_baseline/common/poly_mul_cuda.cu:389-393).
+ */
+#ifndef PCGSQ_H
+#define PCGSQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCGSQ_PRIME 4611686018326724609ULL
+
+enum {
+  PCGSQ_OK = 0,
+  PCGSQ_EINVAL = 1,
+  PCGSQ_EUNSUPPORTED = 2,
+  PCGSQ_EDEVICE = 3,
+  PCGSQ_ENOMEM = 4,
+  PCGSQ_ECALLBACK = 5,
+  PCGSQ_EINTERNAL = 6
+};
+
+/* Tree PRG (half_tree_dpf::PRGType, dpf.h:95; pcg_cuda::DpfGpuPrg, dpf_gpu.h:38). */
+enum { PCGSQ_PRG_AES = 0, PCGSQ_PRG_CHACHA8 = 1 };
+
+/* Poly-mul / NTT backends (pcg_ole::PolyMulBackend, pcg_ole.h:33). */
+enum { PCGSQ_NTT_FOURSTEP = 0, PCGSQ_NTT_NAIVE = 1 };
+
+/* 128-bit block, lo = bytes 0..7, hi = bytes 8..15 (pcg_cuda::DpfBlk,
+ * dpf_gpu.h:23-26; the in-memory layout of emp::block). */
+typedef struct pcgsq_blk {
+  uint64_t lo, hi;
+} pcgsq_blk;
+
+typedef struct pcgsq_ctx pcgsq_ctx;
+
+/* ---- context / runtime -------------------------------------------------- */
+/* Replaces pcg_cuda::is_cuda_available (poly_mul_cuda.h:9): 1 when a gfx950
+ * device is visible. */
+int pcgsq_device_available(void);
+/* Device name/runtime strings (pcg_cuda::device_name / runtime_version,
+ * poly_mul_cuda.h:11-12). */
+int pcgsq_device_name(char* buf, size_t len);
+const char* pcgsq_last_error(void);
+int pcgsq_ctx_create(int device, pcgsq_ctx** out);
+int pcgsq_ctx_destroy(pcgsq_ctx* ctx);
+/* Stream the context launches on (hipStream_t, non-null, default priority). */
+void* pcgsq_ctx_stream(pcgsq_ctx* ctx);
+int pcgsq_ctx_synchronize(pcgsq_ctx* ctx);
+/* Device memory for ctypes callers that hold their own device-resident
+ * buffers (no counterpart in the reference, whose plugin only takes host
+ * pointers). */
+int pcgsq_dev_alloc(pcgsq_ctx* ctx, size_t bytes, void** dev_out);
+int pcgsq_dev_free(pcgsq_ctx* ctx, void* dev);
+int pcgsq_memcpy_h2d(pcgsq_ctx* ctx, void* dev_dst, const void* host_src,
+                     size_t bytes);
+int pcgsq_memcpy_d2h(pcgsq_ctx* ctx, void* host_dst, const void* dev_src,
+                     size_t bytes);
+
+/* ---- half-tree DPF full-domain evaluation -------------------------------- */
+/* Per-level correction-word exchange (pcg_cuda::DpfExchangeFn,
+ * dpf_gpu.h:31-32, as a C function pointer + user argument): fill
+ * peer[0..count) from mine[0..count) for tree level `level`; return 0, or
+ * nonzero to abort the expansion with PCGSQ_ECALLBACK. */
+typedef int (*pcgsq_exchange_fn)(void* user, int level, const pcgsq_blk* mine,
+                                 pcgsq_blk* peer, size_t count);
+
+/* pcg_cuda::dpf_gpu_batch_full_eval (dpf_gpu.h:55-62): one party's batched
+ * half-tree expansion; host roots[B], dltsft[B*levels*nk] ([b][i][k]),
+ * round_keys nk*176 bytes for AES (NULL for ChaCha8), host out_leaves[B*2^n]. */
+int pcgsq_dpf_batch_full_eval(pcgsq_ctx* ctx, int B, int n, int w,
+                              const pcgsq_blk* roots, const pcgsq_blk* dltsft,
+                              const uint8_t* round_keys, int party,
+                              pcgsq_exchange_fn exchange, void* user,
+                              pcgsq_blk* out_leaves, int prg);
+
+/* pcg_cuda::dpf_gpu_batch_full_eval_device (dpf_gpu.h:69-75): same, leaves
+ * stay on the device; *d_leaves aliases context scratch and is valid until
+ * the next DPF call on this context. */
+int pcgsq_dpf_batch_full_eval_device(pcgsq_ctx* ctx, int B, int n, int w,
+                                     const pcgsq_blk* roots,
+                                     const pcgsq_blk* dltsft,
+                                     const uint8_t* round_keys, int party,
+                                     pcgsq_exchange_fn exchange, void* user,
+                                     const pcgsq_blk** d_leaves, int prg);
+
+/* Both parties in one process (no counterpart in the reference, which runs
+ * one party per process: half_tree_dpf.cpp test drivers).  The exchange is
+ * the other party's reduced share, already on the device.  Host arrays as
+ * above per party; optional cw_out[levels*B*nk] ([i][b][k]) receives the
+ * reconstructed CWs (k < m-1). */
+int pcgsq_dpf_batch_full_eval_both(pcgsq_ctx* ctx, int B, int n, int w,
+                                   const pcgsq_blk* roots0,
+                                   const pcgsq_blk* roots1,
+                                   const pcgsq_blk* dltsft0,
+                                   const pcgsq_blk* dltsft1,
+                                   const uint8_t* round_keys,
+                                   pcgsq_blk* leaves0, pcgsq_blk* leaves1,
+                                   pcgsq_blk* cw_out, int prg);
+
+/* ---- output layer (leaf_convert_cuda.h) ----------------------------------- */
+/* pcg_cuda::dpf_out_sums (leaf_convert_cuda.h:43-44): d_leaves device
+ * [B][D]; host sumC[B] (mod P), sumT[B]. */
+int pcgsq_dpf_out_sums(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B,
+                       size_t D, uint64_t prime, uint64_t* sumC,
+                       uint64_t* sumT);
+/* pcg_cuda::dpf_out_scatter_g (leaf_convert_cuda.h:49-52): host CW[B],
+ * host g_out[N] overwritten with the folded y = +-(C + tau*CW). */
+int pcgsq_dpf_out_scatter_g(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B,
+                            size_t D, int t, int party, uint64_t prime,
+                            const uint64_t* CW, uint64_t* g_out, int N);
+
+/* ---- polynomial multiplication over R_p = F_p[X]/(X^N+1) ------------------ */
+/* pcg_cuda::poly_mul_supported / poly_mul_gpuntt_square_supported
+ * (poly_mul_cuda.h:10,163): 1 if supported, else 0 with a reason. */
+int pcgsq_poly_mul_supported(uint64_t prime, int N, int backend, char* reason,
+                             size_t reason_len);
+/* pcg_cuda::poly_mul_u64 / poly_mul_u64_gpuntt_square (poly_mul_cuda.h:
+ * 154-166): host buffers of batch*N words; out may alias a or b.
+ * device_ms (nullable) receives the device time of the call. */
+int pcgsq_poly_mul_u64(pcgsq_ctx* ctx, uint64_t* out, const uint64_t* a,
+                       const uint64_t* b, int batch, int N, uint64_t prime,
+                       int backend, double* device_ms);
+/* Device-buffer variant (no copies): dev_out, dev_a, dev_b. */
+int pcgsq_poly_mul_u64_dev(pcgsq_ctx* ctx, uint64_t* dev_out,
+                           const uint64_t* dev_a, const uint64_t* dev_b,
+                           int batch, int N, int backend);
+/* NTT<P>::forward / inverse (CPU_baseline/ntt.h:65-78), batched, host
+ * buffers, in place; forward output in natural order a(psi^(2k+1)). */
+int pcgsq_ntt_forward(pcgsq_ctx* ctx, uint64_t* a, int batch, int N,
+                      int backend);
+int pcgsq_ntt_inverse(pcgsq_ctx* ctx, uint64_t* a, int batch, int N,
+                      int backend);
+int pcgsq_ntt_forward_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
+                          int backend);
+int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
+                          int backend);
+
+/* ---- AES (aes_gpu.h) -------------------------------------------------------- */
+/* pcg_cuda::aes128_expand_key_host (aes_gpu.h:13). */
+int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]);
+/* pcg_cuda::aes128_ecb_encrypt_gpu (aes_gpu.h:18-19): count blocks, one
+ * 16-byte key per block, host buffers. */
+int pcgsq_aes128_ecb_encrypt(pcgsq_ctx* ctx, const uint8_t* keys,
+                             const uint8_t* in, uint8_t* out, int count);
+/* AES tree-PRG round keys for nk sub-keys derived from a master key
+ * (AESTreePRG ctor, half_tree_dpf/tree_prg.h:45-56): rks[nk*176]. */
+int pcgsq_aes_tree_round_keys(const uint8_t master[16], int nk, uint8_t* rks);
+
+/* ---- PCG-OLE expansion (pcg_ole::PCG_OLE<P>::gen_and_expand) --------------- */
+/* pcg_ole::PCGParams (pcg_ole.h:271-281) without the backends, which are
+ * this library. */
+typedef struct pcgsq_params {
+  int N;    /* ring dimension, power of two, 2N | P-1 */
+  int c;    /* sparse polynomials per party */
+  int t;    /* nonzeros per sparse polynomial, t | N */
+  int w;    /* DPF arity exponent, w | log2(2N/t) */
+  int prg;  /* PCGSQ_PRG_* */
+} pcgsq_params;
+
+typedef struct pcgsq_ole_plan pcgsq_ole_plan;
+
+/* Plan for one parameter set: device buffers, the public a_i (drawn from
+ * pub_seed) and their NTTs stay resident.  hbm_budget_bytes bounds the
+ * expansion's working set (0 = choose). */
+int pcgsq_ole_plan_create(pcgsq_ctx* ctx, const pcgsq_params* p,
+                          uint64_t pub_seed, size_t hbm_budget_bytes,
+                          pcgsq_ole_plan** out);
+int pcgsq_ole_plan_destroy(pcgsq_ole_plan* plan);
+/* Device bytes the plan holds. */
+size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* plan);
+/* Deal the per-expansion inputs from `seed` (the seeded stand-in for the
+ * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199)
+ * and place them on the device. */
+int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed);
+/* Run the whole expansion for both parties on the device-resident inputs
+ * (DPF batches, output layer with Beaver opens, step 4), asynchronously on
+ * the context stream. */
+int pcgsq_ole_run(pcgsq_ole_plan* plan);
+/* Copy the outputs (x_s, z_s with z0 + z1 = x0 * x1) to the host; any
+ * pointer may be NULL.  Synchronizes. */
+int pcgsq_ole_outputs(pcgsq_ole_plan* plan, uint64_t* x0, uint64_t* z0,
+                      uint64_t* x1, uint64_t* z1);
+/* Device pointers of the outputs [4][N] (x0, z0, x1, z1). */
+const uint64_t* pcgsq_ole_outputs_dev(pcgsq_ole_plan* plan);
+/* Convenience: deal + run + outputs. */
+int pcgsq_ole_expand(pcgsq_ctx* ctx, const pcgsq_params* p, uint64_t seed,
+                     uint64_t pub_seed, uint64_t* x0, uint64_t* z0,
+                     uint64_t* x1, uint64_t* z1);
+/* Only the DPF blocks and the output layer (no step 4): afterwards
+ * pcgsq_ole_g returns the folded g_ij of every block for both parties,
+ * [2][c*c][N] (pcg_ole_impl.h:432-449 + the fold of :474-478).  After a full
+ * pcgsq_ole_run that buffer holds NTT-domain values instead. */
+int pcgsq_ole_run_blocks(pcgsq_ole_plan* plan);
+int pcgsq_ole_g(pcgsq_ole_plan* plan, uint64_t* g_host);
+
+/* Dealer value U(seed, domain, a, b) of the documented dealer spec. */
+uint64_t pcgsq_dealer_u64(uint64_t seed, uint32_t domain, uint64_t a,
+                          uint64_t b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

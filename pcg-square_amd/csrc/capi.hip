@@ -1,0 +1,493 @@
+// C ABI entry points (include/pcgsq.h) over the context, DPF, output-layer,
+// NTT and AES engines.  Every function runs inside guarded(): exceptions
+// become status codes and a per-thread last-error string.
+#include <string.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "pcgsq.h"
+#include "prg.hpp"
+
+namespace pcgsq {
+
+namespace {
+thread_local std::string t_last_error;
+
+__global__ void aes_ecb_kernel(const uint8_t* __restrict__ keys, const uint8_t* __restrict__ in,
+                               uint8_t* __restrict__ out, int count, const uint32_t* t0_g) {
+  __shared__ uint32_t t0[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) t0[i] = t0_g[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count) return;
+  // key schedule per block from the 16-byte key (on device, FIPS-197 5.2)
+  uint32_t rk[44];
+  for (int i = 0; i < 4; ++i) {
+    const uint8_t* p = keys + 16 * (size_t)b + 4 * i;
+    rk[i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+  }
+  uint32_t rcon = 1;
+  for (int i = 4; i < 44; ++i) {
+    uint32_t tmp = rk[i - 1];
+    if (i % 4 == 0) {
+      // RotWord then SubWord (bytes little endian in the word)
+      tmp = (tmp >> 8) | (tmp << 24);
+      const uint32_t s0 = (t0[tmp & 0xff] >> 8) & 0xff, s1 = (t0[(tmp >> 8) & 0xff] >> 8) & 0xff,
+                     s2 = (t0[(tmp >> 16) & 0xff] >> 8) & 0xff, s3 = (t0[tmp >> 24] >> 8) & 0xff;
+      tmp = (s0 | (s1 << 8) | (s2 << 16) | (s3 << 24)) ^ rcon;
+      rcon = (rcon << 1) ^ ((rcon & 0x80) ? 0x11b : 0);
+    }
+    rk[i] = rk[i - 4] ^ tmp;
+  }
+  const uint8_t* p = in + 16 * (size_t)b;
+  uint32_t s[4];
+  for (int i = 0; i < 4; ++i)
+    s[i] = (uint32_t)p[4 * i] | ((uint32_t)p[4 * i + 1] << 8) | ((uint32_t)p[4 * i + 2] << 16) |
+           ((uint32_t)p[4 * i + 3] << 24);
+  aes128_encrypt_words(t0, rk, s[0], s[1], s[2], s[3]);
+  uint8_t* o = out + 16 * (size_t)b;
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k) o[4 * i + k] = (uint8_t)(s[i] >> (8 * k));
+}
+
+void upload(DevBuf& buf, const void* src, size_t bytes, hipStream_t st, const char* what) {
+  buf.ensure(bytes, what);
+  if (bytes) hip_check(hipMemcpyAsync(buf.ptr, src, bytes, hipMemcpyHostToDevice, st), what);
+}
+
+// Device copies of the AES T-table and round keys when a host schedule is given.
+void stage_aes(pcgsq_ctx* ctx, const uint8_t* round_keys, int nk, DpfRun& r) {
+  if (r.prg != kPrgAes) return;
+  require(round_keys != nullptr, "AES PRG requires round_keys", kErrInvalid);
+  uint32_t t0[256];
+  aes_ttable_host(t0);
+  upload(ctx->t0, t0, sizeof(t0), ctx->stream, "aes t-table");
+  upload(ctx->rks, round_keys, (size_t)nk * 176, ctx->stream, "aes round keys");
+  r.aes_t0 = ctx->t0.as<uint32_t>();
+  r.aes_rks = ctx->rks.as<uint8_t>();
+}
+
+struct HostEx {
+  pcgsq_exchange_fn fn;
+  void* user;
+};
+int host_ex_tramp(void* u, int level, const Blk* mine, Blk* peer, size_t count) {
+  HostEx* h = static_cast<HostEx*>(u);
+  return h->fn(h->user, level, reinterpret_cast<const pcgsq_blk*>(mine),
+               reinterpret_cast<pcgsq_blk*>(peer), count);
+}
+
+Blk* dpf_single(pcgsq_ctx* ctx, int B, int n, int w, const pcgsq_blk* roots,
+                const pcgsq_blk* dltsft, const uint8_t* round_keys, pcgsq_exchange_fn exchange,
+                void* user, int prg) {
+  require(B > 0, "dpf: B must be positive");
+  require(roots && dltsft && exchange, "dpf: null argument");
+  require(w >= 1 && w <= 4 && n >= w && n % w == 0, "dpf: w must be in [1,4] and divide n");
+  const int levels = n / w, nk = (1 << w) - 1;
+  hipStream_t st = ctx->stream;
+  upload(ctx->in_a, roots, sizeof(Blk) * B, st, "dpf roots");
+  upload(ctx->in_b, dltsft, sizeof(Blk) * (size_t)B * levels * nk, st, "dpf dltsft");
+  ctx->leaves.ensure(sizeof(Blk) * ((size_t)B << n), "dpf leaves");
+  DpfRun r;
+  r.ntrees = B;
+  r.n = n;
+  r.w = w;
+  r.prg = prg;
+  r.roots = ctx->in_a.as<Blk>();
+  r.dltsft = ctx->in_b.as<Blk>();
+  stage_aes(ctx, round_keys, nk, r);
+  HostEx hx{exchange, user};
+  r.host_ex = host_ex_tramp;
+  r.host_user = &hx;
+  r.mode = kLeafStore;
+  r.leaves = ctx->leaves.as<Blk>();
+  dpf_expand(st, r, ctx->dpf);
+  return r.leaves;
+}
+
+int status_of_hip(hipError_t e) { return e == hipErrorOutOfMemory ? kErrNoMemory : kErrDevice; }
+
+}  // namespace
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+}  // namespace pcgsq
+
+using namespace pcgsq;
+
+extern "C" {
+
+const char* pcgsq_last_error(void) { return t_last_error.c_str(); }
+
+int pcgsq_device_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+  hipDeviceProp_t prop;
+  for (int d = 0; d < n; ++d)
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess &&
+        strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      return 1;
+  return 0;
+}
+
+int pcgsq_device_name(char* buf, size_t len) {
+  return guarded([&] {
+    require(buf && len, "device_name: null buffer");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    int rt = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", HIP runtime " +
+                    std::to_string(rt) + ")";
+    strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+  });
+}
+
+int pcgsq_ctx_create(int device, pcgsq_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, "ctx_create: null out");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    require(device >= 0 && device < n, "ctx_create: no such device", kErrUnsupported);
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    require(strncmp(prop.gcnArchName, "gfx950", 6) == 0,
+            std::string("ctx_create: device is ") + prop.gcnArchName + ", built for gfx950",
+            kErrUnsupported);
+    std::unique_ptr<pcgsq_ctx> c(new pcgsq_ctx);
+    c->device = device;
+    c->activate();
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = c.release();
+  });
+}
+
+int pcgsq_ctx_destroy(pcgsq_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    {
+      CtxGuard g(ctx);
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+  });
+}
+
+void* pcgsq_ctx_stream(pcgsq_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pcgsq_ctx_synchronize(pcgsq_ctx* ctx) {
+  return guarded([&] {
+    require(ctx, "null ctx");
+    CtxGuard g(ctx);
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+int pcgsq_dev_alloc(pcgsq_ctx* ctx, size_t bytes, void** dev_out) {
+  return guarded([&] {
+    require(ctx && dev_out, "dev_alloc: null argument");
+    CtxGuard g(ctx);
+    *dev_out = nullptr;
+    hip_check(hipMalloc(dev_out, bytes ? bytes : 1), "hipMalloc");
+  });
+}
+
+int pcgsq_dev_free(pcgsq_ctx* ctx, void* dev) {
+  return guarded([&] {
+    require(ctx, "dev_free: null ctx");
+    CtxGuard g(ctx);
+    hip_check(hipStreamSynchronize(ctx->stream), "dev_free");
+    if (dev) hip_check(hipFree(dev), "hipFree");
+  });
+}
+
+int pcgsq_memcpy_h2d(pcgsq_ctx* ctx, void* dev_dst, const void* host_src, size_t bytes) {
+  return guarded([&] {
+    require(ctx && dev_dst && host_src, "memcpy_h2d: null argument");
+    CtxGuard g(ctx);
+    hip_check(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    hip_check(hipStreamSynchronize(ctx->stream), "H2D");
+  });
+}
+
+int pcgsq_memcpy_d2h(pcgsq_ctx* ctx, void* host_dst, const void* dev_src, size_t bytes) {
+  return guarded([&] {
+    require(ctx && host_dst && dev_src, "memcpy_d2h: null argument");
+    CtxGuard g(ctx);
+    hip_check(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "D2H");
+  });
+}
+
+int pcgsq_dpf_batch_full_eval(pcgsq_ctx* ctx, int B, int n, int w, const pcgsq_blk* roots,
+                              const pcgsq_blk* dltsft, const uint8_t* round_keys, int party,
+                              pcgsq_exchange_fn exchange, void* user, pcgsq_blk* out_leaves,
+                              int prg) {
+  return guarded([&] {
+    require(ctx && out_leaves, "dpf: null argument");
+    require(party == 0 || party == 1, "dpf: party must be 0 or 1");
+    CtxGuard g(ctx);
+    Blk* d = dpf_single(ctx, B, n, w, roots, dltsft, round_keys, exchange, user, prg);
+    hip_check(hipMemcpyAsync(out_leaves, d, sizeof(Blk) * ((size_t)B << n), hipMemcpyDeviceToHost,
+                             ctx->stream), "D2H leaves");
+    hip_check(hipStreamSynchronize(ctx->stream), "D2H leaves");
+  });
+}
+
+int pcgsq_dpf_batch_full_eval_device(pcgsq_ctx* ctx, int B, int n, int w, const pcgsq_blk* roots,
+                                     const pcgsq_blk* dltsft, const uint8_t* round_keys, int party,
+                                     pcgsq_exchange_fn exchange, void* user,
+                                     const pcgsq_blk** d_leaves, int prg) {
+  return guarded([&] {
+    require(ctx && d_leaves, "dpf: null argument");
+    require(party == 0 || party == 1, "dpf: party must be 0 or 1");
+    CtxGuard g(ctx);
+    Blk* d = dpf_single(ctx, B, n, w, roots, dltsft, round_keys, exchange, user, prg);
+    hip_check(hipStreamSynchronize(ctx->stream), "dpf");
+    *d_leaves = reinterpret_cast<const pcgsq_blk*>(d);
+  });
+}
+
+int pcgsq_dpf_batch_full_eval_both(pcgsq_ctx* ctx, int B, int n, int w, const pcgsq_blk* roots0,
+                                   const pcgsq_blk* roots1, const pcgsq_blk* dltsft0,
+                                   const pcgsq_blk* dltsft1, const uint8_t* round_keys,
+                                   pcgsq_blk* leaves0, pcgsq_blk* leaves1, pcgsq_blk* cw_out,
+                                   int prg) {
+  return guarded([&] {
+    require(ctx && roots0 && roots1 && dltsft0 && dltsft1 && leaves0 && leaves1,
+            "dpf_both: null argument");
+    require(B > 0, "dpf_both: B must be positive");
+    require(w >= 1 && w <= 4 && n >= w && n % w == 0, "dpf_both: w must be in [1,4] and divide n");
+    CtxGuard g(ctx);
+    const int levels = n / w, nk = (1 << w) - 1;
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)B * levels * nk;
+    ctx->in_a.ensure(sizeof(Blk) * 2 * B, "dpf roots");
+    ctx->in_b.ensure(sizeof(Blk) * 2 * nd, "dpf dltsft");
+    hip_check(hipMemcpyAsync(ctx->in_a.as<Blk>(), roots0, sizeof(Blk) * B, hipMemcpyHostToDevice, st), "H2D");
+    hip_check(hipMemcpyAsync(ctx->in_a.as<Blk>() + B, roots1, sizeof(Blk) * B, hipMemcpyHostToDevice, st), "H2D");
+    hip_check(hipMemcpyAsync(ctx->in_b.as<Blk>(), dltsft0, sizeof(Blk) * nd, hipMemcpyHostToDevice, st), "H2D");
+    hip_check(hipMemcpyAsync(ctx->in_b.as<Blk>() + nd, dltsft1, sizeof(Blk) * nd, hipMemcpyHostToDevice, st), "H2D");
+    const size_t nleaves = (size_t)B << n;
+    ctx->leaves.ensure(sizeof(Blk) * 2 * nleaves, "dpf leaves");
+    ctx->cwbuf.ensure(sizeof(Blk) * 2 * (size_t)levels * B * nk, "dpf cw");
+    DpfRun r;
+    r.ntrees = 2 * B;
+    r.n = n;
+    r.w = w;
+    r.prg = prg;
+    r.roots = ctx->in_a.as<Blk>();
+    r.dltsft = ctx->in_b.as<Blk>();
+    stage_aes(ctx, round_keys, nk, r);
+    r.mode = kLeafStore;
+    r.leaves = ctx->leaves.as<Blk>();
+    r.cw_out = ctx->cwbuf.as<Blk>();
+    dpf_expand(st, r, ctx->dpf);
+    hip_check(hipMemcpyAsync(leaves0, r.leaves, sizeof(Blk) * nleaves, hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(leaves1, r.leaves + nleaves, sizeof(Blk) * nleaves, hipMemcpyDeviceToHost, st), "D2H");
+    std::vector<Blk> shares;
+    if (cw_out) {
+      shares.resize(2 * (size_t)levels * B * nk);
+      hip_check(hipMemcpyAsync(shares.data(), r.cw_out, sizeof(Blk) * shares.size(),
+                               hipMemcpyDeviceToHost, st), "D2H cw");
+    }
+    hip_check(hipStreamSynchronize(st), "dpf_both");
+    if (cw_out) {
+      // shares are [levels][2B][nk]; CW = share(party 0) ^ share(party 1)
+      Blk* o = reinterpret_cast<Blk*>(cw_out);
+      for (int i = 0; i < levels; ++i)
+        for (int b = 0; b < B; ++b)
+          for (int k = 0; k < nk; ++k) {
+            const Blk a0 = shares[((size_t)i * 2 * B + b) * nk + k];
+            const Blk a1 = shares[((size_t)i * 2 * B + B + b) * nk + k];
+            o[((size_t)i * B + b) * nk + k] = Blk{a0.lo ^ a1.lo, a0.hi ^ a1.hi};
+          }
+    }
+  });
+}
+
+int pcgsq_dpf_out_sums(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B, size_t D, uint64_t prime,
+                       uint64_t* sumC, uint64_t* sumT) {
+  return guarded([&] {
+    require(ctx && d_leaves && sumC && sumT, "out_sums: null argument");
+    require(prime == kPrime, "out_sums: only P = 2^62 - 3*2^25 + 1 is supported", kErrUnsupported);
+    require(B > 0, "out_sums: B must be positive");
+    const int D_log = ilog2_exact(D);
+    require(D_log >= 1 && D_log <= 30, "out_sums: D must be a power of two");
+    CtxGuard g(ctx);
+    const int nparts = leaf_sums_nparts(D_log);
+    ctx->part.ensure(sizeof(uint64_t) * 2 * (size_t)B * nparts, "out partials");
+    ctx->sums.ensure(sizeof(uint64_t) * 2 * (size_t)B, "out sums");
+    uint64_t* s = ctx->sums.as<uint64_t>();
+    launch_leaf_sums(ctx->stream, reinterpret_cast<const Blk*>(d_leaves), B, D_log,
+                     ctx->part.as<uint64_t>(), nparts, s, s + B);
+    hip_check(hipMemcpyAsync(sumC, s, 8 * (size_t)B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipMemcpyAsync(sumT, s + B, 8 * (size_t)B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "out_sums");
+  });
+}
+
+int pcgsq_dpf_out_scatter_g(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B, size_t D, int t,
+                            int party, uint64_t prime, const uint64_t* CW, uint64_t* g_out, int N) {
+  return guarded([&] {
+    require(ctx && d_leaves && CW && g_out, "out_scatter_g: null argument");
+    require(prime == kPrime, "out_scatter_g: only P = 2^62 - 3*2^25 + 1 is supported",
+            kErrUnsupported);
+    require(party == 0 || party == 1, "out_scatter_g: party must be 0 or 1");
+    require(B > 0 && t > 0 && N > 0, "out_scatter_g: bad shape");
+    const int D_log = ilog2_exact(D);
+    require(D_log >= 1 && D_log <= 30, "out_scatter_g: D must be a power of two");
+    require((size_t)N == (size_t)t * (D / 2), "out_scatter_g: N must equal t * D/2");
+    CtxGuard g(ctx);
+    upload(ctx->cwbuf, CW, 8 * (size_t)B, ctx->stream, "H2D cw");
+    ctx->gbuf.ensure(8 * (size_t)N, "out g");
+    GatherArgs ga;
+    ga.leaves = reinterpret_cast<const Blk*>(d_leaves);
+    ga.set_src_stride = 0;
+    ga.CW = ctx->cwbuf.as<uint64_t>();
+    ga.g = ctx->gbuf.as<uint64_t>();
+    ga.nsets = 1;
+    ga.sets_per_party = 1;
+    ga.first_party = party;
+    ga.B = B;
+    ga.t = t;
+    ga.N = N;
+    ga.D_log = D_log;
+    launch_gather_g(ctx->stream, ga);
+    hip_check(hipMemcpyAsync(g_out, ga.g, 8 * (size_t)N, hipMemcpyDeviceToHost, ctx->stream), "D2H g");
+    hip_check(hipStreamSynchronize(ctx->stream), "out_scatter_g");
+  });
+}
+
+int pcgsq_poly_mul_supported(uint64_t prime, int N, int backend, char* reason, size_t reason_len) {
+  std::string why;
+  bool ok = prime == kPrime;
+  if (!ok) why = "only P = 2^62 - 3*2^25 + 1 is supported";
+  if (ok) ok = NttEngine::supported(N, backend, &why);
+  if (!ok && reason && reason_len) {
+    strncpy(reason, why.c_str(), reason_len - 1);
+    reason[reason_len - 1] = 0;
+  }
+  return ok ? 1 : 0;
+}
+
+int pcgsq_poly_mul_u64(pcgsq_ctx* ctx, uint64_t* out, const uint64_t* a, const uint64_t* b,
+                       int batch, int N, uint64_t prime, int backend, double* device_ms) {
+  return guarded([&] {
+    require(ctx && out && a && b, "poly_mul: null argument");
+    require(prime == kPrime, "poly_mul: only P = 2^62 - 3*2^25 + 1 is supported", kErrUnsupported);
+    std::string why;
+    require(NttEngine::supported(N, backend, &why), "poly_mul: " + why, kErrUnsupported);
+    require(batch > 0, "poly_mul: batch must be positive");
+    CtxGuard g(ctx);
+    const size_t bytes = 8 * (size_t)batch * N;
+    hipStream_t st = ctx->stream;
+    upload(ctx->in_a, a, bytes, st, "H2D a");
+    upload(ctx->in_b, b, bytes, st, "H2D b");
+    ctx->in_c.ensure(bytes, "poly_mul out");
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, st), "event");
+    ctx->ntt.poly_mul(st, ctx->in_c.as<uint64_t>(), ctx->in_a.as<uint64_t>(),
+                      ctx->in_b.as<uint64_t>(), batch, N, backend);
+    hip_check(hipEventRecord(e1, st), "event");
+    hip_check(hipMemcpyAsync(out, ctx->in_c.ptr, bytes, hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "poly_mul");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (device_ms) *device_ms = ms;
+  });
+}
+
+int pcgsq_poly_mul_u64_dev(pcgsq_ctx* ctx, uint64_t* dev_out, const uint64_t* dev_a,
+                           const uint64_t* dev_b, int batch, int N, int backend) {
+  return guarded([&] {
+    require(ctx && dev_out && dev_a && dev_b, "poly_mul_dev: null argument");
+    CtxGuard g(ctx);
+    ctx->ntt.poly_mul(ctx->stream, dev_out, dev_a, dev_b, batch, N, backend);
+  });
+}
+
+static int ntt_host(pcgsq_ctx* ctx, uint64_t* a, int batch, int N, int backend, bool inv) {
+  return guarded([&] {
+    require(ctx && a, "ntt: null argument");
+    require(batch > 0, "ntt: batch must be positive");
+    std::string why;
+    require(NttEngine::supported(N, backend, &why), "ntt: " + why, kErrUnsupported);
+    CtxGuard g(ctx);
+    const size_t bytes = 8 * (size_t)batch * N;
+    upload(ctx->in_a, a, bytes, ctx->stream, "H2D ntt");
+    if (inv) ctx->ntt.inverse(ctx->stream, ctx->in_a.as<uint64_t>(), batch, N, backend);
+    else ctx->ntt.forward(ctx->stream, ctx->in_a.as<uint64_t>(), batch, N, backend);
+    hip_check(hipMemcpyAsync(a, ctx->in_a.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "ntt");
+  });
+}
+
+int pcgsq_ntt_forward(pcgsq_ctx* ctx, uint64_t* a, int batch, int N, int backend) {
+  return ntt_host(ctx, a, batch, N, backend, false);
+}
+int pcgsq_ntt_inverse(pcgsq_ctx* ctx, uint64_t* a, int batch, int N, int backend) {
+  return ntt_host(ctx, a, batch, N, backend, true);
+}
+int pcgsq_ntt_forward_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N, int backend) {
+  return guarded([&] {
+    require(ctx && dev_a, "ntt: null argument");
+    CtxGuard g(ctx);
+    ctx->ntt.forward(ctx->stream, dev_a, batch, N, backend);
+  });
+}
+int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N, int backend) {
+  return guarded([&] {
+    require(ctx && dev_a, "ntt: null argument");
+    CtxGuard g(ctx);
+    ctx->ntt.inverse(ctx->stream, dev_a, batch, N, backend);
+  });
+}
+
+int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]) {
+  return guarded([&] {
+    require(key && round_keys, "aes: null argument");
+    aes128_expand_key_host(key, round_keys);
+  });
+}
+
+int pcgsq_aes_tree_round_keys(const uint8_t master[16], int nk, uint8_t* rks) {
+  return guarded([&] {
+    require(master && rks && nk >= 1 && nk <= 15, "aes tree keys: bad argument");
+    aes_tree_round_keys_host(master, nk, rks);
+  });
+}
+
+int pcgsq_aes128_ecb_encrypt(pcgsq_ctx* ctx, const uint8_t* keys, const uint8_t* in, uint8_t* out,
+                             int count) {
+  return guarded([&] {
+    require(ctx && keys && in && out && count > 0, "aes ecb: bad argument");
+    CtxGuard g(ctx);
+    hipStream_t st = ctx->stream;
+    uint32_t t0[256];
+    aes_ttable_host(t0);
+    upload(ctx->t0, t0, sizeof(t0), st, "aes t-table");
+    upload(ctx->in_a, keys, 16 * (size_t)count, st, "aes keys");
+    upload(ctx->in_b, in, 16 * (size_t)count, st, "aes in");
+    ctx->in_c.ensure(16 * (size_t)count, "aes out");
+    hipLaunchKernelGGL(aes_ecb_kernel, dim3((count + 127) / 128), dim3(128), 0, st,
+                       ctx->in_a.as<uint8_t>(), ctx->in_b.as<uint8_t>(), ctx->in_c.as<uint8_t>(),
+                       count, ctx->t0.as<uint32_t>());
+    hip_check(hipGetLastError(), "launch aes_ecb");
+    hip_check(hipMemcpyAsync(out, ctx->in_c.ptr, 16 * (size_t)count, hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "aes ecb");
+  });
+}
+
+}  // extern "C"

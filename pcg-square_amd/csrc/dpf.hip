@@ -1,0 +1,429 @@
+// Half-tree DPF level kernels for gfx950 (see dpf.hpp for the protocol).
+//
+// Frontier layout per level: nodes F[T][j] and hashes H[T][j][k] (k < nk),
+// T = tree, j = node index (MSB digit first, children of j at j*m + c, as in
+// dpf.cpp:554-574).  One thread owns one parent: it rebuilds the m children
+// from the parent's stored hashes and the level's CW, hashes each child with
+// the tree PRG, writes the next frontier and XOR-accumulates the per-column
+// share in registers.  Shares are reduced across the wave with lane shuffles,
+// across the workgroup through LDS, and across workgroups with one 64-bit
+// atomic XOR per (workgroup, word): XOR is associative and commutative, so
+// the result is bit-exact whatever the order.
+#include <string.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+#include "dpf.hpp"
+#include "prg.hpp"
+
+namespace pcgsq {
+
+namespace {
+
+constexpr int kTpb = 256;
+
+struct LevelArgs {
+  const Blk* Fp;     // parents [ntrees][S]
+  const Blk* Hp;     // parents' hashes [ntrees][S][nk]
+  Blk* Fc;           // children [ntrees][S*m]        (hash / store modes)
+  Blk* Hc;           // children's hashes             (hash mode)
+  uint64_t* ct;      // C|tau<<63 per leaf            (convert mode)
+  uint64_t* part;    // partial sums [ntrees][nparts][2] (convert mode)
+  Blk* cw;           // [levels][ntrees][nk] shares; level `level` accumulates
+  const Blk* peer;   // [levels][npeer][nk]
+  const uint32_t* aes_t0;
+  const uint8_t* aes_rks;
+  int peer_shift, npeer;
+  int ntrees, S_log, level, iters, nparts;
+};
+
+enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
+
+template <int W>
+__device__ __forceinline__ void load_cw(const LevelArgs& a, int T, Blk* CW) {
+  constexpr int NK = (1 << W) - 1;
+  const Blk* mine = a.cw + ((size_t)(a.level - 1) * a.ntrees + T) * NK;
+  const int pt = (T + a.peer_shift) % a.npeer;
+  const Blk* peer = a.peer + ((size_t)(a.level - 1) * a.npeer + pt) * NK;
+  Blk fold{0, 0};
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    CW[k] = bxor(mine[k], peer[k]);
+    fold = bxor(fold, CW[k]);
+  }
+  CW[NK] = fold;  // CW_{m-1} = XOR_k CW_k
+}
+
+template <int W, int PRG, int MODE>
+__global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
+  constexpr int M = 1 << W, NK = M - 1;
+  __shared__ typename std::conditional<PRG == kPrgAes, AesLds, int>::type aes_lds_storage;
+  __shared__ uint64_t red[4 * 2 * NK + 8];
+  const AesLds* lds = nullptr;
+  if constexpr (PRG == kPrgAes) {
+    aes_stage(&aes_lds_storage, a.aes_t0, a.aes_rks, NK);
+    lds = &aes_lds_storage;
+  }
+  const size_t S = (size_t)1 << a.S_log;
+  const size_t total = (size_t)a.ntrees << a.S_log;
+  const size_t base = (size_t)blockIdx.x * a.iters * kTpb;
+
+  uint64_t acc[2 * NK];
+#pragma unroll
+  for (int i = 0; i < 2 * NK; ++i) acc[i] = 0;
+  uint64_t accC = 0, accT = 0;
+
+  for (int it = 0; it < a.iters; ++it) {
+    const size_t f = base + (size_t)it * kTpb + threadIdx.x;
+    if (f >= total) break;
+    const int T = (int)(f >> a.S_log);
+    Blk CW[NK + 1];
+    load_cw<W>(a, T, CW);
+    const Blk parent = a.Fp[f];
+    Blk h[NK];
+    Blk hx{0, 0};
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      h[k] = a.Hp[f * NK + k];
+      hx = bxor(hx, h[k]);
+    }
+    const bool t = (parent.lo & 1ULL) != 0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      Blk node = (c < NK) ? h[c < NK ? c : 0] : bxor(hx, parent);
+      if (t) node = bxor(node, CW[c]);
+      const size_t idx = f * M + c;
+      if constexpr (MODE == kModeHash) {
+        Blk hh[NK];
+        tree_hash<PRG, NK>(node, hh, lds);
+        a.Fc[idx] = node;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          a.Hc[idx * NK + k] = hh[k];
+          acc[2 * k] ^= hh[k].lo;
+          acc[2 * k + 1] ^= hh[k].hi;
+        }
+      } else if constexpr (MODE == kModeStore) {
+        a.Fc[idx] = node;
+      } else {
+        const uint64_t C = freduce64(out_hash_lo64(node));
+        const uint64_t tau = node.lo & 1ULL;
+        a.ct[idx] = C | (tau << 63);
+        accC = fadd(accC, C);
+        accT += tau;
+      }
+    }
+  }
+
+  int first = 0;
+  if constexpr (MODE == kModeHash) {
+    if (segment_reduce<2 * NK>(acc, a.S_log, [](int, uint64_t x, uint64_t y) { return x ^ y; },
+                               red, &first)) {
+      const int T = (int)((base + (size_t)first) >> a.S_log);
+      if (T < a.ntrees) {
+        Blk* dst = a.cw + ((size_t)a.level * a.ntrees + T) * NK;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          atomicXor((unsigned long long*)&dst[k].lo, (unsigned long long)acc[2 * k]);
+          atomicXor((unsigned long long*)&dst[k].hi, (unsigned long long)acc[2 * k + 1]);
+        }
+      }
+    }
+  } else if constexpr (MODE == kModeConvert) {
+    // word 0: sum of C mod P; word 1: count of tau (plain)
+    uint64_t v[2] = {accC, accT};
+    if (segment_reduce<2>(v, a.S_log,
+                          [](int i, uint64_t x, uint64_t y) { return i == 0 ? fadd(x, y) : x + y; },
+                          red, &first)) {
+      const size_t f = base + (size_t)first;
+      const int T = (int)(f >> a.S_log);
+      const size_t chunk = (size_t)a.iters * kTpb;
+      const size_t pidx = (a.S_log >= 8) ? ((f & (S - 1)) / chunk) : 0;
+      if (T < a.ntrees) {
+        uint64_t* dst = a.part + ((size_t)T * a.nparts + pidx) * 2;
+        dst[0] = v[0];
+        dst[1] = v[1];
+      }
+    }
+  }
+}
+
+// Level 0: the frontier is the root.  Also moves DltSft from the reference
+// layout [T][levels][nk] into the level-major share array [levels][T][nk]
+// and seeds level 0 with H(root) (dpf.cpp:502, :522-532).
+template <int W, int PRG>
+__global__ __launch_bounds__(kTpb) void dpf_root_kernel(const Blk* __restrict__ roots,
+                                                        const Blk* __restrict__ dlt,
+                                                        const Blk* __restrict__ roots1,
+                                                        const Blk* __restrict__ dlt1,
+                                                        Blk* __restrict__ F0, Blk* __restrict__ H0,
+                                                        Blk* __restrict__ cw, int ntrees,
+                                                        int levels, const uint32_t* aes_t0,
+                                                        const uint8_t* aes_rks) {
+  constexpr int NK = (1 << W) - 1;
+  __shared__ typename std::conditional<PRG == kPrgAes, AesLds, int>::type aes_lds_storage;
+  const AesLds* lds = nullptr;
+  if constexpr (PRG == kPrgAes) {
+    aes_stage(&aes_lds_storage, aes_t0, aes_rks, NK);
+    lds = &aes_lds_storage;
+  }
+  const int T = blockIdx.x * kTpb + threadIdx.x;
+  if (T >= ntrees) return;
+  const int half = ntrees / 2;
+  const bool second = roots1 != nullptr && T >= half;
+  const int Ti = second ? T - half : T;
+  const Blk* dsrc = second ? dlt1 : dlt;
+  for (int i = 0; i < levels; ++i)
+    for (int k = 0; k < NK; ++k)
+      cw[((size_t)i * ntrees + T) * NK + k] = dsrc[((size_t)Ti * levels + i) * NK + k];
+  const Blk r = second ? roots1[Ti] : roots[T];
+  F0[T] = r;
+  Blk h[NK];
+  tree_hash<PRG, NK>(r, h, lds);
+  for (int k = 0; k < NK; ++k) {
+    H0[(size_t)T * NK + k] = h[k];
+    Blk* d = &cw[(size_t)T * NK + k];
+    *d = bxor(*d, h[k]);
+  }
+}
+
+template <int W, int PRG>
+void launch_root(hipStream_t st, const DpfRun& r, Blk* F0, Blk* H0, Blk* cw) {
+  const int levels = r.n / r.w;
+  const int grid = (r.ntrees + kTpb - 1) / kTpb;
+  hipLaunchKernelGGL((dpf_root_kernel<W, PRG>), dim3(grid), dim3(kTpb), 0, st, r.roots, r.dltsft,
+                     r.roots1, r.dltsft1, F0, H0, cw, r.ntrees, levels, r.aes_t0, r.aes_rks);
+  hip_check(hipGetLastError(), "launch dpf_root_kernel");
+}
+
+template <int W, int PRG, int MODE>
+void launch_level(hipStream_t st, LevelArgs a) {
+  const size_t total = (size_t)a.ntrees << a.S_log;
+  const size_t tiles = (total + kTpb - 1) / kTpb;
+  const size_t grid = (tiles + a.iters - 1) / a.iters;
+  require(grid < (1ULL << 31), "dpf: launch too large");
+  hipLaunchKernelGGL((dpf_level_kernel<W, PRG, MODE>), dim3((unsigned)grid), dim3(kTpb), 0, st, a);
+  hip_check(hipGetLastError(), "launch dpf_level_kernel");
+}
+
+template <int W, int PRG>
+void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
+  constexpr int M = 1 << W, NK = M - 1;
+  const int n = r.n, levels = n / W;
+  const size_t maxS = (size_t)1 << (W * (levels - 1));  // frontier at the last parent level
+  // The widest stored level is the last parent level (maxS nodes per tree),
+  // each node with its NK hashes.
+  const size_t need = (size_t)r.ntrees * maxS * (1 + NK) * sizeof(Blk);
+  s.fh[0].ensure(need, "hipMalloc dpf frontier");
+  s.fh[1].ensure(need, "hipMalloc dpf frontier");
+  const size_t cw_bytes = (size_t)levels * r.ntrees * NK * sizeof(Blk);
+  s.cw.ensure(cw_bytes, "hipMalloc dpf cw");
+  if (r.host_ex) s.peer.ensure(cw_bytes, "hipMalloc dpf peer");
+
+  auto frontier = [&](int which, size_t S, Blk*& F, Blk*& H) {
+    F = s.fh[which].as<Blk>();
+    H = F + (size_t)r.ntrees * S;
+  };
+
+  Blk *F, *H;
+  frontier(0, 1, F, H);
+  launch_root<W, PRG>(st, r, F, H, s.cw.as<Blk>());
+
+  std::vector<Blk> h_mine, h_peer;
+  const size_t cw_level = (size_t)r.ntrees * NK;
+  if (r.host_ex) {
+    h_mine.resize(cw_level);
+    h_peer.resize(cw_level);
+  }
+
+  int cur = 0;
+  for (int i = 1; i <= levels; ++i) {
+    // share of level i-1 is complete: exchange it
+    if (r.host_ex) {
+      Blk* dmine = s.cw.as<Blk>() + (size_t)(i - 1) * cw_level;
+      Blk* dpeer = s.peer.as<Blk>() + (size_t)(i - 1) * cw_level;
+      hip_check(hipMemcpyAsync(h_mine.data(), dmine, cw_level * sizeof(Blk),
+                               hipMemcpyDeviceToHost, st), "D2H dpf cw shares");
+      hip_check(hipStreamSynchronize(st), "sync dpf cw shares");
+      const int rc = r.host_ex(r.host_user, i - 1, h_mine.data(), h_peer.data(), cw_level);
+      if (rc != 0) throw Error(kErrCallback, "dpf exchange callback failed at level " +
+                                                 std::to_string(i - 1));
+      hip_check(hipMemcpyAsync(dpeer, h_peer.data(), cw_level * sizeof(Blk),
+                               hipMemcpyHostToDevice, st), "H2D dpf peer shares");
+    }
+    const int S_log = W * (i - 1);
+    const size_t S = (size_t)1 << S_log;
+    LevelArgs a{};
+    frontier(cur, S, F, H);
+    a.Fp = F;
+    a.Hp = H;
+    a.cw = s.cw.as<Blk>();
+    if (r.host_ex) {
+      a.peer = s.peer.as<Blk>();
+      a.npeer = r.ntrees;
+      a.peer_shift = 0;
+    } else {
+      a.peer = s.cw.as<Blk>();
+      a.npeer = r.ntrees;
+      a.peer_shift = r.ntrees / 2;
+    }
+    a.aes_t0 = r.aes_t0;
+    a.aes_rks = r.aes_rks;
+    a.ntrees = r.ntrees;
+    a.S_log = S_log;
+    a.level = i;
+    a.iters = (S >= (size_t)kTpb) ? (int)std::min<size_t>(S / kTpb, (size_t)r.iters_max) : 1;
+    if (i < levels) {
+      Blk *Fn, *Hn;
+      frontier(cur ^ 1, S * M, Fn, Hn);
+      a.Fc = Fn;
+      a.Hc = Hn;
+      launch_level<W, PRG, kModeHash>(st, a);
+      cur ^= 1;
+    } else if (r.mode == kLeafStore) {
+      a.Fc = r.leaves;
+      launch_level<W, PRG, kModeStore>(st, a);
+    } else {
+      a.ct = r.ct;
+      a.part = r.partials;
+      a.nparts = dpf_leaf_nparts(n, W, r.iters_max);
+      launch_level<W, PRG, kModeConvert>(st, a);
+    }
+  }
+  if (r.cw_out)
+    hip_check(hipMemcpyAsync(r.cw_out, s.cw.as<Blk>(), cw_bytes, hipMemcpyDeviceToDevice, st),
+              "copy dpf cw");
+}
+
+}  // namespace
+
+int dpf_leaf_nparts(int n, int w, int iters_max) {
+  const int levels = n / w;
+  const size_t S = (size_t)1 << (w * (levels - 1));
+  if (S < (size_t)kTpb) return 1;
+  const size_t iters = std::min<size_t>(S / kTpb, (size_t)iters_max);
+  return (int)(S / (iters * kTpb));
+}
+
+void dpf_expand(hipStream_t st, const DpfRun& r, DpfScratch& s) {
+  require(r.w >= 1 && r.w <= 4, "dpf: w must be in [1,4]");
+  require(r.n >= r.w && r.n % r.w == 0, "dpf: w must divide n");
+  require(r.n <= 30, "dpf: n too large");
+  require(r.ntrees > 0, "dpf: no instances");
+  require(r.prg == kPrgAes || r.prg == kPrgChaCha8, "dpf: unknown PRG");
+  require(r.prg != kPrgAes || (r.aes_t0 && r.aes_rks), "dpf: AES PRG requires round keys");
+  require(r.host_ex || r.ntrees % 2 == 0, "dpf: in-device exchange pairs trees T and T+ntrees/2");
+  require(!r.roots1 || (r.dltsft1 && r.ntrees % 2 == 0), "dpf: second half needs roots1 and dltsft1");
+  require(r.mode != kLeafStore || r.leaves, "dpf: leaves output missing");
+  require(r.mode != kLeafConvert || (r.ct && r.partials), "dpf: convert outputs missing");
+#define PCGSQ_DISPATCH(W)                                                        \
+  if (r.w == W) {                                                                \
+    if (r.prg == kPrgChaCha8) expand_t<W, kPrgChaCha8>(st, r, s);                \
+    else expand_t<W, kPrgAes>(st, r, s);                                         \
+    return;                                                                      \
+  }
+  PCGSQ_DISPATCH(1)
+  PCGSQ_DISPATCH(2)
+  PCGSQ_DISPATCH(3)
+  PCGSQ_DISPATCH(4)
+#undef PCGSQ_DISPATCH
+}
+
+// ---- host AES (FIPS-197) ------------------------------------------------------
+
+namespace {
+uint8_t gmul(uint8_t a, uint8_t b) {
+  uint8_t r = 0;
+  while (b) {
+    if (b & 1) r ^= a;
+    a = (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1b : 0));
+    b >>= 1;
+  }
+  return r;
+}
+// S-box from the GF(2^8) inverse and the affine map (FIPS-197 Sec. 5.1.1).
+struct SBox {
+  uint8_t s[256];
+  SBox() {
+    for (int x = 0; x < 256; ++x) {
+      uint8_t inv = 0;
+      for (int y = 1; y < 256 && x; ++y)
+        if (gmul((uint8_t)x, (uint8_t)y) == 1) { inv = (uint8_t)y; break; }
+      uint8_t v = inv;
+      for (int r = 1; r <= 4; ++r) v ^= (uint8_t)((inv << r) | (inv >> (8 - r)));
+      s[x] = v ^ 0x63;
+    }
+  }
+};
+const SBox& sbox() {
+  static const SBox sb;
+  return sb;
+}
+}  // namespace
+
+void aes128_expand_key_host(const uint8_t key[16], uint8_t rk[176]) {
+  const uint8_t* S = sbox().s;
+  uint8_t rcon = 1;
+  memcpy(rk, key, 16);
+  for (int i = 4; i < 44; ++i) {
+    uint8_t t[4];
+    memcpy(t, rk + 4 * (i - 1), 4);
+    if (i % 4 == 0) {
+      const uint8_t t0 = t[0];
+      t[0] = (uint8_t)(S[t[1]] ^ rcon);
+      t[1] = S[t[2]];
+      t[2] = S[t[3]];
+      t[3] = S[t0];
+      rcon = gmul(rcon, 2);
+    }
+    for (int k = 0; k < 4; ++k) rk[4 * i + k] = rk[4 * (i - 4) + k] ^ t[k];
+  }
+}
+
+void aes_ttable_host(uint32_t t0[256]) {
+  const uint8_t* S = sbox().s;
+  for (int x = 0; x < 256; ++x) {
+    const uint8_t s = S[x];
+    t0[x] = (uint32_t)gmul(s, 2) | ((uint32_t)s << 8) | ((uint32_t)s << 16) |
+            ((uint32_t)gmul(s, 3) << 24);
+  }
+}
+
+void aes128_encrypt_host(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]) {
+  const uint8_t* S = sbox().s;
+  uint8_t s[16], t[16];
+  for (int i = 0; i < 16; ++i) s[i] = in[i] ^ rk[i];
+  for (int r = 1; r <= 10; ++r) {
+    for (int c = 0; c < 4; ++c)
+      for (int j = 0; j < 4; ++j) t[4 * c + j] = S[s[4 * ((c + j) & 3) + j]];
+    if (r < 10) {
+      for (int c = 0; c < 4; ++c) {
+        const uint8_t a0 = t[4 * c], a1 = t[4 * c + 1], a2 = t[4 * c + 2], a3 = t[4 * c + 3];
+        s[4 * c + 0] = gmul(a0, 2) ^ gmul(a1, 3) ^ a2 ^ a3;
+        s[4 * c + 1] = a0 ^ gmul(a1, 2) ^ gmul(a2, 3) ^ a3;
+        s[4 * c + 2] = a0 ^ a1 ^ gmul(a2, 2) ^ gmul(a3, 3);
+        s[4 * c + 3] = gmul(a0, 3) ^ a1 ^ a2 ^ gmul(a3, 2);
+      }
+    } else {
+      memcpy(s, t, 16);
+    }
+    for (int i = 0; i < 16; ++i) s[i] ^= rk[16 * r + i];
+  }
+  memcpy(out, s, 16);
+}
+
+void aes_tree_round_keys_host(const uint8_t master[16], int nk, uint8_t* rks) {
+  uint8_t mrk[176];
+  aes128_expand_key_host(master, mrk);
+  for (int k = 0; k < nk; ++k) {
+    uint8_t tweak[16] = {0}, c[16], sub[16];
+    for (int b = 0; b < 8; ++b) tweak[b] = (uint8_t)((uint64_t)k >> (8 * b));
+    aes128_encrypt_host(mrk, tweak, c);
+    for (int b = 0; b < 16; ++b) sub[b] = c[b] ^ tweak[b];
+    aes128_expand_key_host(sub, rks + (size_t)k * 176);
+  }
+}
+
+}  // namespace pcgsq

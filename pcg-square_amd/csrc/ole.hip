@@ -1,0 +1,444 @@
+// PCG-OLE expansion plan: both parties on one MI355X.
+//
+// Follows pcg_ole::PCG_OLE<P>::gen_and_expand
+// (CPU_baseline/pcg_ole_2pc/pcg_ole_impl.h:46-490):
+//   1. noise and preprocessing (here: the seeded dealer below),
+//   2. for every DPF block (ii, jj): B = t^2 half-tree DPFs of domain
+//      D = 2N/t for both parties (fused protocol, in-device exchange), the
+//      output layer (ChaCha8 out-hash fused into the last tree level,
+//      Beaver opens, folded scatter into g_ij),
+//   3. step 4: x = sum_i a_i e_i, z = sum_ij (a_i a_j) fold(g_ij).
+// Step 4 is evaluated in the NTT domain: with A_i = NTT(a_i),
+// X = sum_i A_i .* NTT(e_i) and Z = sum_ij A_i .* A_j .* NTT(g_ij), then one
+// inverse each.  Products in R_p are unique, so this is bit-identical to the
+// reference's c + 2c^2 separate poly-muls while computing 2c + 2c^2 forward
+// and 4 inverse transforms per expansion (both parties).
+//
+// Dealer spec (shared with the oracle, oracle/orc_pcg.c, which must agree):
+//   U(seed, dom, a, b) = sm(sm(sm(seed ^ dom*K) ^ a) ^ b), sm = splitmix64
+//   finalizer, K = 0xD6E8FEB86659FD93.
+//   pos[s][i][k] = k*bin + U(seed,1,s,i*t+k) % bin;
+//   pay[s][i][k] = U(seed,2,s,i*t+k) % (P-1) + 1;
+//   instance g = blk*B + idx (blk = ii*c + jj, idx = kk*t + ll);
+//   delta_s = (U(seed,3,s,2g), U(seed,3,s,2g+1)), lsb forced 1 (s=0) / 0 (s=1);
+//   root_s = delta_s ^ W, W = (0x0123456789abcdef, 0xdeadbeefcafe0001);
+//   alpha = pos0[ii][kk] % bin + pos1[jj][ll] % bin;
+//   dltsft0[i][k] = (U(seed,4,g,2e), U(seed,4,g,2e+1)), e = i*nk + k;
+//   dltsft1 = dltsft0 ^ ([digit_i(alpha) = k] ? Delta : 0);
+//   beta0 = U(seed,5,g,0) % P, beta1 = pay0*pay1 - beta0;
+//   r_s = U(seed,6,g,s) % P, s_s = U(seed,7,g,s) % P,
+//   rs0 = U(seed,8,g,0) % P, rs1 = (r0+r1)(s0+s1) - rs0;
+//   public_a[i][j] = U(pub_seed,9,i,j) % P.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "context.hpp"
+#include "pcgsq.h"
+
+namespace pcgsq {
+
+namespace {
+
+uint64_t sm64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
+
+__global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t* pay, int c, int t,
+                               int N) {
+  // e: [2][c][N] zero-filled beforehand; pos/pay: [2][c][t]
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * c * t) return;
+  const int poly = i / t;
+  e[(size_t)poly * N + pos[i]] = pay[i];
+}
+
+// X_s = sum_i A_i E_{s,i};  Z_s = sum_i A_i sum_j A_j G_{s,ij}  (NTT domain)
+__global__ void step4_dot_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ G,
+                                 const uint64_t* __restrict__ E, uint64_t* __restrict__ out,
+                                 int c, int N) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (q >= (size_t)N) return;
+  uint64_t x = 0, z = 0;
+  for (int i = 0; i < c; ++i) {
+    const uint64_t ai = A[(size_t)i * N + q];
+    x = fadd(x, fmul(ai, E[((size_t)s * c + i) * N + q]));
+    uint64_t inner = 0;
+    for (int j = 0; j < c; ++j) {
+      const uint64_t aj = A[(size_t)j * N + q];
+      inner = fadd(inner, fmul(aj, G[(((size_t)s * c + i) * c + j) * N + q]));
+    }
+    z = fadd(z, fmul(ai, inner));
+  }
+  out[((size_t)2 * s) * N + q] = x;
+  out[((size_t)2 * s + 1) * N + q] = z;
+}
+
+__global__ void zero_kernel(uint64_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int ilog(uint64_t x) { return ilog2_exact(x); }
+
+}  // namespace
+
+uint64_t dealer_u64(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
+  return sm64(sm64(sm64(seed ^ ((uint64_t)domain * 0xD6E8FEB86659FD93ULL)) ^ a) ^ b);
+}
+
+struct OleDims {
+  int N, c, t, w, prg;
+  int B, bin, D, D_log, levels, nk, nblk, ninst, L;
+};
+
+static OleDims make_dims(const pcgsq_params& p) {
+  OleDims d{};
+  require(p.N > 0 && (p.N & (p.N - 1)) == 0, "ole: N must be a power of two");
+  require(p.t > 0 && p.N % p.t == 0, "ole: t must divide N");
+  require(p.c >= 1 && p.c <= 16, "ole: c out of range");
+  require(p.prg == PCGSQ_PRG_AES || p.prg == PCGSQ_PRG_CHACHA8, "ole: unknown PRG");
+  d.N = p.N; d.c = p.c; d.t = p.t; d.w = p.w; d.prg = p.prg;
+  d.L = ilog((uint64_t)p.N);
+  require(d.L >= 2 && d.L <= 24, "ole: N must be in [4, 2^24] (2N | P-1)", kErrUnsupported);
+  d.B = p.t * p.t;
+  d.bin = p.N / p.t;
+  d.D = 2 * d.bin;
+  d.D_log = ilog((uint64_t)d.D);
+  require(d.D_log > 0, "ole: 2N/t must be a power of two");
+  require(p.w >= 1 && p.w <= 4 && d.D_log % p.w == 0, "ole: w must be in [1,4] and divide log2(2N/t)");
+  d.levels = d.D_log / p.w;
+  d.nk = (1 << p.w) - 1;
+  d.nblk = p.c * p.c;
+  d.ninst = d.nblk * d.B;
+  return d;
+}
+
+}  // namespace pcgsq
+
+using namespace pcgsq;
+
+struct pcgsq_ole_plan {
+  pcgsq_ctx* ctx = nullptr;
+  OleDims d{};
+  uint64_t pub_seed = 0;
+  int group_blocks = 1;
+  bool dealt = false;
+  DpfScratch dpf;
+  // device-resident inputs
+  DevBuf roots, dlt, bt, noise;
+  DevBuf aes_t0, aes_rks;
+  // public NTT(a_i)
+  DevBuf ahat;
+  // working set
+  DevBuf ct, part, cw, fw, out;
+};
+
+namespace {
+
+size_t plan_group_bytes(const OleDims& d, int gb) {
+  const size_t trees = (size_t)2 * gb * d.B;
+  const size_t maxS = (size_t)1 << (d.w * (d.levels - 1));
+  const size_t fh = trees * maxS * (1 + d.nk) * sizeof(Blk) * 2;
+  const size_t ct = trees * (size_t)d.D * sizeof(uint64_t);
+  return fh + ct;
+}
+
+void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
+  const OleDims& d = pl->d;
+  if (budget == 0) budget = (size_t)96 << 30;
+  int gb = d.nblk;
+  while (gb > 1 && plan_group_bytes(d, gb) > budget) --gb;
+  // equal groups keep every launch the same shape
+  while (d.nblk % gb) --gb;
+  require(plan_group_bytes(d, gb) <= budget || gb == 1, "ole: HBM budget too small", kErrNoMemory);
+  pl->group_blocks = gb;
+  const size_t trees_all = (size_t)2 * d.ninst;
+  pl->roots.ensure(trees_all * sizeof(Blk), "ole roots");
+  pl->dlt.ensure(trees_all * d.levels * d.nk * sizeof(Blk), "ole dltsft");
+  pl->bt.ensure((size_t)4 * 2 * d.ninst * sizeof(uint64_t), "ole beaver");
+  pl->noise.ensure((size_t)2 * 2 * d.c * d.t * sizeof(uint64_t), "ole noise");
+  pl->ahat.ensure((size_t)d.c * d.N * sizeof(uint64_t), "ole ahat");
+  const size_t gtrees = (size_t)2 * gb * d.B;
+  pl->ct.ensure(gtrees * d.D * sizeof(uint64_t), "ole ct");
+  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  pl->part.ensure(gtrees * nparts * 2 * sizeof(uint64_t), "ole partials");
+  pl->cw.ensure((size_t)d.ninst * sizeof(uint64_t), "ole cw");
+  pl->fw.ensure(((size_t)2 * d.nblk + 2 * d.c) * d.N * sizeof(uint64_t), "ole ntt buffers");
+  pl->out.ensure((size_t)4 * d.N * sizeof(uint64_t), "ole outputs");
+  if (d.prg == PCGSQ_PRG_AES) {
+    uint32_t t0[256];
+    aes_ttable_host(t0);
+    std::vector<uint8_t> rks((size_t)d.nk * 176);
+    static const uint8_t S[16] = {0x2b, 0x7e, 0x15, 0x16, 0x28, 0xae, 0xd2, 0xa6,
+                                  0xab, 0xf7, 0x15, 0x88, 0x09, 0xcf, 0x4f, 0x3c};
+    aes_tree_round_keys_host(S, d.nk, rks.data());  // pcg_ole_impl.h:89-92
+    pl->aes_t0.ensure(sizeof(t0), "ole aes");
+    pl->aes_rks.ensure(rks.size(), "ole aes");
+    hip_check(hipMemcpy(pl->aes_t0.ptr, t0, sizeof(t0), hipMemcpyHostToDevice), "ole aes");
+    hip_check(hipMemcpy(pl->aes_rks.ptr, rks.data(), rks.size(), hipMemcpyHostToDevice), "ole aes");
+  }
+  // ahat: a_0 = 1, a_i = public_a[i-1]
+  std::vector<uint64_t> a((size_t)d.c * d.N, 0);
+  a[0] = 1;
+  for (int i = 1; i < d.c; ++i)
+    for (int j = 0; j < d.N; ++j)
+      a[(size_t)i * d.N + j] = dealer_u64(pl->pub_seed, 9, (uint64_t)(i - 1), (uint64_t)j) % kPrime;
+  hip_check(hipMemcpy(pl->ahat.ptr, a.data(), a.size() * 8, hipMemcpyHostToDevice), "ole ahat");
+  pl->ctx->ntt.forward(pl->ctx->stream, pl->ahat.as<uint64_t>(), d.c, d.N, kNttFourStep);
+  hip_check(hipStreamSynchronize(pl->ctx->stream), "ole ahat");
+}
+
+void plan_deal(pcgsq_ole_plan* pl, uint64_t seed) {
+  const OleDims& d = pl->d;
+  const uint64_t bin = (uint64_t)d.bin;
+  const int nk = d.nk, levels = d.levels;
+  std::vector<uint64_t> noise((size_t)2 * 2 * d.c * d.t);   // [pos|pay][s][c][t]
+  uint64_t* pos = noise.data();
+  uint64_t* pay = noise.data() + (size_t)2 * d.c * d.t;
+  for (int s = 0; s < 2; ++s)
+    for (int i = 0; i < d.c; ++i)
+      for (int k = 0; k < d.t; ++k) {
+        const size_t o = ((size_t)s * d.c + i) * d.t + k;
+        pos[o] = (uint64_t)k * bin + dealer_u64(seed, 1, (uint64_t)s, (uint64_t)(i * d.t + k)) % bin;
+        pay[o] = dealer_u64(seed, 2, (uint64_t)s, (uint64_t)(i * d.t + k)) % (kPrime - 1) + 1;
+      }
+  const size_t ni = (size_t)d.ninst;
+  std::vector<Blk> roots(2 * ni);
+  std::vector<Blk> dlt(2 * ni * levels * nk);
+  std::vector<uint64_t> bt((size_t)4 * 2 * ni);  // [beta|r|s|rs][party][inst]
+  for (int ii = 0; ii < d.c; ++ii)
+    for (int jj = 0; jj < d.c; ++jj) {
+      const int blk = ii * d.c + jj;
+      for (int idx = 0; idx < d.B; ++idx) {
+        const int kk = idx / d.t, ll = idx % d.t;
+        const uint64_t g = (uint64_t)blk * d.B + idx;
+        const uint64_t p0 = pos[(size_t)ii * d.t + kk], p1 = pos[((size_t)d.c + jj) * d.t + ll];
+        const uint64_t alpha = p0 % bin + p1 % bin;
+        Blk d0{dealer_u64(seed, 3, 0, 2 * g), dealer_u64(seed, 3, 0, 2 * g + 1)};
+        Blk d1{dealer_u64(seed, 3, 1, 2 * g), dealer_u64(seed, 3, 1, 2 * g + 1)};
+        d0.lo |= 1ULL;   // dpf.cpp:78-82
+        d1.lo &= ~1ULL;
+        const Blk Delta{d0.lo ^ d1.lo, d0.hi ^ d1.hi};
+        roots[g] = Blk{d0.lo ^ kWlo, d0.hi ^ kWhi};
+        roots[ni + g] = Blk{d1.lo ^ kWlo, d1.hi ^ kWhi};
+        for (int i = 0; i < levels; ++i) {
+          const int digit = (int)((alpha >> (d.D_log - d.w * (i + 1))) & (uint64_t)((1 << d.w) - 1));
+          for (int k = 0; k < nk; ++k) {
+            const uint64_t e = (uint64_t)(i * nk + k);
+            Blk s0{dealer_u64(seed, 4, g, 2 * e), dealer_u64(seed, 4, g, 2 * e + 1)};
+            dlt[(g * levels + i) * nk + k] = s0;
+            if (digit == k) { s0.lo ^= Delta.lo; s0.hi ^= Delta.hi; }
+            dlt[((ni + g) * levels + i) * nk + k] = s0;
+          }
+        }
+        const uint64_t pay0 = pay[(size_t)ii * d.t + kk], pay1 = pay[((size_t)d.c + jj) * d.t + ll];
+        const uint64_t beta = hmul(pay0, pay1);
+        uint64_t* BETA = bt.data();
+        uint64_t* R = bt.data() + 2 * ni;
+        uint64_t* S = bt.data() + 4 * ni;
+        uint64_t* RS = bt.data() + 6 * ni;
+        BETA[g] = dealer_u64(seed, 5, g, 0) % kPrime;
+        BETA[ni + g] = (beta + kPrime - BETA[g]) % kPrime;
+        R[g] = dealer_u64(seed, 6, g, 0) % kPrime;
+        R[ni + g] = dealer_u64(seed, 6, g, 1) % kPrime;
+        S[g] = dealer_u64(seed, 7, g, 0) % kPrime;
+        S[ni + g] = dealer_u64(seed, 7, g, 1) % kPrime;
+        const uint64_t rs = hmul((R[g] + R[ni + g]) % kPrime, (S[g] + S[ni + g]) % kPrime);
+        RS[g] = dealer_u64(seed, 8, g, 0) % kPrime;
+        RS[ni + g] = (rs + kPrime - RS[g]) % kPrime;
+      }
+    }
+  hipStream_t st = pl->ctx->stream;
+  hip_check(hipMemcpyAsync(pl->noise.ptr, noise.data(), noise.size() * 8, hipMemcpyHostToDevice, st), "deal");
+  hip_check(hipMemcpyAsync(pl->roots.ptr, roots.data(), roots.size() * sizeof(Blk), hipMemcpyHostToDevice, st), "deal");
+  hip_check(hipMemcpyAsync(pl->dlt.ptr, dlt.data(), dlt.size() * sizeof(Blk), hipMemcpyHostToDevice, st), "deal");
+  hip_check(hipMemcpyAsync(pl->bt.ptr, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, st), "deal");
+  hip_check(hipStreamSynchronize(st), "deal");
+  pl->dealt = true;
+}
+
+// DPF blocks + output layer: leaves g_ij (folded) for both parties in fw.
+void plan_run_blocks(pcgsq_ole_plan* pl) {
+  require(pl->dealt, "ole: run before deal");
+  const OleDims& d = pl->d;
+  hipStream_t st = pl->ctx->stream;
+  const int gb = pl->group_blocks;
+  const size_t ni = (size_t)d.ninst;
+  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  uint64_t* G = pl->fw.as<uint64_t>();                                  // [2][nblk][N]
+  uint64_t* E = G + (size_t)2 * d.nblk * d.N;                          // [2][c][N]
+  const uint64_t* BT = pl->bt.as<uint64_t>();
+  for (int b0 = 0; b0 < d.nblk; b0 += gb) {
+    const size_t i0 = (size_t)b0 * d.B;   // first instance of the group
+    DpfRun r;
+    r.ntrees = 2 * gb * d.B;
+    r.n = d.D_log;
+    r.w = d.w;
+    r.prg = d.prg;
+    r.roots = pl->roots.as<Blk>() + i0;
+    r.roots1 = pl->roots.as<Blk>() + ni + i0;
+    r.dltsft = pl->dlt.as<Blk>() + i0 * d.levels * d.nk;
+    r.dltsft1 = pl->dlt.as<Blk>() + (ni + i0) * d.levels * d.nk;
+    r.aes_t0 = pl->aes_t0.as<uint32_t>();
+    r.aes_rks = pl->aes_rks.as<uint8_t>();
+    r.mode = kLeafConvert;
+    r.ct = pl->ct.as<uint64_t>();
+    r.partials = pl->part.as<uint64_t>();
+    dpf_expand(st, r, pl->dpf);
+    launch_beaver_cw(st, pl->part.as<uint64_t>(), gb * d.B, nparts, BT + i0, BT + 2 * ni + i0,
+                     BT + 4 * ni + i0, BT + 6 * ni + i0, ni, pl->cw.as<uint64_t>() + i0);
+    GatherArgs ga;
+    ga.ct = pl->ct.as<uint64_t>();
+    ga.set_src_stride = (size_t)d.B * d.D;
+    ga.CW = pl->cw.as<uint64_t>() + i0;
+    ga.cw_stride = (size_t)d.B;
+    ga.g = G + (size_t)b0 * d.N;
+    ga.g_party_stride = (size_t)d.nblk * d.N;
+    ga.nsets = 2 * gb;
+    ga.sets_per_party = gb;
+    ga.B = d.B;
+    ga.t = d.t;
+    ga.N = d.N;
+    ga.D_log = d.D_log;
+    launch_gather_g(st, ga);
+  }
+}
+
+void plan_step4(pcgsq_ole_plan* pl) {
+  const OleDims& d = pl->d;
+  hipStream_t st = pl->ctx->stream;
+  uint64_t* G = pl->fw.as<uint64_t>();
+  uint64_t* E = G + (size_t)2 * d.nblk * d.N;
+  // step 4 in the NTT domain (a fill kernel, not a memset node, so the
+  // sequence can be captured into a graph)
+  const size_t ne_words = (size_t)2 * d.c * d.N;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ne_words + 255) / 256)), dim3(256), 0, st, E,
+                     ne_words);
+  const int ne = 2 * d.c * d.t;
+  hipLaunchKernelGGL(build_e_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, E,
+                     pl->noise.as<uint64_t>(), pl->noise.as<uint64_t>() + ne, d.c, d.t, d.N);
+  hip_check(hipGetLastError(), "launch build_e");
+  pl->ctx->ntt.forward(st, G, 2 * d.nblk + 2 * d.c, d.N, kNttFourStep);
+  hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 2), dim3(256), 0, st,
+                     pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N);
+  hip_check(hipGetLastError(), "launch step4_dot");
+  pl->ctx->ntt.inverse(st, pl->out.as<uint64_t>(), 4, d.N, kNttFourStep);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcgsq_ole_plan_create(pcgsq_ctx* ctx, const pcgsq_params* p, uint64_t pub_seed,
+                          size_t hbm_budget_bytes, pcgsq_ole_plan** out) {
+  return guarded([&] {
+    require(ctx && p && out, "ole_plan_create: null argument");
+    CtxGuard g(ctx);
+    std::unique_ptr<pcgsq_ole_plan> pl(new pcgsq_ole_plan);
+    pl->ctx = ctx;
+    pl->d = make_dims(*p);
+    pl->pub_seed = pub_seed;
+    plan_alloc(pl.get(), hbm_budget_bytes);
+    *out = pl.release();
+  });
+}
+
+int pcgsq_ole_plan_destroy(pcgsq_ole_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    CtxGuard g(plan->ctx);
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole destroy");
+    delete plan;
+  });
+}
+
+size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* pl) {
+  if (!pl) return 0;
+  size_t b = 0;
+  for (const DevBuf* x : {&pl->roots, &pl->dlt, &pl->bt, &pl->noise, &pl->aes_t0, &pl->aes_rks,
+                          &pl->ahat, &pl->ct, &pl->part, &pl->cw, &pl->fw, &pl->out,
+                          &pl->dpf.fh[0], &pl->dpf.fh[1], &pl->dpf.cw, &pl->dpf.peer})
+    b += x->bytes;
+  return b;
+}
+
+int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed) {
+  return guarded([&] {
+    require(plan, "ole_deal: null plan");
+    CtxGuard g(plan->ctx);
+    plan_deal(plan, seed);
+  });
+}
+
+int pcgsq_ole_run(pcgsq_ole_plan* plan) {
+  return guarded([&] {
+    require(plan, "ole_run: null plan");
+    CtxGuard g(plan->ctx);
+    plan_run_blocks(plan);
+    plan_step4(plan);
+  });
+}
+
+int pcgsq_ole_run_blocks(pcgsq_ole_plan* plan) {
+  return guarded([&] {
+    require(plan, "ole_run_blocks: null plan");
+    CtxGuard g(plan->ctx);
+    plan_run_blocks(plan);
+  });
+}
+
+int pcgsq_ole_outputs(pcgsq_ole_plan* plan, uint64_t* x0, uint64_t* z0, uint64_t* x1,
+                      uint64_t* z1) {
+  return guarded([&] {
+    require(plan, "ole_outputs: null plan");
+    CtxGuard g(plan->ctx);
+    const size_t N = (size_t)plan->d.N;
+    uint64_t* dst[4] = {x0, z0, x1, z1};
+    for (int i = 0; i < 4; ++i)
+      if (dst[i])
+        hip_check(hipMemcpyAsync(dst[i], plan->out.as<uint64_t>() + i * N, N * 8,
+                                 hipMemcpyDeviceToHost, plan->ctx->stream), "ole outputs");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole outputs");
+  });
+}
+
+const uint64_t* pcgsq_ole_outputs_dev(pcgsq_ole_plan* plan) {
+  return plan ? plan->out.as<uint64_t>() : nullptr;
+}
+
+int pcgsq_ole_g(pcgsq_ole_plan* plan, uint64_t* g_host) {
+  return guarded([&] {
+    require(plan && g_host, "ole_g: null argument");
+    CtxGuard g(plan->ctx);
+    const size_t n = (size_t)2 * plan->d.nblk * plan->d.N;
+    hip_check(hipMemcpyAsync(g_host, plan->fw.ptr, n * 8, hipMemcpyDeviceToHost, plan->ctx->stream),
+              "ole g");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole g");
+  });
+}
+
+int pcgsq_ole_expand(pcgsq_ctx* ctx, const pcgsq_params* p, uint64_t seed, uint64_t pub_seed,
+                     uint64_t* x0, uint64_t* z0, uint64_t* x1, uint64_t* z1) {
+  pcgsq_ole_plan* pl = nullptr;
+  int rc = pcgsq_ole_plan_create(ctx, p, pub_seed, 0, &pl);
+  if (rc) return rc;
+  rc = pcgsq_ole_deal(pl, seed);
+  if (!rc) rc = pcgsq_ole_run(pl);
+  if (!rc) rc = pcgsq_ole_outputs(pl, x0, z0, x1, z1);
+  const int rc2 = pcgsq_ole_plan_destroy(pl);
+  return rc ? rc : rc2;
+}
+
+uint64_t pcgsq_dealer_u64(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
+  return dealer_u64(seed, domain, a, b);
+}
+
+}  // extern "C"

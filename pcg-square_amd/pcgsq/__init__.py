@@ -1,0 +1,409 @@
+"""Python host interface of libpcgsq (ctypes over include/pcgsq.h).
+
+Mirrors the reference's device plugin seam (GPU_baseline/common/*.h,
+pcg_cuda::*) and the local-compute entries of CPU_baseline (NTT<P>,
+poly_mul_batch, PCG_OLE<P>::gen_and_expand) with the same argument meaning.
+Errors raise PcgsqError (status code + message); an unsupported shape raises
+Unsupported, the reference's "exit 77" case.  There is no CPU fallback: if
+libpcgsq.so is missing or no gfx950 device is present, calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpcgsq.so")
+P = 4611686018326724609
+PRG_AES, PRG_CHACHA8 = 0, 1
+NTT_FOURSTEP, NTT_NAIVE = 0, 1
+
+OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ECALLBACK, EINTERNAL = range(7)
+
+
+class PcgsqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pcgsq error {code}: {msg}")
+        self.code = code
+
+
+class Unsupported(PcgsqError):
+    pass
+
+
+class Blk(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+class Params(C.Structure):
+    _fields_ = [("N", C.c_int), ("c", C.c_int), ("t", C.c_int), ("w", C.c_int),
+                ("prg", C.c_int)]
+
+
+EXCHANGE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+
+_lib = None
+u64p = C.POINTER(C.c_uint64)
+u8p = C.POINTER(C.c_uint8)
+vp = C.c_void_p
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree libpcgsq.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PcgsqError(EINTERNAL, f"{LIB_PATH} not built: run pcg-square_amd/build_native.py")
+    L = C.CDLL(LIB_PATH)
+    L.pcgsq_last_error.restype = C.c_char_p
+    L.pcgsq_device_available.restype = C.c_int
+    L.pcgsq_device_name.argtypes = [C.c_char_p, C.c_size_t]
+    L.pcgsq_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pcgsq_ctx_destroy.argtypes = [vp]
+    L.pcgsq_ctx_stream.argtypes = [vp]
+    L.pcgsq_ctx_stream.restype = vp
+    L.pcgsq_ctx_synchronize.argtypes = [vp]
+    L.pcgsq_dev_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.pcgsq_dev_free.argtypes = [vp, vp]
+    L.pcgsq_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+    L.pcgsq_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+    L.pcgsq_dpf_batch_full_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int,
+                                            EXCHANGE_FN, vp, vp, C.c_int]
+    L.pcgsq_dpf_batch_full_eval_device.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                                   C.c_int, EXCHANGE_FN, vp, C.POINTER(vp),
+                                                   C.c_int]
+    L.pcgsq_dpf_batch_full_eval_both.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                                 vp, vp, vp, vp, C.c_int]
+    L.pcgsq_dpf_out_sums.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_uint64, u64p, u64p]
+    L.pcgsq_dpf_out_scatter_g.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int,
+                                          C.c_uint64, u64p, u64p, C.c_int]
+    L.pcgsq_poly_mul_supported.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+    L.pcgsq_poly_mul_u64.argtypes = [vp, u64p, u64p, u64p, C.c_int, C.c_int, C.c_uint64,
+                                     C.c_int, C.POINTER(C.c_double)]
+    L.pcgsq_poly_mul_u64_dev.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    for f in ("pcgsq_ntt_forward", "pcgsq_ntt_inverse"):
+        getattr(L, f).argtypes = [vp, u64p, C.c_int, C.c_int, C.c_int]
+    for f in ("pcgsq_ntt_forward_dev", "pcgsq_ntt_inverse_dev"):
+        getattr(L, f).argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
+    L.pcgsq_aes128_expand_key.argtypes = [u8p, u8p]
+    L.pcgsq_aes128_ecb_encrypt.argtypes = [vp, u8p, u8p, u8p, C.c_int]
+    L.pcgsq_aes_tree_round_keys.argtypes = [u8p, C.c_int, u8p]
+    L.pcgsq_ole_plan_create.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_size_t,
+                                        C.POINTER(vp)]
+    L.pcgsq_ole_plan_destroy.argtypes = [vp]
+    L.pcgsq_ole_plan_bytes.argtypes = [vp]
+    L.pcgsq_ole_plan_bytes.restype = C.c_size_t
+    L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
+    L.pcgsq_ole_run.argtypes = [vp]
+    L.pcgsq_ole_run_blocks.argtypes = [vp]
+    L.pcgsq_ole_outputs.argtypes = [vp, u64p, u64p, u64p, u64p]
+    L.pcgsq_ole_outputs_dev.argtypes = [vp]
+    L.pcgsq_ole_outputs_dev.restype = vp
+    L.pcgsq_ole_g.argtypes = [vp, u64p]
+    L.pcgsq_ole_expand.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_uint64, u64p, u64p,
+                                   u64p, u64p]
+    L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
+    L.pcgsq_dealer_u64.restype = C.c_uint64
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().pcgsq_last_error().decode(errors="replace")
+        raise (Unsupported if rc == EUNSUPPORTED else PcgsqError)(rc, msg)
+
+
+def _p(a: np.ndarray, t=u64p):
+    return a.ctypes.data_as(t)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _blocks(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 2)
+
+
+def device_available() -> bool:
+    """pcg_cuda::is_cuda_available (GPU_baseline/common/poly_mul_cuda.h:9)."""
+    return bool(lib().pcgsq_device_available())
+
+
+def dealer_u64(seed: int, domain: int, a: int, b: int) -> int:
+    return lib().pcgsq_dealer_u64(seed, domain, a, b)
+
+
+class Context:
+    """One device + stream + owned scratch (replaces the reference's
+    process-global grow-only pools, dpf_gpu.cu:42-75)."""
+
+    def __init__(self, device: int = 0):
+        h = vp()
+        _check(lib().pcgsq_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().pcgsq_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return lib().pcgsq_ctx_stream(self.h)
+
+    def synchronize(self) -> None:
+        _check(lib().pcgsq_ctx_synchronize(self.h))
+
+    def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
+        """Copy a host array into a new device buffer owned by this context."""
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, arr.nbytes)
+        _check(lib().pcgsq_memcpy_h2d(self.h, buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(lib().pcgsq_device_name(buf, 256))
+        return buf.value.decode()
+
+    # ---- DPF (GPU_baseline/common/dpf_gpu.h:55-75) -------------------------
+    def dpf_batch_full_eval(self, roots, dltsft, n, w, exchange, party=0, prg=PRG_CHACHA8,
+                            round_keys=None):
+        """One party's batched full-domain half-tree DPF with a host exchange
+        callable(level, mine: ndarray[(count,2)]) -> peer ndarray."""
+        roots, dltsft = _blocks(roots), _blocks(dltsft)
+        B = len(roots)
+        out = np.empty((B << n, 2), dtype=np.uint64)
+        cb = self._exchange(exchange)
+        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
+        _check(lib().pcgsq_dpf_batch_full_eval(
+            self.h, B, n, w, roots.ctypes.data, dltsft.ctypes.data,
+            None if rk is None else rk.ctypes.data, party, cb, None, out.ctypes.data, prg))
+        return out
+
+    def dpf_batch_full_eval_device(self, roots, dltsft, n, w, exchange, party=0,
+                                   prg=PRG_CHACHA8, round_keys=None) -> int:
+        roots, dltsft = _blocks(roots), _blocks(dltsft)
+        cb = self._exchange(exchange)
+        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
+        d = vp()
+        _check(lib().pcgsq_dpf_batch_full_eval_device(
+            self.h, len(roots), n, w, roots.ctypes.data, dltsft.ctypes.data,
+            None if rk is None else rk.ctypes.data, party, cb, None, C.byref(d), prg))
+        return d.value
+
+    def dpf_full_eval_both(self, roots0, roots1, dlt0, dlt1, n, w, prg=PRG_CHACHA8,
+                           round_keys=None, want_cw=False):
+        roots0, roots1 = _blocks(roots0), _blocks(roots1)
+        dlt0, dlt1 = _blocks(dlt0), _blocks(dlt1)
+        B = len(roots0)
+        l0 = np.empty((B << n, 2), dtype=np.uint64)
+        l1 = np.empty((B << n, 2), dtype=np.uint64)
+        nk = (1 << w) - 1
+        cw = np.empty(((n // w) * B * nk, 2), dtype=np.uint64) if want_cw else None
+        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
+        _check(lib().pcgsq_dpf_batch_full_eval_both(
+            self.h, B, n, w, roots0.ctypes.data, roots1.ctypes.data, dlt0.ctypes.data,
+            dlt1.ctypes.data, None if rk is None else rk.ctypes.data, l0.ctypes.data,
+            l1.ctypes.data, cw.ctypes.data if want_cw else None, prg))
+        return (l0, l1, cw) if want_cw else (l0, l1)
+
+    def _exchange(self, fn):
+        def tramp(user, level, mine, peer, count):
+            try:
+                m = np.ctypeslib.as_array(C.cast(mine, u64p), shape=(count * 2,)).reshape(-1, 2)
+                res = _blocks(fn(level, m.copy()))
+                if res.shape != (count, 2):
+                    return 1
+                C.memmove(peer, res.ctypes.data, count * 16)
+                return 0
+            except Exception:
+                return 1
+        self._cb = EXCHANGE_FN(tramp)   # keep alive for the call
+        return self._cb
+
+    # ---- output layer (GPU_baseline/common/leaf_convert_cuda.h:43-52) -------
+    def dpf_out_sums(self, d_leaves: int, B: int, D: int):
+        sc = np.empty(B, dtype=np.uint64)
+        st = np.empty(B, dtype=np.uint64)
+        _check(lib().pcgsq_dpf_out_sums(self.h, d_leaves, B, D, P, _p(sc), _p(st)))
+        return sc, st
+
+    def dpf_out_scatter_g(self, d_leaves: int, B: int, D: int, t: int, party: int, CW, N: int):
+        cw = _u64(CW)
+        g = np.empty(N, dtype=np.uint64)
+        _check(lib().pcgsq_dpf_out_scatter_g(self.h, d_leaves, B, D, t, party, P, _p(cw), _p(g),
+                                             N))
+        return g
+
+    # ---- poly-mul / NTT (poly_mul_cuda.h, CPU_baseline/ntt.h) --------------
+    def poly_mul(self, a, b, backend=NTT_FOURSTEP):
+        a, b = _u64(a), _u64(b)
+        shape = a.shape
+        a2, b2 = a.reshape(-1, shape[-1]), b.reshape(-1, shape[-1])
+        out = np.empty_like(a2)
+        ms = C.c_double()
+        _check(lib().pcgsq_poly_mul_u64(self.h, _p(out), _p(a2), _p(b2), a2.shape[0],
+                                        a2.shape[1], P, backend, C.byref(ms)))
+        self.last_device_ms = ms.value
+        return out.reshape(shape)
+
+    def ntt_forward(self, a, backend=NTT_FOURSTEP):
+        x = _u64(a).copy()
+        x2 = x.reshape(-1, x.shape[-1])
+        _check(lib().pcgsq_ntt_forward(self.h, _p(x2), x2.shape[0], x2.shape[1], backend))
+        return x
+
+    def ntt_inverse(self, a, backend=NTT_FOURSTEP):
+        x = _u64(a).copy()
+        x2 = x.reshape(-1, x.shape[-1])
+        _check(lib().pcgsq_ntt_inverse(self.h, _p(x2), x2.shape[0], x2.shape[1], backend))
+        return x
+
+    def poly_mul_dev(self, out_ptr: int, a_ptr: int, b_ptr: int, batch: int, N: int,
+                     backend=NTT_FOURSTEP):
+        _check(lib().pcgsq_poly_mul_u64_dev(self.h, out_ptr, a_ptr, b_ptr, batch, N, backend))
+
+    def ntt_forward_dev(self, ptr: int, batch: int, N: int, backend=NTT_FOURSTEP):
+        _check(lib().pcgsq_ntt_forward_dev(self.h, ptr, batch, N, backend))
+
+    def ntt_inverse_dev(self, ptr: int, batch: int, N: int, backend=NTT_FOURSTEP):
+        _check(lib().pcgsq_ntt_inverse_dev(self.h, ptr, batch, N, backend))
+
+    # ---- AES (aes_gpu.h) ------------------------------------------------------
+    def aes128_ecb_encrypt(self, keys: bytes, pt: bytes) -> bytes:
+        n = len(pt) // 16
+        k = (C.c_uint8 * len(keys)).from_buffer_copy(keys)
+        i = (C.c_uint8 * len(pt)).from_buffer_copy(pt)
+        o = (C.c_uint8 * len(pt))()
+        _check(lib().pcgsq_aes128_ecb_encrypt(self.h, k, i, o, n))
+        return bytes(o)
+
+
+class DeviceBuffer:
+    """Raw device memory on a context's GPU (freed on close/GC)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        h = vp()
+        _check(lib().pcgsq_dev_alloc(ctx.h, nbytes, C.byref(h)))
+        self.ptr = h.value
+
+    def to_host(self, dtype=np.uint64) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(lib().pcgsq_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr and self.ctx.h:
+            lib().pcgsq_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def poly_mul_supported(N: int, backend=NTT_FOURSTEP, prime: int = P) -> tuple[bool, str]:
+    buf = C.create_string_buffer(256)
+    ok = lib().pcgsq_poly_mul_supported(prime, N, backend, buf, 256)
+    return bool(ok), buf.value.decode()
+
+
+def aes128_expand_key(key: bytes) -> bytes:
+    k = (C.c_uint8 * 16).from_buffer_copy(key)
+    rk = (C.c_uint8 * 176)()
+    _check(lib().pcgsq_aes128_expand_key(k, rk))
+    return bytes(rk)
+
+
+def aes_tree_round_keys(nk: int, master: bytes = bytes.fromhex(
+        "2b7e151628aed2a6abf7158809cf4f3c")) -> np.ndarray:
+    m = (C.c_uint8 * 16).from_buffer_copy(master)
+    rks = np.zeros(nk * 176, dtype=np.uint8)
+    _check(lib().pcgsq_aes_tree_round_keys(m, nk, _p(rks, u8p)))
+    return rks
+
+
+class OlePlan:
+    """PCG_OLE<P>::gen_and_expand for both parties on one GPU
+    (CPU_baseline/pcg_ole_2pc/pcg_ole_impl.h:46-490)."""
+
+    def __init__(self, ctx: Context, N: int, c: int, t: int, w: int = 1, prg: int = PRG_CHACHA8,
+                 pub_seed: int = 0, hbm_budget: int = 0):
+        self.ctx = ctx
+        self.N, self.c, self.t, self.w, self.prg = N, c, t, w, prg
+        p = Params(N, c, t, w, prg)
+        h = vp()
+        _check(lib().pcgsq_ole_plan_create(ctx.h, C.byref(p), pub_seed, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pcgsq_ole_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().pcgsq_ole_plan_bytes(self.h)
+
+    def deal(self, seed: int) -> None:
+        _check(lib().pcgsq_ole_deal(self.h, seed))
+
+    def run(self) -> None:
+        _check(lib().pcgsq_ole_run(self.h))
+
+    def run_blocks(self) -> None:
+        _check(lib().pcgsq_ole_run_blocks(self.h))
+
+    def outputs(self):
+        x0, z0, x1, z1 = (np.empty(self.N, dtype=np.uint64) for _ in range(4))
+        _check(lib().pcgsq_ole_outputs(self.h, _p(x0), _p(z0), _p(x1), _p(z1)))
+        return x0, z0, x1, z1
+
+    def g(self) -> np.ndarray:
+        out = np.empty((2, self.c * self.c, self.N), dtype=np.uint64)
+        _check(lib().pcgsq_ole_g(self.h, _p(out)))
+        return out
+
+    def outputs_dev(self) -> int:
+        return lib().pcgsq_ole_outputs_dev(self.h)
+
+
+def ole_expand(ctx: Context, N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0):
+    with OlePlan(ctx, N, c, t, w, prg, pub_seed) as pl:
+        pl.deal(seed)
+        pl.run()
+        return pl.outputs()

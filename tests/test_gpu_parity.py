@@ -1,0 +1,244 @@
+"""HIP path vs the CPU oracle, through the C ABI (libpcgsq.so).  Needs an
+MI355X.  Every comparison is bit-exact (all arithmetic is integer)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def pq():
+    import pcgsq
+    if not pcgsq.device_available():
+        pytest.fail("no gfx950 device visible (the GPU tests must run on an MI355X)")
+    return pcgsq
+
+
+@pytest.fixture(scope="module")
+def ctx(pq):
+    c = pq.Context(0)
+    yield c
+    c.close()
+
+
+def _dealt(n, w, B, seed=5):
+    rng = np.random.default_rng(seed)
+    m, levels = 1 << w, n // w
+    nk = m - 1
+    alphas = rng.integers(0, 1 << n, B)
+    d0 = rng.integers(0, 2**63, (B, 2), dtype=np.uint64)
+    d1 = rng.integers(0, 2**63, (B, 2), dtype=np.uint64)
+    d0[:, 0] |= np.uint64(1)
+    d1[:, 0] &= ~np.uint64(1)
+    W = rng.integers(0, 2**63, 2, dtype=np.uint64)
+    dl0 = rng.integers(0, 2**63, (B, levels, nk, 2), dtype=np.uint64)
+    dl1 = dl0.copy()
+    for b in range(B):
+        for i in range(levels):
+            k = O.lib().orc_dpf_digit(n, w, i, int(alphas[b]))
+            if k < nk:
+                dl1[b, i, k] ^= d0[b] ^ d1[b]
+    return alphas, d0 ^ W, d1 ^ W, dl0, dl1, d0 ^ d1
+
+
+@pytest.mark.parametrize("n,w,B,prg", [
+    (6, 1, 3, O.PRG_CHACHA8), (9, 1, 16, O.PRG_CHACHA8), (12, 1, 300, O.PRG_CHACHA8),
+    (6, 2, 5, O.PRG_CHACHA8), (9, 3, 4, O.PRG_CHACHA8), (8, 4, 3, O.PRG_CHACHA8),
+    (6, 1, 3, O.PRG_AES), (8, 2, 7, O.PRG_AES), (9, 3, 2, O.PRG_AES), (10, 1, 70, O.PRG_AES)])
+def test_dpf_both_parties_matches_oracle(ctx, pq, n, w, B, prg):
+    rks = O.aes_tree_round_keys((1 << w) - 1) if prg == O.PRG_AES else None
+    alphas, r0, r1, dl0, dl1, delta = _dealt(n, w, B)
+    e0, e1, ecw = O.dpf_full_eval_both(n, w, prg, r0, r1, dl0, dl1, rks=rks, want_cw=True)
+    g0, g1, gcw = ctx.dpf_full_eval_both(r0, r1, dl0, dl1, n, w, prg=prg, round_keys=rks,
+                                         want_cw=True)
+    assert np.array_equal(g0, e0)
+    assert np.array_equal(g1, e1)
+    # oracle cw is [b][i][m]; device cw is [i][b][nk]
+    m, levels = 1 << w, n // w
+    ecw = ecw.reshape(B, levels, m, 2)[:, :, : m - 1, :].transpose(1, 0, 2, 3)
+    assert np.array_equal(gcw.reshape(levels, B, m - 1, 2), ecw)
+    x = (g0 ^ g1).reshape(B, 1 << n, 2)
+    for b in range(B):
+        nz = np.nonzero(x[b].any(axis=1))[0]
+        assert list(nz) == [alphas[b]]
+        assert np.array_equal(x[b, alphas[b]], delta[b])
+
+
+def test_dpf_plateau_recipe_through_dropin_api(ctx, pq, golden):
+    """GPU_baseline/dpf_real_bench.cu:48-110 through the single-party entry
+    points (host exchange returning a zero peer share), dpf_out_sums and
+    dpf_out_scatter_g; checksum vs the reference's plateau.txt."""
+    case = golden["dpf_plateau"]["cases"][0]
+    n, B, t = case["n"], case["B"], case["t"]
+    levels = n
+    roots = np.zeros((B, 2), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        roots[:, 0] = np.uint64(0x9E3779B97F4A7C15) * (np.arange(B, dtype=np.uint64) + np.uint64(1))
+    roots[:, 1] = np.arange(B, dtype=np.uint64) + 1
+    i = np.arange(B * levels, dtype=np.uint64)
+    dlt = np.stack([np.uint64(0x1234567) * (i + np.uint64(1)), i], axis=1)
+    calls = []
+
+    def exchange(level, mine):
+        calls.append(level)
+        return np.zeros_like(mine)
+
+    D = 1 << n
+    d_leaves = ctx.dpf_batch_full_eval_device(roots, dlt, n, 1, exchange, party=0,
+                                              prg=pq.PRG_CHACHA8)
+    assert calls == list(range(levels))
+    sc, st = ctx.dpf_out_sums(d_leaves, B, D)
+    CW = (np.uint64(0xABCDEF) * (np.arange(B, dtype=np.uint64) + np.uint64(1))) % np.uint64(P)
+    gN = t * (D // 2)
+    g = ctx.dpf_out_scatter_g(d_leaves, B, D, t, 0, CW, gN)
+    cks = 0
+    for b in range(B):
+        cks ^= int(sc[b]) ^ int(st[b])
+    for k in range(0, gN, 97):
+        cks ^= int(g[k])
+    assert "%016x" % cks == case["checksum"]
+
+
+def test_host_exchange_callback_failure_reported(ctx, pq):
+    roots = np.ones((2, 2), dtype=np.uint64)
+    dlt = np.ones((2 * 6, 2), dtype=np.uint64)
+
+    def bad(level, mine):
+        raise RuntimeError("peer went away")
+
+    with pytest.raises(pq.PcgsqError) as ei:
+        ctx.dpf_batch_full_eval(roots, dlt, 6, 1, bad)
+    assert ei.value.code == pq.ECALLBACK
+
+
+def test_out_hash_bitexact_fixture(ctx, pq):
+    """GPU_baseline/test/out_hash_cuda_bitexact.cu:50-110: t=3, B=9, D=1024,
+    leaves and CWs from mt19937_64(12345), both parties."""
+    t, B, D = 3, 9, 1024
+    N = t * D // 2
+    m = O.MT64(12345)
+    leaves = m.take(B * D * 2).reshape(B * D, 2)
+    CW = m.take(B, P)
+    dl = ctx.to_device(leaves)
+    sc, st = ctx.dpf_out_sums(dl.ptr, B, D)
+    esc, est = O.out_sums(leaves, B, D)
+    assert np.array_equal(sc, esc) and np.array_equal(st, est)
+    for party in (0, 1):
+        g = ctx.dpf_out_scatter_g(dl.ptr, B, D, t, party, CW, N)
+        assert np.array_equal(g, O.out_scatter_g(leaves, B, D, t, party, CW, N))
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("logN", [2, 3, 4, 5, 7, 10, 11, 12, 13, 14, 16])
+def test_ntt_forward_inverse_match_oracle(ctx, pq, backend, logN):
+    N = 1 << logN
+    if backend == 0 and logN < 2:
+        pytest.skip("four-step needs N >= 4")
+    rng = np.random.default_rng(logN + 100 * backend)
+    batch = 3
+    a = rng.integers(0, P, (batch, N), dtype=np.uint64)
+    fa = ctx.ntt_forward(a, backend)
+    for k in range(batch):
+        assert np.array_equal(fa[k], O.ntt_forward(a[k]))
+    assert np.array_equal(ctx.ntt_inverse(fa, backend), a)
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("logN", [2, 6, 9, 12, 15, 17])
+def test_poly_mul_matches_oracle(ctx, pq, backend, logN):
+    N = 1 << logN
+    rng = np.random.default_rng(logN)
+    a = rng.integers(0, P, (2, N), dtype=np.uint64)
+    b = rng.integers(0, P, (2, N), dtype=np.uint64)
+    got = ctx.poly_mul(a, b, backend)
+    exp = O.poly_mul_batch(a, b, threads=8)
+    assert np.array_equal(got, exp)
+
+
+def test_poly_mul_patterns(ctx, pq):
+    # GPU_baseline/test/poly_mul_cuda.cpp:56-71 patterns (zero, one, impulse, max)
+    N = 1 << 12
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, P, N, dtype=np.uint64)
+    one = np.zeros(N, np.uint64); one[0] = 1
+    imp = np.zeros(N, np.uint64); imp[N - 1] = 1
+    mx = np.full(N, P - 1, np.uint64)
+    for backend in (0, 1):
+        assert np.array_equal(ctx.poly_mul(a, one, backend), a)
+        assert np.array_equal(ctx.poly_mul(a, np.zeros(N, np.uint64), backend), np.zeros(N, np.uint64))
+        assert np.array_equal(ctx.poly_mul(a, imp, backend), O.poly_mul(a, imp))
+        assert np.array_equal(ctx.poly_mul(mx, mx, backend), O.poly_mul(mx, mx))
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_reference_ntt_checksums_at_2_20(ctx, pq, golden, backend):
+    N, batch = 1 << 20, 16
+    m = O.MT64(11)
+    a = m.take(batch * N, P).reshape(batch, N)
+    b = m.take(batch * N, P).reshape(batch, N)
+    c = ctx.poly_mul(a, b, backend)
+    assert "%016x" % O.fnv1a64(c) == golden["ntt_naive_fnv"]["by_logN"]["20"]
+    j = np.arange(batch * N, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        a = ((np.uint64(0x9E3779B97F4A7C15) * (j + np.uint64(1))) % np.uint64(P)).reshape(batch, N)
+        b = ((np.uint64(0xC2B2AE3D27D4EB4F) * (j + np.uint64(7))) % np.uint64(P)).reshape(batch, N)
+    c = ctx.poly_mul(a, b, backend)
+    assert "%016x" % int(np.bitwise_xor.reduce(c.ravel())) == golden["ntt_4step_xor"]["by_logN"]["20"]
+
+
+def test_aes_ecb_fips_and_random(ctx, pq, golden):
+    for case in golden["aes128_fips197"]["cases"]:
+        ct = ctx.aes128_ecb_encrypt(bytes.fromhex(case["key"]), bytes.fromhex(case["pt"]))
+        assert ct.hex() == case["ct"]
+    rng = np.random.default_rng(7)
+    n = 513
+    keys = rng.integers(0, 256, n * 16, dtype=np.uint8).tobytes()
+    pts = rng.integers(0, 256, n * 16, dtype=np.uint8).tobytes()
+    got = ctx.aes128_ecb_encrypt(keys, pts)
+    for i in range(0, n, 37):
+        rk = O.aes128_expand_key(keys[16 * i:16 * i + 16])
+        assert got[16 * i:16 * i + 16] == O.aes128_encrypt(rk, pts[16 * i:16 * i + 16])
+    assert bytes(pq.aes_tree_round_keys(3)) == bytes(O.aes_tree_round_keys(3))
+
+
+@pytest.mark.parametrize("N,c,t,w,prg", [
+    (1 << 12, 2, 16, 1, O.PRG_CHACHA8),     # the reference's small CPU config
+    (1 << 10, 2, 4, 1, O.PRG_AES),
+    (1 << 11, 3, 8, 1, O.PRG_CHACHA8),
+    (1 << 12, 2, 8, 2, O.PRG_CHACHA8)])
+def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
+    seed = 11
+    with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=3) as pl:
+        pl.deal(seed)
+        pl.run_blocks()
+        g = pl.g()
+        eg0, eg1 = O.pcg_ole_blocks(N, c, t, w, prg, seed=seed, blk_begin=0, blk_end=c * c)
+        assert np.array_equal(g[0], eg0)
+        assert np.array_equal(g[1], eg1)
+        pl.run()
+        x0, z0, x1, z1 = pl.outputs()
+    ex0, ez0, ex1, ez1 = O.pcg_ole_expand(N, c, t, w, prg, seed=seed, pub_seed=3)
+    for got, exp in ((x0, ex0), (z0, ez0), (x1, ex1), (z1, ez1)):
+        assert np.array_equal(got, exp)
+
+
+def test_ole_relation_full_size(ctx, pq):
+    """N=2^20, c=4, t=16: x0*x1 == z0+z1 (CPU oracle poly-mul), the outputs
+    are deterministic across runs, and differ across seeds."""
+    N = 1 << 20
+    with pq.OlePlan(ctx, N, 4, 16, 1, pq.PRG_CHACHA8, pub_seed=0) as pl:
+        pl.deal(1)
+        pl.run()
+        x0, z0, x1, z1 = pl.outputs()
+        pl.run()
+        again = pl.outputs()
+        for u, v in zip((x0, z0, x1, z1), again):
+            assert np.array_equal(u, v)
+    prod = O.poly_mul(x0, x1)
+    s = (z0 + z1) % np.uint64(P)   # z < P < 2^62: no uint64 overflow
+    assert np.array_equal(prod, s)
+    assert int((x0 != 0).sum()) > N // 2
