@@ -1,24 +1,79 @@
 """PCG-OLE expansion throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 A step is one full PCG-OLE expansion for both parties at N=2^20, c=4, t=16,
-w=1, ChaCha8 (the reference's 2PC benchmark shape, pcg_ole_impl.h:46-490):
-4096 half-tree DPFs of domain 2^17 per party with the fused correction-word
-protocol, the Beaver-corrected output layer and step 4.  Inputs (dealt from a
-seed) are resident in HBM before timing starts.  Prints one JSON line.
+w=1, ChaCha8 (the shape of the reference's 2PC benchmark,
+CPU_baseline/test/bench_pcg_ole_2pc.cpp, pcg_ole_impl.h:46-490): 4096
+half-tree DPFs of domain 2^17 per party with the fused correction-word
+protocol, the Beaver-corrected output layer and step 4.  The per-step inputs
+come from a seeded dealer and are resident in HBM before the timed region.
+
+Prints one JSON line: OLEs/s (N OLEs per expansion), p50/p99 latency of one
+request per launch, the roofline of the dominant kernel (the fused leaf-level
+DPF + ChaCha8 out-hash kernel, VALU-bound), the NTT passes' HBM roofline,
+and the CPU oracle timed on this host.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pcg-square_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "OLEs/sec at N=2^20,c=4,t=16 (bit-exact) + p50/p99 request latency"
+CHACHA8_OPS = 384                    # int ops per ChaCha8 block (8 rounds x 4 QR x 12)
+VALU_NOMINAL = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 1 op/lane/clock
+HBM_PEAK = 8.0e12                    # MI355X HBM3E, bytes/s
+
+
+def percentile(xs, q):
+    s = sorted(xs)
+    k = max(0, min(len(s) - 1, int(round(q / 100.0 * (len(s) - 1)))))
+    return s[k]
+
+
+def pmc_traffic(path: str):
+    """HBM bytes per launch of the dominant kernel from a committed
+    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE), if present."""
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(N: int, c: int, t: int, threads: int) -> dict:
+    """The CPU oracle (a C restatement of CPU_baseline's local compute) on a
+    bounded sample: one DPF block of the c^2 (both parties, DPF + output
+    layer) and the c + 2c^2 poly-muls of step 4, each with `threads` OpenMP
+    threads; the full expansion is c^2 blocks + step 4."""
+    import numpy as np
+
+    import oracle as O
+    t0 = time.perf_counter()
+    O.pcg_ole_blocks(N, c, t, 1, O.PRG_CHACHA8, seed=1, blk_begin=0, blk_end=1, threads=threads)
+    t_block = time.perf_counter() - t0
+    npm = c + 2 * c * c
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, O.P, (npm, N), dtype=np.uint64)
+    b = rng.integers(0, O.P, (npm, N), dtype=np.uint64)
+    t0 = time.perf_counter()
+    O.poly_mul_batch(a, b, threads=threads)
+    t_step4 = time.perf_counter() - t0
+    full = c * c * t_block + t_step4
+    return {"value": N / full, "unit": "OLEs/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle (C, u128 %P, per-node ChaCha8) on {threads} host threads: 1 of "
+                       f"{c * c} DPF blocks both parties ({t_block:.2f} s) + {npm} poly-muls of "
+                       f"step 4 ({t_step4:.2f} s); expansion = {c * c} x block + step 4 = "
+                       f"{full:.1f} s")}
 
 
 def main() -> int:
@@ -32,8 +87,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              "the path's sharding is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build measures one GPU (asked for --gpus {args.gpus}); how the "
+              "path shards over several GPUs is described in DESIGN.md", file=sys.stderr)
         return 2
 
     import pcgsq
@@ -43,28 +98,88 @@ def main() -> int:
         pl.deal(1)
         for _ in range(args.warmup):
             pl.run()
+        # ---- throughput: K back-to-back expansions ----
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             pl.run()
         ctx.synchronize()
         dt = time.perf_counter() - t0
-    ms = dt * 1e3 / args.steps
+        # ---- latency: one request per launch, host-visible completion ----
+        lats = []
+        for _ in range(max(5, min(args.steps, 20))):
+            s = time.perf_counter()
+            pl.run()
+            ctx.synchronize()
+            lats.append((time.perf_counter() - s) * 1e3)
+        # ---- live per-launch timing of every kernel (HIP events on the stream) ----
+        ctx.profile(True)
+        nprof = 2
+        for _ in range(nprof):
+            pl.run()
+        ctx.synchronize()
+        zones = ctx.profile_read()
+        ctx.profile(False)
+        ceiling = ctx.probe_chacha8()
+        copy_bps = ctx.probe_copy(1 << 30)
+
+    leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
+    n_leaf, ms_leaf = zones["dpf.leaf_convert"]
+    avg_leaf = ms_leaf / n_leaf
+    ach = leaves * CHACHA8_OPS / (avg_leaf * 1e-3)
+    peak_meas = ceiling * CHACHA8_OPS
+    peak = max(VALU_NOMINAL, peak_meas)
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "dpf_leaf_convert_N20_c4_t16_pmc.txt"))
+    roofline = {"bound": "valu", "kernel": "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + "
+                                           "out-hash)",
+                "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "Tops/s",
+                "frac": ach / peak, "traffic": traffic,
+                "peak_nominal": VALU_NOMINAL / 1e12, "peak_measured": peak_meas / 1e12,
+                "algorithmic": f"{leaves} ChaCha8 out-hash blocks x {CHACHA8_OPS} ops per launch",
+                "avg_launch_ms": avg_leaf}
+    tree_ms = sum(v[1] for k, v in zones.items() if k.startswith("dpf.level.")) / nprof
+    tree_blocks = 2 * c * c * t * t * ((2 * N // t) - 1)
+    # NTT: the forward batch of step 4 (2c^2 + 2c transforms of length N)
+    nf = 2 * c * c + 2 * c
+    f1 = zones.get("ntt.fwd_pass1", (1, 0.0))
+    f2 = zones.get("ntt.fwd_pass2", (1, 0.0))
+    ntt_bytes_per_pass = nf * N * 16                 # read + write 8 B per coefficient
+    ntt_avg = (f1[1] / f1[0] + f2[1] / f2[0]) / 2
+    roof_ntt = {"bound": "hbm", "kernel": "ntt_pass_kernel (four-step, forward, batch %d, N=%d)"
+                % (nf, N), "achieved": ntt_bytes_per_pass / (ntt_avg * 1e-3) / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ntt_bytes_per_pass / (ntt_avg * 1e-3) / HBM_PEAK,
+                "traffic": None, "avg_launch_ms": ntt_avg}
     out = {
-        "metric": "OLEs/sec at N=2^20,c=4,t=16 (bit-exact) + p50/p99 request latency",
+        "metric": METRIC,
         "value": N * args.steps / dt,
         "unit": "OLEs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": dt * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic (seeded dealer)",
-        "config": {"workload": f"pcg-ole expansion both parties N={N} c={c} t={t} w=1 chacha8"},
+        "data": "synthetic: seeded dealer inputs (noise, Delta/DltSft, beta, Beaver shares)",
+        "config": {"workload": "pcg-ole expansion, both parties on one GPU", "N": N, "c": c,
+                   "t": t, "w": 1, "prg": "chacha8", "instances_per_party": c * c * t * t,
+                   "dpf_domain": 2 * N // t},
+        "latency_ms": {"p50": percentile(lats, 50), "p99": percentile(lats, 99),
+                       "form": "one request per launch, host-visible completion",
+                       "requests": len(lats)},
+        "per_party_OLEs_per_s": N * args.steps / dt / 2,
+        "roofline": roofline,
+        "roofline_ntt": roof_ntt,
+        "phases_ms": {k: v[1] / nprof for k, v in sorted(zones.items())},
+        "tree_prg_rate_Gblocks_per_s": tree_blocks / (tree_ms * 1e-3) / 1e9,
+        "chacha8_ceiling_Gblocks_per_s": ceiling / 1e9,
+        "hbm_copy_GBps": copy_bps / 1e9,
     }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(N, c, t, threads)
     print(json.dumps(out), flush=True)
     return 0
 

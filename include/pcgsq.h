@@ -66,6 +66,18 @@ int pcgsq_ctx_destroy(pcgsq_ctx* ctx);
 /* Stream the context launches on (hipStream_t, non-null, default priority). */
 void* pcgsq_ctx_stream(pcgsq_ctx* ctx);
 int pcgsq_ctx_synchronize(pcgsq_ctx* ctx);
+/* Per-launch device timing on the context stream, by zone name (the
+ * counterpart of the reference's PCG_DPF_LEVEL_MS / PCG_SQUARE_STAGE_MS
+ * splits, dpf_gpu.cu:427-493).  enable resets the totals;  read writes
+ * {"zone": [launches, total_ms], ...} as JSON text. */
+int pcgsq_ctx_profile(pcgsq_ctx* ctx, int enable);
+int pcgsq_ctx_profile_read(pcgsq_ctx* ctx, char* buf, size_t len);
+/* Register-only ChaCha8 block throughput of the device (blocks/s): the
+ * compute ceiling the DPF kernels are measured against. */
+int pcgsq_probe_chacha8(pcgsq_ctx* ctx, int iters, double* blocks_per_s);
+/* Plain device copy of `bytes` (read + write counted): achievable HBM
+ * bytes/s, and the known-size kernel that HBM counters are scaled by. */
+int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s);
 /* Device memory for ctypes callers that hold their own device-resident
  * buffers (no counterpart in the reference, whose plugin only takes host
  * pointers). */

@@ -218,7 +218,29 @@ void orc_output_layer_both(const orc_blk* leaves0, const orc_blk* leaves1,
     uint64_t* de = (uint64_t*)malloc(sizeof(uint64_t) * B * 4);
     uint64_t* cwp = (uint64_t*)malloc(sizeof(uint64_t) * B * 2);
     uint64_t* CW = (uint64_t*)malloc(sizeof(uint64_t) * B);
-    for (int p = 0; p < 2; ++p) orc_out_sums(lv[p], B, D, sumC + p * B, sumT + p * B);
+    /* all_C / all_tau (pcg_ole_impl.h:338-357): C = lo64(H'(leaf)) % P and
+     * tau = lsb(leaf) per leaf, hashed once, with the per-instance sums. */
+    uint64_t* allC[2];
+    uint8_t* allT[2];
+    for (int p = 0; p < 2; ++p) {
+        allC[p] = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)B * D);
+        allT[p] = (uint8_t*)malloc((size_t)B * D);
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < B; ++b) {
+            uint64_t c = 0, tt = 0;
+            for (size_t d = 0; d < D; ++d) {
+                const orc_blk lf = lv[p][(size_t)b * D + d];
+                const uint64_t C = orc_out_hash_lo64(lf) % ORC_P;
+                const uint8_t tau = (uint8_t)(lf.lo & 1);
+                allC[p][(size_t)b * D + d] = C;
+                allT[p][(size_t)b * D + d] = tau;
+                c = orc_fadd(c, C);
+                tt += tau;
+            }
+            sumC[p * B + b] = c;
+            sumT[p * B + b] = tt;
+        }
+    }
     /* A/B shares and the first Beaver open (pcg_ole_impl.h:358-371) */
     for (int p = 0; p < 2; ++p)
         for (int b = 0; b < B; ++b) {
@@ -251,9 +273,8 @@ void orc_output_layer_both(const orc_blk* leaves0, const orc_blk* leaves1,
             const int kk = b / t, ll = b % t;
             const size_t base = (size_t)(kk + ll) * bin;
             for (size_t d = 0; d < D; ++d) {
-                const orc_blk lf = lv[p][(size_t)b * D + d];
-                uint64_t y = orc_out_hash_lo64(lf) % ORC_P;
-                if (lf.lo & 1) y = orc_fadd(y, CW[b]);
+                uint64_t y = allC[p][(size_t)b * D + d];
+                if (allT[p][(size_t)b * D + d]) y = orc_fadd(y, CW[b]);
                 if (p == 1) y = orc_fneg(y);
                 const size_t pos = base + d;
                 if (pos < 2 * (size_t)N) g2[pos] = orc_fadd(g2[pos], y);
@@ -261,6 +282,7 @@ void orc_output_layer_both(const orc_blk* leaves0, const orc_blk* leaves1,
         }
         for (int d = 0; d < N; ++d) gr[p][d] = orc_fsub(g2[d], g2[(size_t)d + N]);
     }
+    for (int p = 0; p < 2; ++p) { free(allC[p]); free(allT[p]); }
     free(g2); free(sumC); free(sumT); free(de); free(cwp); free(CW);
 }
 

@@ -1,6 +1,7 @@
 // C ABI entry points (include/pcgsq.h) over the context, DPF, output-layer,
 // NTT and AES engines.  Every function runs inside guarded(): exceptions
 // become status codes and a per-thread last-error string.
+#include <stdio.h>
 #include <string.h>
 
 #include <chrono>
@@ -51,6 +52,27 @@ __global__ void aes_ecb_kernel(const uint8_t* __restrict__ keys, const uint8_t* 
   uint8_t* o = out + 16 * (size_t)b;
   for (int i = 0; i < 4; ++i)
     for (int k = 0; k < 4; ++k) o[4 * i + k] = (uint8_t)(s[i] >> (8 * k));
+}
+
+// Register-only ChaCha8 block loop: the compute ceiling of the tree PRG on
+// this device (each block keyed by the previous output, so nothing folds).
+__global__ void chacha8_probe_kernel(uint32_t* out, int iters) {
+  uint32_t k0 = blockIdx.x * blockDim.x + threadIdx.x, k1 = 0x9e3779b9u, k2 = k0 * 3u, k3 = ~k0;
+  uint32_t acc = 0;
+  for (int i = 0; i < iters; ++i) {
+    uint32_t o[4];
+    chacha8<4>(k0, k1, k2, k3, (uint64_t)i, 0, o);
+    acc ^= o[0] ^ o[1] ^ o[2] ^ o[3];
+    k0 ^= acc;
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+// Plain 16-byte-per-thread copy: the known-size reference for HBM counters.
+__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 void upload(DevBuf& buf, const void* src, size_t bytes, hipStream_t st, const char* what) {
@@ -162,6 +184,7 @@ int pcgsq_ctx_create(int device, pcgsq_ctx** out) {
             kErrUnsupported);
     std::unique_ptr<pcgsq_ctx> c(new pcgsq_ctx);
     c->device = device;
+    c->ntt.prof = &c->prof;
     c->activate();
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     *out = c.release();
@@ -187,6 +210,97 @@ int pcgsq_ctx_synchronize(pcgsq_ctx* ctx) {
     require(ctx, "null ctx");
     CtxGuard g(ctx);
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+int pcgsq_ctx_profile(pcgsq_ctx* ctx, int enable) {
+  return guarded([&] {
+    require(ctx, "null ctx");
+    CtxGuard g(ctx);
+    ctx->prof.reset();
+    ctx->prof.enabled = enable != 0;
+  });
+}
+
+int pcgsq_ctx_profile_read(pcgsq_ctx* ctx, char* buf, size_t len) {
+  return guarded([&] {
+    require(ctx && buf && len, "profile_read: null argument");
+    CtxGuard g(ctx);
+    ctx->prof.resolve();
+    std::string s = "{";
+    bool first = true;
+    for (const auto& kv : ctx->prof.totals) {
+      char tmp[160];
+      snprintf(tmp, sizeof tmp, "%s\"%s\": [%ld, %.6f]", first ? "" : ", ", kv.first.c_str(),
+               kv.second.count, kv.second.ms);
+      s += tmp;
+      first = false;
+    }
+    s += "}";
+    require(s.size() < len, "profile_read: buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+int pcgsq_probe_chacha8(pcgsq_ctx* ctx, int iters, double* blocks_per_s) {
+  return guarded([&] {
+    require(ctx && blocks_per_s && iters > 0, "probe: bad argument");
+    CtxGuard g(ctx);
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, ctx->device), "props");
+    const int grid = prop.multiProcessorCount * 8, tpb = 256;
+    ctx->in_d.ensure((size_t)grid * tpb * 4, "probe");
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      hip_check(hipEventRecord(e0, ctx->stream), "event");
+      hipLaunchKernelGGL(chacha8_probe_kernel, dim3(grid), dim3(tpb), 0, ctx->stream,
+                         ctx->in_d.as<uint32_t>(), iters);
+      hip_check(hipGetLastError(), "launch probe");
+      hip_check(hipEventRecord(e1, ctx->stream), "event");
+      hip_check(hipEventSynchronize(e1), "event");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+      if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *blocks_per_s = (double)grid * tpb * iters / (best * 1e-3);
+  });
+}
+
+int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s) {
+  return guarded([&] {
+    require(ctx && bytes_per_s && bytes >= 16, "probe_copy: bad argument");
+    CtxGuard g(ctx);
+    const size_t n = bytes / 16;
+    DevBuf a, b;
+    a.ensure(n * 16, "probe_copy");
+    b.ensure(n * 16, "probe_copy");
+    hip_check(hipMemsetAsync(a.ptr, 1, n * 16, ctx->stream), "probe_copy");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, ctx->device), "props");
+    const int grid = prop.multiProcessorCount * 16;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      hip_check(hipEventRecord(e0, ctx->stream), "event");
+      hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                         a.as<uint4>(), b.as<uint4>(), n);
+      hip_check(hipGetLastError(), "launch copy probe");
+      hip_check(hipEventRecord(e1, ctx->stream), "event");
+      hip_check(hipEventSynchronize(e1), "event");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+      if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *bytes_per_s = 2.0 * (double)n * 16 / (best * 1e-3);   // read + write
   });
 }
 

@@ -7,6 +7,7 @@
 #include "dpf.hpp"
 #include "internal.hpp"
 #include "ntt.hpp"
+#include "profiler.hpp"
 
 struct pcgsq_ctx {
   int device = 0;
@@ -15,6 +16,7 @@ struct pcgsq_ctx {
   pcgsq::DpfScratch dpf;
   pcgsq::NttEngine ntt;
   pcgsq::DevBuf in_a, in_b, in_c, in_d, leaves, part, sums, cwbuf, gbuf, t0, rks;
+  pcgsq::Profiler prof;
 
   void activate() { pcgsq::hip_check(hipSetDevice(device), "hipSetDevice"); }
 };

@@ -184,6 +184,7 @@ void launch_gather_g(hipStream_t st, const GatherArgs& a) {
   require(a.N > 0 && a.t > 0 && a.B > 0 && a.nsets > 0 && a.sets_per_party > 0,
           "gather: bad shape");
   require(a.nsets <= 65535, "gather: too many sets");
+  ProfZone pz(a.prof, st, "out.gather");
   const int grid = (a.N + kTpb - 1) / kTpb;
   if (a.leaves)
     hipLaunchKernelGGL(gather_g_kernel<true>, dim3(grid, a.nsets), dim3(kTpb), 0, st, a);

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "profiler.hpp"
 
 namespace pcgsq {
 
@@ -47,6 +48,7 @@ struct GatherArgs {
   int first_party = 0;            // party of set 0 (single-party calls)
   int B = 0, t = 0, N = 0;
   int D_log = 0;                  // D = 2*bin = 2^D_log
+  Profiler* prof = nullptr;
 };
 void launch_gather_g(hipStream_t st, const GatherArgs& a);
 

@@ -9,6 +9,7 @@
 // across the workgroup through LDS, and across workgroups with one 64-bit
 // atomic XOR per (workgroup, word): XOR is associative and commutative, so
 // the result is bit-exact whatever the order.
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -199,7 +200,12 @@ void launch_root(hipStream_t st, const DpfRun& r, Blk* F0, Blk* H0, Blk* cw) {
 }
 
 template <int W, int PRG, int MODE>
-void launch_level(hipStream_t st, LevelArgs a) {
+void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
+  char zone[32];
+  snprintf(zone, sizeof zone, MODE == kModeHash ? "dpf.level.%02d"
+                              : MODE == kModeStore ? "dpf.leaf_store" : "dpf.leaf_convert",
+           a.level);
+  ProfZone pz(prof, st, zone);
   const size_t total = (size_t)a.ntrees << a.S_log;
   const size_t tiles = (total + kTpb - 1) / kTpb;
   const size_t grid = (tiles + a.iters - 1) / a.iters;
@@ -229,7 +235,10 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
 
   Blk *F, *H;
   frontier(0, 1, F, H);
-  launch_root<W, PRG>(st, r, F, H, s.cw.as<Blk>());
+  {
+    ProfZone pz(r.prof, st, "dpf.root");
+    launch_root<W, PRG>(st, r, F, H, s.cw.as<Blk>());
+  }
 
   std::vector<Blk> h_mine, h_peer;
   const size_t cw_level = (size_t)r.ntrees * NK;
@@ -280,16 +289,16 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
       frontier(cur ^ 1, S * M, Fn, Hn);
       a.Fc = Fn;
       a.Hc = Hn;
-      launch_level<W, PRG, kModeHash>(st, a);
+      launch_level<W, PRG, kModeHash>(st, a, r.prof);
       cur ^= 1;
     } else if (r.mode == kLeafStore) {
       a.Fc = r.leaves;
-      launch_level<W, PRG, kModeStore>(st, a);
+      launch_level<W, PRG, kModeStore>(st, a, r.prof);
     } else {
       a.ct = r.ct;
       a.part = r.partials;
       a.nparts = dpf_leaf_nparts(n, W, r.iters_max);
-      launch_level<W, PRG, kModeConvert>(st, a);
+      launch_level<W, PRG, kModeConvert>(st, a, r.prof);
     }
   }
   if (r.cw_out)

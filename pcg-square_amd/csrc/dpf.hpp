@@ -15,6 +15,7 @@
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "profiler.hpp"
 
 namespace pcgsq {
 
@@ -51,6 +52,7 @@ struct DpfRun {
   uint64_t* partials = nullptr;   // kLeafConvert: [ntrees][nparts][2]
   Blk* cw_out = nullptr;          // optional [levels][ntrees][nk] copy of the shares
   int iters_max = 8;              // tiles of 256 parents per workgroup
+  Profiler* prof = nullptr;       // optional per-launch timing
 };
 
 // Scratch owned by a context; grows on demand.

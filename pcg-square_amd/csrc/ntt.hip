@@ -360,7 +360,9 @@ int cols_per_wg(int n_log, int ncols) {
 }
 
 template <bool INV, bool TW, bool PROD>
-void launch_pass(hipStream_t st, PassArgs a, int batch) {
+void launch_pass(hipStream_t st, PassArgs a, int batch, Profiler* prof) {
+  ProfZone pz(prof, st, INV ? (TW ? "ntt.inv_pass1" : "ntt.inv_pass2")
+                                  : (TW ? "ntt.fwd_pass1" : "ntt.fwd_pass2"));
   const size_t lds = ((size_t)a.C << a.n_log) * sizeof(uint64_t);
   require(a.ncols % a.C == 0, "ntt: tile does not divide the columns");
   require(lds <= lds_budget(), "ntt: tile exceeds LDS");
@@ -377,7 +379,7 @@ void launch_pass(hipStream_t st, PassArgs a, int batch) {
 }
 
 void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t* tmp,
-                       uint64_t* out, int batch) {
+                       uint64_t* out, int batch, Profiler* prof) {
   const int n1 = 1 << pl.n1_log, n2 = 1 << pl.n2_log;
   PassArgs a{};
   // pass 1: columns i2, length-n1 negacyclic over i1, twiddle psi^(i2(2k1+1))
@@ -388,7 +390,7 @@ void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t
   a.n_log = pl.n1_log; a.ncols = n2; a.C = cols_per_wg(pl.n1_log, n2);
   a.in_ps = n2; a.in_cs = 1; a.out_ps = n2; a.out_cs = 1;
   a.in_brv = 0; a.out_brv = 1; a.N_log = pl.L;
-  launch_pass<false, true, false>(st, a, batch);
+  launch_pass<false, true, false>(st, a, batch, prof);
   // pass 2: rows k1, length-n2 cyclic over i2, written transposed (natural k)
   PassArgs b{};
   b.in = tmp; b.out = out;
@@ -396,11 +398,11 @@ void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t
   b.n_log = pl.n2_log; b.ncols = n1; b.C = cols_per_wg(pl.n2_log, n1);
   b.in_ps = 1; b.in_cs = n2; b.out_ps = n1; b.out_cs = 1;
   b.in_brv = 0; b.out_brv = 1; b.N_log = pl.L;
-  launch_pass<false, false, false>(st, b, batch);
+  launch_pass<false, false, false>(st, b, batch, prof);
 }
 
 void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const uint64_t* in2,
-                       uint64_t* tmp, uint64_t* out, int batch) {
+                       uint64_t* tmp, uint64_t* out, int batch, Profiler* prof) {
   const int n1 = 1 << pl.n1_log, n2 = 1 << pl.n2_log;
   PassArgs a{};
   // pass I1: columns k1 of the [k2][k1] input, inverse cyclic over k2,
@@ -412,8 +414,8 @@ void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const ui
   a.n_log = pl.n2_log; a.ncols = n1; a.C = cols_per_wg(pl.n2_log, n1);
   a.in_ps = n1; a.in_cs = 1; a.out_ps = 1; a.out_cs = n2;
   a.in_brv = 1; a.out_brv = 0; a.N_log = pl.L;
-  if (in2) launch_pass<true, true, true>(st, a, batch);
-  else launch_pass<true, true, false>(st, a, batch);
+  if (in2) launch_pass<true, true, true>(st, a, batch, prof);
+  else launch_pass<true, true, false>(st, a, batch, prof);
   // pass I2: columns i2, inverse negacyclic over k1
   PassArgs b{};
   b.in = tmp; b.out = out;
@@ -421,7 +423,7 @@ void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const ui
   b.n_log = pl.n1_log; b.ncols = n2; b.C = cols_per_wg(pl.n1_log, n2);
   b.in_ps = n2; b.in_cs = 1; b.out_ps = n2; b.out_cs = 1;
   b.in_brv = 1; b.out_brv = 0; b.N_log = pl.L;
-  launch_pass<true, false, false>(st, b, batch);
+  launch_pass<true, false, false>(st, b, batch, prof);
 }
 
 void naive_forward_brv(hipStream_t st, NttPlan& pl, uint64_t* d, int batch) {
@@ -462,7 +464,7 @@ void NttEngine::forward(hipStream_t st, uint64_t* d, int batch, int N, int backe
   const size_t total = (size_t)batch * N;
   if (backend == kNttFourStep) {
     uint64_t* tmp = scratch(total, 0);
-    four_step_forward(st, pl, d, tmp, d, batch);
+    four_step_forward(st, pl, d, tmp, d, batch, prof);
   } else {
     naive_forward_brv(st, pl, d, batch);
     uint64_t* tmp = scratch(total, 0);
@@ -480,7 +482,7 @@ void NttEngine::inverse(hipStream_t st, uint64_t* d, int batch, int N, int backe
   const size_t total = (size_t)batch * N;
   if (backend == kNttFourStep) {
     uint64_t* tmp = scratch(total, 0);
-    four_step_inverse(st, pl, d, nullptr, tmp, d, batch);
+    four_step_inverse(st, pl, d, nullptr, tmp, d, batch, prof);
   } else {
     uint64_t* tmp = scratch(total, 0);
     hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(total)), dim3(kEwThreads), 0, st, d, tmp,
@@ -494,7 +496,7 @@ void NttEngine::inverse_of_product(hipStream_t st, uint64_t* out, const uint64_t
                                    const uint64_t* y, int batch, int N) {
   NttPlan& pl = plan(N);
   uint64_t* tmp = scratch((size_t)batch * N, 0);
-  four_step_inverse(st, pl, x, y, tmp, out, batch);
+  four_step_inverse(st, pl, x, y, tmp, out, batch, prof);
 }
 
 void NttEngine::poly_mul(hipStream_t st, uint64_t* out, const uint64_t* a, const uint64_t* b,
@@ -508,9 +510,9 @@ void NttEngine::poly_mul(hipStream_t st, uint64_t* out, const uint64_t* a, const
   uint64_t* fb = scratch(total, 2);
   if (backend == kNttFourStep) {
     uint64_t* tmp = scratch(total, 0);
-    four_step_forward(st, pl, a, tmp, fa, batch);
-    four_step_forward(st, pl, b, tmp, fb, batch);
-    four_step_inverse(st, pl, fa, fb, tmp, out, batch);
+    four_step_forward(st, pl, a, tmp, fa, batch, prof);
+    four_step_forward(st, pl, b, tmp, fb, batch, prof);
+    four_step_inverse(st, pl, fa, fb, tmp, out, batch, prof);
   } else {
     // textbook: twist, DIF stages, pointwise in the bit-reversed domain,
     // DIT stages, twist back (naive_ntt_bench.cu:147-159)

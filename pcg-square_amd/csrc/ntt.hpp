@@ -23,6 +23,7 @@
 #include <memory>
 
 #include "internal.hpp"
+#include "profiler.hpp"
 
 namespace pcgsq {
 
@@ -48,6 +49,7 @@ class NttEngine {
   // Scratch for a batch of transforms (grows on demand).
   uint64_t* scratch(size_t words, int which);
   NttPlan& plan(int N);
+  Profiler* prof = nullptr;  // optional per-launch timing
 
  private:
   struct Impl;

@@ -292,9 +292,13 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
     r.mode = kLeafConvert;
     r.ct = pl->ct.as<uint64_t>();
     r.partials = pl->part.as<uint64_t>();
+    r.prof = &pl->ctx->prof;
     dpf_expand(st, r, pl->dpf);
-    launch_beaver_cw(st, pl->part.as<uint64_t>(), gb * d.B, nparts, BT + i0, BT + 2 * ni + i0,
-                     BT + 4 * ni + i0, BT + 6 * ni + i0, ni, pl->cw.as<uint64_t>() + i0);
+    {
+      ProfZone pzb(&pl->ctx->prof, st, "out.beaver");
+      launch_beaver_cw(st, pl->part.as<uint64_t>(), gb * d.B, nparts, BT + i0, BT + 2 * ni + i0,
+                       BT + 4 * ni + i0, BT + 6 * ni + i0, ni, pl->cw.as<uint64_t>() + i0);
+    }
     GatherArgs ga;
     ga.ct = pl->ct.as<uint64_t>();
     ga.set_src_stride = (size_t)d.B * d.D;
@@ -308,6 +312,7 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
     ga.t = d.t;
     ga.N = d.N;
     ga.D_log = d.D_log;
+    ga.prof = &pl->ctx->prof;
     launch_gather_g(st, ga);
   }
 }
@@ -326,7 +331,9 @@ void plan_step4(pcgsq_ole_plan* pl) {
   hipLaunchKernelGGL(build_e_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, E,
                      pl->noise.as<uint64_t>(), pl->noise.as<uint64_t>() + ne, d.c, d.t, d.N);
   hip_check(hipGetLastError(), "launch build_e");
+  pl->ctx->ntt.prof = &pl->ctx->prof;
   pl->ctx->ntt.forward(st, G, 2 * d.nblk + 2 * d.c, d.N, kNttFourStep);
+  ProfZone pzd(&pl->ctx->prof, st, "ole.step4_dot");
   hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 2), dim3(256), 0, st,
                      pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N);
   hip_check(hipGetLastError(), "launch step4_dot");

@@ -66,6 +66,10 @@ def lib() -> C.CDLL:
     L.pcgsq_ctx_stream.argtypes = [vp]
     L.pcgsq_ctx_stream.restype = vp
     L.pcgsq_ctx_synchronize.argtypes = [vp]
+    L.pcgsq_ctx_profile.argtypes = [vp, C.c_int]
+    L.pcgsq_ctx_profile_read.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.pcgsq_probe_chacha8.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.pcgsq_probe_copy.argtypes = [vp, C.c_size_t, C.POINTER(C.c_double)]
     L.pcgsq_dev_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.pcgsq_dev_free.argtypes = [vp, vp]
     L.pcgsq_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
@@ -170,6 +174,29 @@ class Context:
 
     def synchronize(self) -> None:
         _check(lib().pcgsq_ctx_synchronize(self.h))
+
+    def profile(self, enable: bool = True) -> None:
+        """Start (and reset) or stop per-launch device timing."""
+        _check(lib().pcgsq_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """{zone: (launches, total_ms)} of the launches since profile(True)."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().pcgsq_ctx_profile_read(self.h, buf, 1 << 16))
+        return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}
+
+    def probe_chacha8(self, iters: int = 2048) -> float:
+        """Register-only ChaCha8 blocks/s of this device."""
+        v = C.c_double()
+        _check(lib().pcgsq_probe_chacha8(self.h, iters, C.byref(v)))
+        return v.value
+
+    def probe_copy(self, nbytes: int = 1 << 30) -> float:
+        """Achievable HBM bytes/s of a plain copy (read + write)."""
+        v = C.c_double()
+        _check(lib().pcgsq_probe_copy(self.h, nbytes, C.byref(v)))
+        return v.value
 
     def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
         """Copy a host array into a new device buffer owned by this context."""
