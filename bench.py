@@ -87,8 +87,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU (asked for --gpus {args.gpus}); how the "
-              "path shards over several GPUs is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the expansion "
+              "shards by DPF block across GPUs but no multi-GPU run is provided", file=sys.stderr)
         return 2
 
     import pcgsq

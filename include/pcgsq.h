@@ -3,7 +3,7 @@
  * PCG-OLE expansion path of tkgong/pcg-square.
  *
  * Every entry point replaces a function of the reference's device plugin seam
- * (GPU_baseline/common/*.h, namespace pcg_cuda) or a local-compute entry of
+ * (GPU_baseline/common/ headers, namespace pcg_cuda) or a local-compute entry of
  * CPU_baseline, cited next to it.  Conventions:
  *   - return value: PCGSQ_OK (0) or a PCGSQ_E* status; no C++ exception ever
  *     crosses this boundary; pcgsq_last_error() holds the message (per thread);

@@ -1,0 +1,281 @@
+// pcg_hip.hpp -- C++ drop-in for the reference's device plugin seam.
+//
+// Same names, argument meaning, ownership and error behaviour as the
+// reference's pcg_cuda:: headers (GPU_baseline/common/dpf_gpu.h,
+// leaf_convert_cuda.h, poly_mul_cuda.h, aes_gpu.h), implemented over the C
+// ABI of libpcgsq (include/pcgsq.h).  A CPU_baseline build switches from the
+// CUDA plugin to this one by including this header where it includes the
+// pcg_cuda headers and aliasing the namespace:
+//
+//     #include "pcgsq/pcg_hip.hpp"
+//     namespace pcg_cuda = pcg_hip;
+//
+// Like the reference plugin, the functions use one process-wide device
+// context (device 0), are synchronous, serialize on an internal mutex, take
+// host pointers, and throw std::runtime_error on failure.  Pointers returned
+// by the *_device functions alias internal scratch and stay valid until the
+// next DPF call.
+#ifndef PCGSQ_PCG_HIP_HPP
+#define PCGSQ_PCG_HIP_HPP
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pcgsq.h"
+
+namespace pcg_hip {
+
+// ---- common/dpf_gpu.h -------------------------------------------------------
+struct alignas(16) DpfBlk {   // dpf_gpu.h:23-26
+  uint64_t lo;
+  uint64_t hi;
+};
+using DpfExchangeFn =         // dpf_gpu.h:31-32
+    std::function<void(const DpfBlk* my_cws, DpfBlk* peer_cws, size_t count)>;
+enum class DpfGpuPrg { AES, CHACHA8 };   // dpf_gpu.h:38
+
+// ---- common/leaf_convert_cuda.h ----------------------------------------------
+constexpr uint64_t kOutHashDomain = 0x6f75745f68617368ULL;   // leaf_convert_cuda.h:38
+
+// ---- common/poly_mul_cuda.h --------------------------------------------------
+struct PolyMulStats {   // poly_mul_cuda.h:9-11
+  double device_ms = 0.0;
+};
+
+namespace detail {
+
+inline pcgsq_ctx* ctx() {
+  static pcgsq_ctx* c = [] {
+    pcgsq_ctx* p = nullptr;
+    if (pcgsq_ctx_create(0, &p) != PCGSQ_OK)
+      throw std::runtime_error(std::string("pcg_hip: ") + pcgsq_last_error());
+    return p;
+  }();
+  return c;
+}
+
+inline std::mutex& mu() {
+  static std::mutex m;
+  return m;
+}
+
+inline void check(int rc) {
+  if (rc != PCGSQ_OK) throw std::runtime_error(std::string("pcg_hip: ") + pcgsq_last_error());
+}
+
+inline int exchange_tramp(void* user, int /*level*/, const pcgsq_blk* mine, pcgsq_blk* peer,
+                          size_t count) {
+  try {
+    (*static_cast<const DpfExchangeFn*>(user))(reinterpret_cast<const DpfBlk*>(mine),
+                                               reinterpret_cast<DpfBlk*>(peer), count);
+    return 0;
+  } catch (...) {
+    return 1;   // reported as PCGSQ_ECALLBACK, rethrown as runtime_error
+  }
+}
+
+inline int prg_code(DpfGpuPrg p) { return p == DpfGpuPrg::AES ? PCGSQ_PRG_AES : PCGSQ_PRG_CHACHA8; }
+
+}  // namespace detail
+
+// dpf_gpu.h:55-62
+inline void dpf_gpu_batch_full_eval(int B, int n, int w, const DpfBlk* roots,
+                                    const DpfBlk* dltsft, const uint8_t* round_keys, int party,
+                                    const DpfExchangeFn& exchange, DpfBlk* out_leaves,
+                                    DpfGpuPrg prg = DpfGpuPrg::AES) {
+  if (B <= 0) return;
+  std::lock_guard<std::mutex> lk(detail::mu());
+  detail::check(pcgsq_dpf_batch_full_eval(
+      detail::ctx(), B, n, w, reinterpret_cast<const pcgsq_blk*>(roots),
+      reinterpret_cast<const pcgsq_blk*>(dltsft), round_keys, party, detail::exchange_tramp,
+      const_cast<DpfExchangeFn*>(&exchange), reinterpret_cast<pcgsq_blk*>(out_leaves),
+      detail::prg_code(prg)));
+}
+
+// dpf_gpu.h:69-75
+inline const DpfBlk* dpf_gpu_batch_full_eval_device(int B, int n, int w, const DpfBlk* roots,
+                                                    const DpfBlk* dltsft,
+                                                    const uint8_t* round_keys, int party,
+                                                    const DpfExchangeFn& exchange,
+                                                    DpfGpuPrg prg = DpfGpuPrg::AES) {
+  if (B <= 0) return nullptr;
+  std::lock_guard<std::mutex> lk(detail::mu());
+  const pcgsq_blk* d = nullptr;
+  detail::check(pcgsq_dpf_batch_full_eval_device(
+      detail::ctx(), B, n, w, reinterpret_cast<const pcgsq_blk*>(roots),
+      reinterpret_cast<const pcgsq_blk*>(dltsft), round_keys, party, detail::exchange_tramp,
+      const_cast<DpfExchangeFn*>(&exchange), &d, detail::prg_code(prg)));
+  return reinterpret_cast<const DpfBlk*>(d);
+}
+
+// leaf_convert_cuda.h:43-44
+inline void dpf_out_sums(const DpfBlk* d_leaves, int B, size_t D, uint64_t prime,
+                         uint64_t* sumC_host, uint64_t* sumT_host) {
+  if (B <= 0) return;
+  std::lock_guard<std::mutex> lk(detail::mu());
+  detail::check(pcgsq_dpf_out_sums(detail::ctx(), reinterpret_cast<const pcgsq_blk*>(d_leaves), B,
+                                   D, prime, sumC_host, sumT_host));
+}
+
+// leaf_convert_cuda.h:49-52
+inline void dpf_out_scatter_g(const DpfBlk* d_leaves, int B, size_t D, int t, int party,
+                              uint64_t prime, const uint64_t* CW_host, uint64_t* g_out_host,
+                              int N) {
+  if (B <= 0) return;
+  std::lock_guard<std::mutex> lk(detail::mu());
+  detail::check(pcgsq_dpf_out_scatter_g(detail::ctx(),
+                                        reinterpret_cast<const pcgsq_blk*>(d_leaves), B, D, t,
+                                        party, prime, CW_host, g_out_host, N));
+}
+
+// poly_mul_cuda.h:9
+inline bool is_cuda_available() { return pcgsq_device_available() != 0; }
+
+inline std::string device_name() {
+  char buf[256] = {0};
+  std::lock_guard<std::mutex> lk(detail::mu());
+  (void)detail::ctx();
+  detail::check(pcgsq_device_name(buf, sizeof buf));
+  return buf;
+}
+
+inline std::string runtime_version() { return device_name(); }
+
+namespace detail {
+inline bool supported(uint64_t prime, int N, int backend, std::string* reason) {
+  char buf[256] = {0};
+  const bool ok = pcgsq_poly_mul_supported(prime, N, backend, buf, sizeof buf) != 0;
+  if (!ok && reason) *reason = buf;
+  return ok;
+}
+inline void poly_mul(uint64_t* out, const uint64_t* a, const uint64_t* b, int batch, int N,
+                     uint64_t prime, int backend, PolyMulStats* stats) {
+  std::lock_guard<std::mutex> lk(mu());
+  double ms = 0;
+  check(pcgsq_poly_mul_u64(ctx(), out, a, b, batch, N, prime, backend, &ms));
+  if (stats) stats->device_ms = ms;
+}
+}  // namespace detail
+
+// v1 seam (poly_mul_cuda.h:10,15-17): served by the naive per-stage NTT.
+inline bool poly_mul_supported(uint64_t prime, int N, std::string* reason = nullptr) {
+  return detail::supported(prime, N, PCGSQ_NTT_NAIVE, reason);
+}
+inline void poly_mul_u64(uint64_t* out, const uint64_t* a, const uint64_t* b, int batch, int N,
+                         uint64_t prime, PolyMulStats* stats = nullptr) {
+  detail::poly_mul(out, a, b, batch, N, prime, PCGSQ_NTT_NAIVE, stats);
+}
+// four-step ("square") seam (poly_mul_cuda.h:24-29)
+inline bool poly_mul_gpuntt_square_supported(uint64_t prime, int N,
+                                             std::string* reason = nullptr) {
+  return detail::supported(prime, N, PCGSQ_NTT_FOURSTEP, reason);
+}
+inline void poly_mul_u64_gpuntt_square(uint64_t* out, const uint64_t* a, const uint64_t* b,
+                                       int batch, int N, uint64_t prime,
+                                       PolyMulStats* stats = nullptr) {
+  detail::poly_mul(out, a, b, batch, N, prime, PCGSQ_NTT_FOURSTEP, stats);
+}
+// The names pcg_ole.h:147-157,226-230 calls for its "merge" and "4step"
+// backends (declared by no reference header): both map to the four-step NTT.
+inline bool poly_mul_gpuntt_supported(uint64_t prime, int N, std::string* reason = nullptr) {
+  return poly_mul_gpuntt_square_supported(prime, N, reason);
+}
+inline void poly_mul_u64_gpuntt(uint64_t* out, const uint64_t* a, const uint64_t* b, int batch,
+                                int N, uint64_t prime, PolyMulStats* stats = nullptr) {
+  poly_mul_u64_gpuntt_square(out, a, b, batch, N, prime, stats);
+}
+inline bool poly_mul_gpuntt_4step_supported(uint64_t prime, int N,
+                                            std::string* reason = nullptr) {
+  return poly_mul_gpuntt_square_supported(prime, N, reason);
+}
+inline void poly_mul_u64_gpuntt_4step(uint64_t* out, const uint64_t* a, const uint64_t* b,
+                                      int batch, int N, uint64_t prime,
+                                      PolyMulStats* stats = nullptr) {
+  poly_mul_u64_gpuntt_square(out, a, b, batch, N, prime, stats);
+}
+
+// ---- common/aes_gpu.h ---------------------------------------------------------
+inline void aes128_expand_key_host(const uint8_t key[16], uint8_t round_keys[176]) {
+  detail::check(pcgsq_aes128_expand_key(key, round_keys));
+}
+inline void aes128_ecb_encrypt_gpu(const uint8_t* keys, const uint8_t* in, uint8_t* out,
+                                   int count) {
+  std::lock_guard<std::mutex> lk(detail::mu());
+  detail::check(pcgsq_aes128_ecb_encrypt(detail::ctx(), keys, in, out, count));
+}
+
+// ---- CPU_baseline/ntt.h: NTT<P> on the device ------------------------------------
+// Same interface as the reference's NTT<P> (ntt.h:23-90) for the project prime,
+// over raw residues (FFp<P> stores exactly one uint64 residue).
+template <uint64_t P>
+class NTT {
+  static_assert(P == PCGSQ_PRIME, "pcg_hip::NTT supports P = 2^62 - 3*2^25 + 1 only");
+
+ public:
+  explicit NTT(int N) : N_(N) {
+    std::string why;
+    if (!detail::supported(P, N, PCGSQ_NTT_FOURSTEP, &why))
+      throw std::runtime_error("pcg_hip::NTT: " + why);
+  }
+  void forward(uint64_t* a) const { run(a, false); }
+  void inverse(uint64_t* a) const { run(a, true); }
+  void poly_mul(uint64_t* c, const uint64_t* a, const uint64_t* b) const {
+    detail::poly_mul(c, a, b, 1, N_, P, PCGSQ_NTT_FOURSTEP, nullptr);
+  }
+  int N() const { return N_; }
+
+ private:
+  void run(uint64_t* a, bool inv) const {
+    std::lock_guard<std::mutex> lk(detail::mu());
+    detail::check(inv ? pcgsq_ntt_inverse(detail::ctx(), a, 1, N_, PCGSQ_NTT_FOURSTEP)
+                      : pcgsq_ntt_forward(detail::ctx(), a, 1, N_, PCGSQ_NTT_FOURSTEP));
+  }
+  int N_;
+};
+
+// ---- PCG-OLE expansion for both parties on one device ----------------------------
+// The local compute of pcg_ole::PCG_OLE<P>::gen_and_expand
+// (pcg_ole_impl.h:46-490) for both parties at once, with the 2PC
+// preprocessing replaced by the seeded dealer.
+class OleExpander {
+ public:
+  OleExpander(int N, int c, int t, int w = 1, DpfGpuPrg prg = DpfGpuPrg::CHACHA8,
+              uint64_t pub_seed = 0, size_t hbm_budget = 0)
+      : N_(N) {
+    pcgsq_params p{N, c, t, w, detail::prg_code(prg)};
+    std::lock_guard<std::mutex> lk(detail::mu());
+    detail::check(pcgsq_ole_plan_create(detail::ctx(), &p, pub_seed, hbm_budget, &plan_));
+  }
+  ~OleExpander() {
+    if (plan_) pcgsq_ole_plan_destroy(plan_);
+  }
+  OleExpander(const OleExpander&) = delete;
+  OleExpander& operator=(const OleExpander&) = delete;
+
+  struct Output {
+    std::vector<uint64_t> x0, z0, x1, z1;   // z0 + z1 = x0 * x1 in R_p
+  };
+  Output expand(uint64_t seed) {
+    Output o;
+    o.x0.resize(N_); o.z0.resize(N_); o.x1.resize(N_); o.z1.resize(N_);
+    std::lock_guard<std::mutex> lk(detail::mu());
+    detail::check(pcgsq_ole_deal(plan_, seed));
+    detail::check(pcgsq_ole_run(plan_));
+    detail::check(pcgsq_ole_outputs(plan_, o.x0.data(), o.z0.data(), o.x1.data(), o.z1.data()));
+    return o;
+  }
+
+ private:
+  int N_;
+  pcgsq_ole_plan* plan_ = nullptr;
+};
+
+}  // namespace pcg_hip
+
+#endif  // PCGSQ_PCG_HIP_HPP
