@@ -167,6 +167,13 @@ int pcgsq_ntt_forward_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
 int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
                           int backend);
 
+/* Shape of the four-step passes for length N on this context (no
+ * counterpart in the reference; set by the autotuner, pcgsq/autotune.py):
+ * column length 2^n1_log (0 = ceil(log2 N / 2)), radix = butterfly stages per
+ * register group (2..5), cols_max = sub-transforms per workgroup (power of
+ * two).  Results do not depend on the shape. */
+int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int radix, int cols_max);
+
 /* ---- AES (aes_gpu.h) -------------------------------------------------------- */
 /* pcg_cuda::aes128_expand_key_host (aes_gpu.h:13). */
 int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]);

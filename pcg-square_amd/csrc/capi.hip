@@ -569,6 +569,14 @@ int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N, int
   });
 }
 
+int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int radix, int cols_max) {
+  return guarded([&] {
+    require(ctx, "ntt_tune: null ctx");
+    CtxGuard g(ctx);
+    ctx->ntt.tune(N, n1_log, radix, cols_max);
+  });
+}
+
 int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]) {
   return guarded([&] {
     require(key && round_keys, "aes: null argument");

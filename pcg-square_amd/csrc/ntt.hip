@@ -48,20 +48,92 @@ struct PassArgs {
   int N_log;                // whole transform length (for the twiddle modulus 2N)
 };
 
-// LDS element (c, q) of a tile of C sub-transforms of length n: column c is
-// rotated by c words so that the C lanes that load one row segment (same q,
-// consecutive c) hit different banks, while butterfly accesses (consecutive
-// q in one column) stay contiguous.
-__device__ __forceinline__ int lidx(int c, int q, int n_log) {
-  return (c << n_log) + ((q + c) & ((1 << n_log) - 1));
+// LDS element (c, e) of a tile of C sub-transforms of length n: column c is
+// rotated by c words (the C lanes that load one row segment, same e and
+// consecutive c, hit different banks) and the low 4 index bits are XORed with
+// the next 4 (the 16 elements a thread gathers for one radix-16 set, which
+// are 16 apart in the last stage group, land in different banks).
+__device__ __forceinline__ int lidx(int c, int e, int n_log) {
+  const int x = (e + c) & ((1 << n_log) - 1);
+  return (c << n_log) + (x ^ ((x >> 4) & 15));
+}
+
+// One group of r consecutive butterfly stages [s0, s0 + r) of every
+// sub-transform in the tile, done in registers: each thread gathers sets of
+// 2^r elements spaced t_min = n >> (s0 + r) apart, runs r stages on them and
+// writes them back.  Forward: Cooley-Tukey with merged roots, twiddle
+// tab[2^s + i] on the upper input, values lazily reduced in [0, 4P).
+// Inverse: Gentleman-Sande in the reverse stage order, values in [0, 2P).
+template <bool INVERSE, int R>
+__device__ __forceinline__ void stage_group(uint64_t* __restrict__ lds,
+                                            const uint64_t* __restrict__ tab, int n_log, int C,
+                                            int s0) {
+  constexpr int M = 1 << R;
+  const int tmin_log = n_log - (s0 + R);
+  const int sets_log = n_log - R;
+  const int nsets = C << sets_log;
+  for (int gs = threadIdx.x; gs < nsets; gs += blockDim.x) {
+    const int c = gs >> sets_log;
+    const int sigma = gs & ((1 << sets_log) - 1);
+    const int L = sigma & ((1 << tmin_log) - 1);
+    const int Hh = sigma >> tmin_log;
+    const int base = (Hh << (tmin_log + R)) | L;
+    uint64_t v[M];
+#pragma unroll
+    for (int w = 0; w < M; ++w) v[w] = lds[lidx(c, base | (w << tmin_log), n_log)];
+    if constexpr (!INVERSE) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int bit = R - 1 - q;
+#pragma unroll
+        for (int w = 0; w < M; ++w) {
+          if ((w >> bit) & 1) continue;
+          const int w2 = w | (1 << bit);
+          const int ti = 2 * ((1 << (s0 + q)) + ((Hh << q) | (w >> (R - q))));
+          const uint64_t U = red2p(v[w]);
+          const uint64_t T = shoup_lazy(v[w2], tab[ti], tab[ti + 1]);
+          v[w] = U + T;
+          v[w2] = U - T + kTwoP;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = R - 1; q >= 0; --q) {
+        const int bit = R - 1 - q;
+#pragma unroll
+        for (int w = 0; w < M; ++w) {
+          if ((w >> bit) & 1) continue;
+          const int w2 = w | (1 << bit);
+          const int ti = 2 * ((1 << (s0 + q)) + ((Hh << q) | (w >> (R - q))));
+          const uint64_t U = v[w], V = v[w2];   // both < 2P
+          v[w] = red2p(U + V);
+          v[w2] = shoup_lazy(U - V + kTwoP, tab[ti], tab[ti + 1]);
+        }
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < M; ++w) lds[lidx(c, base | (w << tmin_log), n_log)] = v[w];
+  }
+}
+
+template <bool INVERSE, int R>
+__device__ __forceinline__ void run_group(uint64_t* lds, const uint64_t* tab, int n_log, int C,
+                                          int s0, int r) {
+  // r <= R: the last group of a length that R does not divide
+  if constexpr (R >= 1) {
+    if (r == R) {
+      stage_group<INVERSE, R>(lds, tab, n_log, C, s0);
+      return;
+    }
+    if constexpr (R > 1) run_group<INVERSE, R - 1>(lds, tab, n_log, C, s0, r);
+  }
 }
 
 // One pass of the four-step transform: C whole sub-transforms of length n are
-// loaded into LDS, transformed with radix-2 butterflies (Cooley-Tukey with
-// merged roots forward, Gentleman-Sande inverse), optionally multiplied by a
-// twiddle and written back with arbitrary strides (which is where the
-// transposition of the four-step happens).
-template <bool INVERSE, bool TWIDDLE, bool PRODUCT>
+// loaded into LDS, transformed in groups of R stages in registers, optionally
+// multiplied by a twiddle and written back with arbitrary strides (which is
+// where the transposition of the four-step happens).
+template <bool INVERSE, bool TWIDDLE, bool PRODUCT, int R>
 __global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(PassArgs a) {
   extern __shared__ uint64_t lds[];
   const int n = 1 << a.n_log;
@@ -74,7 +146,7 @@ __global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(PassArgs a) {
   const int tile = n * C;
 
   // ---- load (coalesced along whichever stride is 1) ----
-  for (int e = threadIdx.x; e < tile; e += kPassThreads) {
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
     int c, p;
     if (a.in_cs == 1) { c = e % C; p = e / C; } else { c = e / n; p = e % n; }
     const size_t addr = (size_t)p * a.in_ps + (size_t)(c0 + c) * a.in_cs;
@@ -85,43 +157,19 @@ __global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(PassArgs a) {
   }
   __syncthreads();
 
-  const int half = n >> 1;
-  const int nbf = C * half;
+  const int ngroups = (a.n_log + R - 1) / R;
   if constexpr (!INVERSE) {
-    for (int s = 0; s < a.n_log; ++s) {
-      const int t_log = a.n_log - 1 - s;
-      const int t = 1 << t_log;
-      const int m = 1 << s;
-      for (int b = threadIdx.x; b < nbf; b += kPassThreads) {
-        const int c = b >> (a.n_log - 1);
-        const int k = b & (half - 1);
-        const int i = k >> t_log, j = k & (t - 1);
-        const int j1 = (i << (t_log + 1)) + j;
-        const int x0 = lidx(c, j1, a.n_log), x1 = lidx(c, j1 + t, a.n_log);
-        const uint64_t U = red2p(lds[x0]);
-        const uint64_t W = a.tab[2 * (m + i)], Wq = a.tab[2 * (m + i) + 1];
-        const uint64_t T = shoup_lazy(lds[x1], W, Wq);
-        lds[x0] = U + T;
-        lds[x1] = U - T + kTwoP;
-      }
+    for (int gi = 0; gi < ngroups; ++gi) {
+      const int s0 = gi * R;
+      const int r = a.n_log - s0 < R ? a.n_log - s0 : R;
+      run_group<false, R>(lds, a.tab, a.n_log, C, s0, r);
       __syncthreads();
     }
   } else {
-    for (int s = a.n_log - 1; s >= 0; --s) {
-      const int t_log = a.n_log - 1 - s;
-      const int t = 1 << t_log;
-      const int m = 1 << s;
-      for (int b = threadIdx.x; b < nbf; b += kPassThreads) {
-        const int c = b >> (a.n_log - 1);
-        const int k = b & (half - 1);
-        const int i = k >> t_log, j = k & (t - 1);
-        const int j1 = (i << (t_log + 1)) + j;
-        const int x0 = lidx(c, j1, a.n_log), x1 = lidx(c, j1 + t, a.n_log);
-        const uint64_t U = lds[x0], V = lds[x1];   // both < 2P
-        const uint64_t W = a.tab[2 * (m + i)], Wq = a.tab[2 * (m + i) + 1];
-        lds[x0] = red2p(U + V);
-        lds[x1] = shoup_lazy(U - V + kTwoP, W, Wq);
-      }
+    for (int gi = ngroups - 1; gi >= 0; --gi) {
+      const int s0 = gi * R;
+      const int r = a.n_log - s0 < R ? a.n_log - s0 : R;
+      run_group<true, R>(lds, a.tab, a.n_log, C, s0, r);
       __syncthreads();
     }
   }
@@ -129,7 +177,7 @@ __global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(PassArgs a) {
   // ---- store ----
   const uint64_t two_n_mask = (2ULL << a.N_log) - 1;
   const uint64_t lo_mask = (1ULL << a.tw_lo_bits) - 1;
-  for (int e = threadIdx.x; e < tile; e += kPassThreads) {
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
     int c, p;
     if (a.out_cs == 1) { c = e % C; p = e / C; } else { c = e / n; p = e % n; }
     const int q = a.out_brv ? brv_dev(p, a.n_log) : p;
@@ -222,16 +270,18 @@ unsigned grid_for(size_t n) { return (unsigned)((n + kEwThreads - 1) / kEwThread
 
 struct NttPlan {
   int N = 0, L = 0, n1_log = 0, n2_log = 0, tw_lo_bits = 0;
+  int radix = 4;       // butterfly stages per register group in a pass
+  int cols_max = 16;   // sub-transforms per workgroup (LDS tile width)
   uint64_t psi = 0;
   DevBuf n1_neg_fwd, n1_neg_inv, n2_cyc_fwd, n2_cyc_inv;
   DevBuf tw_fwd_lo, tw_fwd_hi, tw_inv_lo, tw_inv_hi;
   bool naive_ready = false;
   DevBuf nv_w, nv_winv, nv_twf, nv_twi;
 
-  explicit NttPlan(int N_) : N(N_) {
+  NttPlan(int N_, int n1_log_) : N(N_) {
     L = ilog2_exact((uint64_t)N);
-    n1_log = (L + 1) / 2;
-    n2_log = L / 2;
+    n1_log = n1_log_ > 0 ? n1_log_ : (L + 1) / 2;
+    n2_log = L - n1_log;
     psi = hpow(3, (kPrime - 1) / (2 * (uint64_t)N));
     const uint64_t psi_inv = hinv(psi);
     const int n1 = 1 << n1_log, n2 = 1 << n2_log;
@@ -327,8 +377,23 @@ bool NttEngine::supported(int N, int backend, std::string* reason) {
 NttPlan& NttEngine::plan(int N) {
   std::lock_guard<std::mutex> lk(impl_->mu);
   auto& p = impl_->plans[N];
-  if (!p) p.reset(new NttPlan(N));
+  if (!p) p.reset(new NttPlan(N, 0));
   return *p;
+}
+
+void NttEngine::tune(int N, int n1_log, int radix, int cols_max) {
+  const int L = ilog2_exact((uint64_t)N);
+  require(L >= 2 && L <= 24, "ntt tune: N must be 2^2..2^24");
+  require(n1_log == 0 || (n1_log >= 1 && n1_log < L), "ntt tune: bad split");
+  require(radix >= 2 && radix <= 5, "ntt tune: radix must be 2..5");
+  require(cols_max >= 1 && cols_max <= 64 && (cols_max & (cols_max - 1)) == 0,
+          "ntt tune: cols must be a power of two <= 64");
+  std::lock_guard<std::mutex> lk(impl_->mu);
+  auto& p = impl_->plans[N];
+  const int want = n1_log > 0 ? n1_log : (L + 1) / 2;
+  if (!p || p->n1_log != want) p.reset(new NttPlan(N, want));
+  p->radix = radix;
+  p->cols_max = cols_max;
 }
 
 uint64_t* NttEngine::scratch(size_t words, int which) {
@@ -352,29 +417,39 @@ size_t lds_budget() {
   return g_lds_budget;
 }
 
-int cols_per_wg(int n_log, int ncols) {
-  int C = 16;
+int cols_per_wg(int n_log, int ncols, int cols_max) {
+  int C = cols_max;
   while (C > 1 && (((size_t)C << n_log) * sizeof(uint64_t)) > lds_budget()) C >>= 1;
   while (C > ncols) C >>= 1;
   return C;
 }
 
+template <bool INV, bool TW, bool PROD, int R>
+void launch_pass_r(hipStream_t st, const PassArgs& a, int batch, size_t lds) {
+  static bool attr_set = false;
+  if (!attr_set && lds > 65536) {
+    hip_check(hipFuncSetAttribute((const void*)ntt_pass_kernel<INV, TW, PROD, R>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget()),
+              "ntt: raise LDS limit");
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((ntt_pass_kernel<INV, TW, PROD, R>), dim3(a.ncols / a.C, batch),
+                     dim3(kPassThreads), lds, st, a);
+}
+
 template <bool INV, bool TW, bool PROD>
-void launch_pass(hipStream_t st, PassArgs a, int batch, Profiler* prof) {
+void launch_pass(hipStream_t st, PassArgs a, int batch, Profiler* prof, int radix) {
   ProfZone pz(prof, st, INV ? (TW ? "ntt.inv_pass1" : "ntt.inv_pass2")
                                   : (TW ? "ntt.fwd_pass1" : "ntt.fwd_pass2"));
   const size_t lds = ((size_t)a.C << a.n_log) * sizeof(uint64_t);
   require(a.ncols % a.C == 0, "ntt: tile does not divide the columns");
   require(lds <= lds_budget(), "ntt: tile exceeds LDS");
-  static bool attr_set = false;
-  if (!attr_set && lds > 65536) {
-    hip_check(hipFuncSetAttribute((const void*)ntt_pass_kernel<INV, TW, PROD>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget()),
-              "ntt: raise LDS limit");
-    attr_set = true;
+  switch (radix) {
+    case 2: launch_pass_r<INV, TW, PROD, 2>(st, a, batch, lds); break;
+    case 3: launch_pass_r<INV, TW, PROD, 3>(st, a, batch, lds); break;
+    case 5: launch_pass_r<INV, TW, PROD, 5>(st, a, batch, lds); break;
+    default: launch_pass_r<INV, TW, PROD, 4>(st, a, batch, lds); break;
   }
-  hipLaunchKernelGGL((ntt_pass_kernel<INV, TW, PROD>), dim3(a.ncols / a.C, batch),
-                     dim3(kPassThreads), lds, st, a);
   hip_check(hipGetLastError(), "launch ntt_pass_kernel");
 }
 
@@ -387,18 +462,18 @@ void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t
   a.tab = pl.n1_neg_fwd.as<uint64_t>();
   a.tw_lo = pl.tw_fwd_lo.as<uint64_t>(); a.tw_hi = pl.tw_fwd_hi.as<uint64_t>();
   a.tw_lo_bits = pl.tw_lo_bits; a.tw_swap = 0;
-  a.n_log = pl.n1_log; a.ncols = n2; a.C = cols_per_wg(pl.n1_log, n2);
+  a.n_log = pl.n1_log; a.ncols = n2; a.C = cols_per_wg(pl.n1_log, n2, pl.cols_max);
   a.in_ps = n2; a.in_cs = 1; a.out_ps = n2; a.out_cs = 1;
   a.in_brv = 0; a.out_brv = 1; a.N_log = pl.L;
-  launch_pass<false, true, false>(st, a, batch, prof);
+  launch_pass<false, true, false>(st, a, batch, prof, pl.radix);
   // pass 2: rows k1, length-n2 cyclic over i2, written transposed (natural k)
   PassArgs b{};
   b.in = tmp; b.out = out;
   b.tab = pl.n2_cyc_fwd.as<uint64_t>();
-  b.n_log = pl.n2_log; b.ncols = n1; b.C = cols_per_wg(pl.n2_log, n1);
+  b.n_log = pl.n2_log; b.ncols = n1; b.C = cols_per_wg(pl.n2_log, n1, pl.cols_max);
   b.in_ps = 1; b.in_cs = n2; b.out_ps = n1; b.out_cs = 1;
   b.in_brv = 0; b.out_brv = 1; b.N_log = pl.L;
-  launch_pass<false, false, false>(st, b, batch, prof);
+  launch_pass<false, false, false>(st, b, batch, prof, pl.radix);
 }
 
 void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const uint64_t* in2,
@@ -411,19 +486,19 @@ void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const ui
   a.tab = pl.n2_cyc_inv.as<uint64_t>();
   a.tw_lo = pl.tw_inv_lo.as<uint64_t>(); a.tw_hi = pl.tw_inv_hi.as<uint64_t>();
   a.tw_lo_bits = pl.tw_lo_bits; a.tw_swap = 1;
-  a.n_log = pl.n2_log; a.ncols = n1; a.C = cols_per_wg(pl.n2_log, n1);
+  a.n_log = pl.n2_log; a.ncols = n1; a.C = cols_per_wg(pl.n2_log, n1, pl.cols_max);
   a.in_ps = n1; a.in_cs = 1; a.out_ps = 1; a.out_cs = n2;
   a.in_brv = 1; a.out_brv = 0; a.N_log = pl.L;
-  if (in2) launch_pass<true, true, true>(st, a, batch, prof);
-  else launch_pass<true, true, false>(st, a, batch, prof);
+  if (in2) launch_pass<true, true, true>(st, a, batch, prof, pl.radix);
+  else launch_pass<true, true, false>(st, a, batch, prof, pl.radix);
   // pass I2: columns i2, inverse negacyclic over k1
   PassArgs b{};
   b.in = tmp; b.out = out;
   b.tab = pl.n1_neg_inv.as<uint64_t>();
-  b.n_log = pl.n1_log; b.ncols = n2; b.C = cols_per_wg(pl.n1_log, n2);
+  b.n_log = pl.n1_log; b.ncols = n2; b.C = cols_per_wg(pl.n1_log, n2, pl.cols_max);
   b.in_ps = n2; b.in_cs = 1; b.out_ps = n2; b.out_cs = 1;
   b.in_brv = 1; b.out_brv = 0; b.N_log = pl.L;
-  launch_pass<true, false, false>(st, b, batch, prof);
+  launch_pass<true, false, false>(st, b, batch, prof, pl.radix);
 }
 
 void naive_forward_brv(hipStream_t st, NttPlan& pl, uint64_t* d, int batch) {

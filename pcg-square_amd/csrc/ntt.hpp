@@ -49,6 +49,10 @@ class NttEngine {
   // Scratch for a batch of transforms (grows on demand).
   uint64_t* scratch(size_t words, int which);
   NttPlan& plan(int N);
+  // Shape of the four-step passes for length N (chosen by the autotuner):
+  // n1 = 2^n1_log column length (0 = ceil(log2 N / 2)), radix = stages per
+  // register group, cols_max = sub-transforms per workgroup.
+  void tune(int N, int n1_log, int radix, int cols_max);
   Profiler* prof = nullptr;  // optional per-launch timing
 
  private:

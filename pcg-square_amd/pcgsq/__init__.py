@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
         getattr(L, f).argtypes = [vp, u64p, C.c_int, C.c_int, C.c_int]
     for f in ("pcgsq_ntt_forward_dev", "pcgsq_ntt_inverse_dev"):
         getattr(L, f).argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
+    L.pcgsq_ntt_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.pcgsq_aes128_expand_key.argtypes = [u8p, u8p]
     L.pcgsq_aes128_ecb_encrypt.argtypes = [vp, u8p, u8p, u8p, C.c_int]
     L.pcgsq_aes_tree_round_keys.argtypes = [u8p, C.c_int, u8p]
@@ -146,10 +147,13 @@ class Context:
     """One device + stream + owned scratch (replaces the reference's
     process-global grow-only pools, dpf_gpu.cu:42-75)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, apply_tuning: bool = True):
         h = vp()
         _check(lib().pcgsq_ctx_create(device, C.byref(h)))
         self.h = h
+        if apply_tuning:
+            from . import autotune
+            autotune.apply(self)
 
     def close(self) -> None:
         if self.h:
@@ -303,6 +307,10 @@ class Context:
         x2 = x.reshape(-1, x.shape[-1])
         _check(lib().pcgsq_ntt_inverse(self.h, _p(x2), x2.shape[0], x2.shape[1], backend))
         return x
+
+    def ntt_tune(self, N: int, n1_log: int = 0, radix: int = 4, cols_max: int = 16) -> None:
+        """Shape of the four-step passes for length N (results do not change)."""
+        _check(lib().pcgsq_ntt_tune(self.h, N, n1_log, radix, cols_max))
 
     def poly_mul_dev(self, out_ptr: int, a_ptr: int, b_ptr: int, batch: int, N: int,
                      backend=NTT_FOURSTEP):
