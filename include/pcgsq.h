@@ -167,12 +167,14 @@ int pcgsq_ntt_forward_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
 int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
                           int backend);
 
-/* Shape of the four-step passes for length N on this context (no
- * counterpart in the reference; set by the autotuner, pcgsq/autotune.py):
- * column length 2^n1_log (0 = ceil(log2 N / 2)), radix = butterfly stages per
- * register group (2..5), cols_max = sub-transforms per workgroup (power of
- * two).  Results do not depend on the shape. */
-int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int radix, int cols_max);
+/* Shape of the NTT passes for length N on this context (no counterpart in the
+ * reference; set by the autotuner, pcgsq/autotune.py): column length
+ * 2^n1_log (n1_log = log2 N: one pass, allowed up to N = 2^13; 0: default,
+ * one pass up to 2^13 and ceil(log2 N / 2) above), variant bit 0 / bit 1 =
+ * alternative register grouping of the length-n1 / length-n2 passes, cols1 /
+ * cols2 = sub-transforms per workgroup tile of those passes (power of two,
+ * 0 = automatic).  Results do not depend on the shape. */
+int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int variant, int cols1, int cols2);
 
 /* ---- AES (aes_gpu.h) -------------------------------------------------------- */
 /* pcg_cuda::aes128_expand_key_host (aes_gpu.h:13). */

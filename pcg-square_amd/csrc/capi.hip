@@ -569,11 +569,11 @@ int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N, int
   });
 }
 
-int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int radix, int cols_max) {
+int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int variant, int cols1, int cols2) {
   return guarded([&] {
     require(ctx, "ntt_tune: null ctx");
     CtxGuard g(ctx);
-    ctx->ntt.tune(N, n1_log, radix, cols_max);
+    ctx->ntt.tune(N, n1_log, variant, cols1, cols2);
   });
 }
 

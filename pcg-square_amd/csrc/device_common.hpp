@@ -43,16 +43,14 @@ PCGSQ_HD uint64_t freduce64(uint64_t x) {
   return r >= kP ? r - kP : r;
 }
 
-// High 64 bits of a 64x64 product from four 32x32->64 partial products.
+// High 64 bits of a 64x64 product as a chain of 32x32+64 multiply-adds
+// (v_mad_u64_u32): t and u never overflow 64 bits.
 PCGSQ_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
   const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32);
   const uint32_t bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
-  const uint64_t ll = (uint64_t)al * bl;
-  const uint64_t lh = (uint64_t)al * bh;
-  const uint64_t hl = (uint64_t)ah * bl;
-  const uint64_t hh = (uint64_t)ah * bh;
-  const uint64_t mid = (ll >> 32) + (uint32_t)lh + (uint32_t)hl;  // < 3*2^32
-  return hh + (lh >> 32) + (hl >> 32) + (mid >> 32);
+  const uint64_t t = (uint64_t)ah * bl + (((uint64_t)al * bl) >> 32);
+  const uint64_t u = (uint64_t)al * bh + (uint32_t)t;
+  return (uint64_t)ah * bh + ((t >> 32) + (u >> 32));
 }
 
 // q*P mod 2^64 using the shape of P (shifts and adds only).
@@ -61,10 +59,10 @@ PCGSQ_HD uint64_t mulP(uint64_t q) {
 }
 
 // Shoup: x*w mod P given wq = floor(w * 2^64 / P), w < P.  Any x < 2^64;
-// result in [0, 2P).
+// result in [0, 2P).  x*w - q*P is formed as (x*w + 3q*2^25) - (q + q*2^62).
 PCGSQ_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wq) {
   const uint64_t q = mulhi64(x, wq);
-  return x * w - mulP(q);
+  return (x * w + ((q * 3) << 25)) - (q + (q << 62));
 }
 PCGSQ_HD uint64_t shoup(uint64_t x, uint64_t w, uint64_t wq) {
   uint64_t r = shoup_lazy(x, w, wq);

@@ -6,12 +6,13 @@
 
 #include "device_common.hpp"
 #include "ntt.hpp"
+#include "ntt_pass.hpp"
 
 namespace pcgsq {
 
 namespace {
 
-constexpr int kPassThreads = 1024;
+constexpr int kPassThreads = 512;
 constexpr int kEwThreads = 256;
 
 inline int brv_bits(uint64_t x, int bits) {
@@ -24,173 +25,15 @@ __device__ __forceinline__ int brv_dev(int x, int bits) {
   return (int)(__brev((unsigned)x) >> (32 - bits));
 }
 
-// Lazy-reduction helpers (Harvey): values live in [0, 4P) (< 2^64).
-__device__ __forceinline__ uint64_t red2p(uint64_t x) { return x >= kTwoP ? x - kTwoP : x; }
-__device__ __forceinline__ uint64_t red_full(uint64_t x) {  // [0,4P) -> [0,P)
-  x = x >= kTwoP ? x - kTwoP : x;
-  return x >= kP ? x - kP : x;
-}
-
-struct PassArgs {
-  const uint64_t* in;
-  const uint64_t* in2;      // product pass: second factor (same layout as in)
-  uint64_t* out;
-  const uint64_t* tab;      // sub-transform table: [n][2] = (w, w')
-  const uint64_t* tw_lo;    // two-level twiddle tables [.][2] (or null)
-  const uint64_t* tw_hi;
-  int tw_lo_bits;
-  int tw_swap;              // 0: e = c*(2p+1); 1: e = p*(2c+1)
-  int n_log;                // sub-transform length n = 2^n_log
-  int C;                    // sub-transforms per workgroup
-  int ncols;                // sub-transforms per transform
-  size_t in_ps, in_cs, out_ps, out_cs;  // element strides (position, column)
-  int in_brv, out_brv;
-  int N_log;                // whole transform length (for the twiddle modulus 2N)
-};
-
-// LDS element (c, e) of a tile of C sub-transforms of length n: column c is
-// rotated by c words (the C lanes that load one row segment, same e and
-// consecutive c, hit different banks) and the low 4 index bits are XORed with
-// the next 4 (the 16 elements a thread gathers for one radix-16 set, which
-// are 16 apart in the last stage group, land in different banks).
-__device__ __forceinline__ int lidx(int c, int e, int n_log) {
-  const int x = (e + c) & ((1 << n_log) - 1);
-  return (c << n_log) + (x ^ ((x >> 4) & 15));
-}
-
-// One group of r consecutive butterfly stages [s0, s0 + r) of every
-// sub-transform in the tile, done in registers: each thread gathers sets of
-// 2^r elements spaced t_min = n >> (s0 + r) apart, runs r stages on them and
-// writes them back.  Forward: Cooley-Tukey with merged roots, twiddle
-// tab[2^s + i] on the upper input, values lazily reduced in [0, 4P).
-// Inverse: Gentleman-Sande in the reverse stage order, values in [0, 2P).
-template <bool INVERSE, int R>
-__device__ __forceinline__ void stage_group(uint64_t* __restrict__ lds,
-                                            const uint64_t* __restrict__ tab, int n_log, int C,
-                                            int s0) {
-  constexpr int M = 1 << R;
-  const int tmin_log = n_log - (s0 + R);
-  const int sets_log = n_log - R;
-  const int nsets = C << sets_log;
-  for (int gs = threadIdx.x; gs < nsets; gs += blockDim.x) {
-    const int c = gs >> sets_log;
-    const int sigma = gs & ((1 << sets_log) - 1);
-    const int L = sigma & ((1 << tmin_log) - 1);
-    const int Hh = sigma >> tmin_log;
-    const int base = (Hh << (tmin_log + R)) | L;
-    uint64_t v[M];
-#pragma unroll
-    for (int w = 0; w < M; ++w) v[w] = lds[lidx(c, base | (w << tmin_log), n_log)];
-    if constexpr (!INVERSE) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int bit = R - 1 - q;
-#pragma unroll
-        for (int w = 0; w < M; ++w) {
-          if ((w >> bit) & 1) continue;
-          const int w2 = w | (1 << bit);
-          const int ti = 2 * ((1 << (s0 + q)) + ((Hh << q) | (w >> (R - q))));
-          const uint64_t U = red2p(v[w]);
-          const uint64_t T = shoup_lazy(v[w2], tab[ti], tab[ti + 1]);
-          v[w] = U + T;
-          v[w2] = U - T + kTwoP;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int q = R - 1; q >= 0; --q) {
-        const int bit = R - 1 - q;
-#pragma unroll
-        for (int w = 0; w < M; ++w) {
-          if ((w >> bit) & 1) continue;
-          const int w2 = w | (1 << bit);
-          const int ti = 2 * ((1 << (s0 + q)) + ((Hh << q) | (w >> (R - q))));
-          const uint64_t U = v[w], V = v[w2];   // both < 2P
-          v[w] = red2p(U + V);
-          v[w2] = shoup_lazy(U - V + kTwoP, tab[ti], tab[ti + 1]);
-        }
-      }
-    }
-#pragma unroll
-    for (int w = 0; w < M; ++w) lds[lidx(c, base | (w << tmin_log), n_log)] = v[w];
-  }
-}
-
-template <bool INVERSE, int R>
-__device__ __forceinline__ void run_group(uint64_t* lds, const uint64_t* tab, int n_log, int C,
-                                          int s0, int r) {
-  // r <= R: the last group of a length that R does not divide
-  if constexpr (R >= 1) {
-    if (r == R) {
-      stage_group<INVERSE, R>(lds, tab, n_log, C, s0);
-      return;
-    }
-    if constexpr (R > 1) run_group<INVERSE, R - 1>(lds, tab, n_log, C, s0, r);
-  }
-}
-
-// One pass of the four-step transform: C whole sub-transforms of length n are
-// loaded into LDS, transformed in groups of R stages in registers, optionally
-// multiplied by a twiddle and written back with arbitrary strides (which is
-// where the transposition of the four-step happens).
-template <bool INVERSE, bool TWIDDLE, bool PRODUCT, int R>
-__global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(PassArgs a) {
+// One four-step (or single-pass) NTT pass as a grid of tiles.  Tiles of one
+// transform are dealt to the eight XCDs in contiguous runs, so tiles that
+// share 128-byte lines (C < 16 columns of 8 bytes) run under one L2.
+template <int K, int RA, int RB, int RC>
+__global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(NttPassDesc d, uint32_t tiles) {
   extern __shared__ uint64_t lds[];
-  const int n = 1 << a.n_log;
-  const int C = a.C;
-  const size_t batch_off = (size_t)blockIdx.y << a.N_log;
-  const int c0 = blockIdx.x * C;
-  const uint64_t* in = a.in + batch_off;
-  const uint64_t* in2 = PRODUCT ? a.in2 + batch_off : nullptr;
-  uint64_t* out = a.out + batch_off;
-  const int tile = n * C;
-
-  // ---- load (coalesced along whichever stride is 1) ----
-  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int c, p;
-    if (a.in_cs == 1) { c = e % C; p = e / C; } else { c = e / n; p = e % n; }
-    const size_t addr = (size_t)p * a.in_ps + (size_t)(c0 + c) * a.in_cs;
-    uint64_t v = in[addr];
-    if constexpr (PRODUCT) v = fmul(v, in2[addr]);
-    const int q = a.in_brv ? brv_dev(p, a.n_log) : p;
-    lds[lidx(c, q, a.n_log)] = v;
-  }
-  __syncthreads();
-
-  const int ngroups = (a.n_log + R - 1) / R;
-  if constexpr (!INVERSE) {
-    for (int gi = 0; gi < ngroups; ++gi) {
-      const int s0 = gi * R;
-      const int r = a.n_log - s0 < R ? a.n_log - s0 : R;
-      run_group<false, R>(lds, a.tab, a.n_log, C, s0, r);
-      __syncthreads();
-    }
-  } else {
-    for (int gi = ngroups - 1; gi >= 0; --gi) {
-      const int s0 = gi * R;
-      const int r = a.n_log - s0 < R ? a.n_log - s0 : R;
-      run_group<true, R>(lds, a.tab, a.n_log, C, s0, r);
-      __syncthreads();
-    }
-  }
-
-  // ---- store ----
-  const uint64_t two_n_mask = (2ULL << a.N_log) - 1;
-  const uint64_t lo_mask = (1ULL << a.tw_lo_bits) - 1;
-  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int c, p;
-    if (a.out_cs == 1) { c = e % C; p = e / C; } else { c = e / n; p = e % n; }
-    const int q = a.out_brv ? brv_dev(p, a.n_log) : p;
-    uint64_t v = lds[lidx(c, q, a.n_log)];
-    if constexpr (TWIDDLE) {
-      const uint64_t cc = (uint64_t)(c0 + c), pp = (uint64_t)p;
-      const uint64_t ex = (a.tw_swap ? pp * (2 * cc + 1) : cc * (2 * pp + 1)) & two_n_mask;
-      const uint64_t xl = ex & lo_mask, xh = ex >> a.tw_lo_bits;
-      v = shoup_lazy(v, a.tw_lo[2 * xl], a.tw_lo[2 * xl + 1]);
-      v = shoup_lazy(v, a.tw_hi[2 * xh], a.tw_hi[2 * xh + 1]);
-    }
-    out[(size_t)p * a.out_ps + (size_t)(c0 + c) * a.out_cs] = red_full(v);
-  }
+  uint32_t bid = blockIdx.x;
+  if ((tiles & 7) == 0) bid = (bid & 7) * (tiles >> 3) + (bid >> 3);
+  nttpass::pass_tile<K, RA, RB, RC>(d, lds, blockIdx.y, bid);
 }
 
 // ---- naive per-stage kernels ------------------------------------------------
@@ -270,20 +113,26 @@ unsigned grid_for(size_t n) { return (unsigned)((n + kEwThreads - 1) / kEwThread
 
 struct NttPlan {
   int N = 0, L = 0, n1_log = 0, n2_log = 0, tw_lo_bits = 0;
-  int radix = 4;       // butterfly stages per register group in a pass
-  int cols_max = 16;   // sub-transforms per workgroup (LDS tile width)
-  uint64_t psi = 0;
+  int variant = 0;           // bit 0: alternative stage grouping of the n1 passes, bit 1: n2
+  int cols1 = 0, cols2 = 0;  // columns per tile of the n1 / n2 passes (0 = automatic)
+  uint64_t psi = 0, ninv = 0, ninv_q = 0;
   DevBuf n1_neg_fwd, n1_neg_inv, n2_cyc_fwd, n2_cyc_inv;
   DevBuf tw_fwd_lo, tw_fwd_hi, tw_inv_lo, tw_inv_hi;
   bool naive_ready = false;
   DevBuf nv_w, nv_winv, nv_twf, nv_twi;
 
+  bool single() const { return n2_log == 0; }
+
+  // n1_log == L: one pass (whole transform in one tile); otherwise the
+  // four-step split N = n1 * n2.
   NttPlan(int N_, int n1_log_) : N(N_) {
     L = ilog2_exact((uint64_t)N);
-    n1_log = n1_log_ > 0 ? n1_log_ : (L + 1) / 2;
+    n1_log = n1_log_;
     n2_log = L - n1_log;
     psi = hpow(3, (kPrime - 1) / (2 * (uint64_t)N));
     const uint64_t psi_inv = hinv(psi);
+    ninv = hinv((uint64_t)N);
+    ninv_q = shoup_companion(ninv);
     const int n1 = 1 << n1_log, n2 = 1 << n2_log;
     // length-n1 negacyclic sub-transform, root psi1 = psi^n2 (order 2*n1)
     const uint64_t psi1 = hpow(psi, (uint64_t)n2), psi1_inv = hinv(psi1);
@@ -294,6 +143,7 @@ struct NttPlan {
     }
     upload(n1_neg_fwd, with_companions(f), "ntt tables");
     upload(n1_neg_inv, with_companions(fi), "ntt tables");
+    if (single()) return;
     // length-n2 cyclic sub-transform, root w2 = psi^(2 n1) (order n2)
     const uint64_t w2 = hpow(psi, 2 * (uint64_t)n1), w2_inv = hinv(w2);
     std::vector<uint64_t> g(n2, 0), gi(n2, 0);
@@ -312,7 +162,6 @@ struct NttPlan {
     const int hi_bits = L + 1 - tw_lo_bits;
     std::vector<uint64_t> lo(1 << tw_lo_bits), hi(1 << hi_bits), loi(1 << tw_lo_bits),
         hii(1 << hi_bits);
-    const uint64_t ninv = hinv((uint64_t)N);
     for (size_t x = 0; x < lo.size(); ++x) {
       lo[x] = hpow(psi, x);
       loi[x] = hpow(psi_inv, x);
@@ -330,7 +179,7 @@ struct NttPlan {
   void ensure_naive() {
     if (naive_ready) return;
     const uint64_t omega = hmul(psi, psi), omega_inv = hinv(omega);
-    const uint64_t psi_inv = hinv(psi), ninv = hinv((uint64_t)N);
+    const uint64_t psi_inv = hinv(psi);
     std::vector<uint64_t> w(std::max(N / 2, 1)), wi(std::max(N / 2, 1)), tf(N), ti(N);
     w[0] = wi[0] = 1;
     for (int k = 1; k < N / 2; ++k) {
@@ -368,32 +217,37 @@ bool NttEngine::supported(int N, int backend, std::string* reason) {
   if (N <= 0 || (N & (N - 1))) return fail("N must be a power of two");
   const int L = ilog2_exact((uint64_t)N);
   if (L > 24) return fail("N > 2^24: 2N does not divide P-1");
-  if (backend == kNttFourStep && L < 2) return fail("four-step needs N >= 4");
   if (backend != kNttFourStep && backend != kNttNaive) return fail("unknown backend");
-  if (backend == kNttNaive && L < 1) return fail("naive needs N >= 2");
+  if (L < 1) return fail("N must be at least 2");
   return true;
 }
+
+int NttEngine::default_split(int L) { return L <= kNttSingleMax ? L : (L + 1) / 2; }
 
 NttPlan& NttEngine::plan(int N) {
   std::lock_guard<std::mutex> lk(impl_->mu);
   auto& p = impl_->plans[N];
-  if (!p) p.reset(new NttPlan(N, 0));
+  if (!p) p.reset(new NttPlan(N, default_split(ilog2_exact((uint64_t)N))));
   return *p;
 }
 
-void NttEngine::tune(int N, int n1_log, int radix, int cols_max) {
+void NttEngine::tune(int N, int n1_log, int variant, int cols1, int cols2) {
   const int L = ilog2_exact((uint64_t)N);
-  require(L >= 2 && L <= 24, "ntt tune: N must be 2^2..2^24");
-  require(n1_log == 0 || (n1_log >= 1 && n1_log < L), "ntt tune: bad split");
-  require(radix >= 2 && radix <= 5, "ntt tune: radix must be 2..5");
-  require(cols_max >= 1 && cols_max <= 64 && (cols_max & (cols_max - 1)) == 0,
-          "ntt tune: cols must be a power of two <= 64");
+  require(L >= 1 && L <= 24, "ntt tune: N must be 2^1..2^24");
+  const int want = n1_log > 0 ? n1_log : default_split(L);
+  require(want == L ? L <= kNttSingleMax
+                    : (want >= 6 && L - want >= 6 && want <= kNttSingleMax &&
+                       L - want <= kNttSingleMax),
+          "ntt tune: one pass needs N <= 2^13; a split needs passes of length 2^6..2^13");
+  require(variant >= 0 && variant <= 3, "ntt tune: variant must be 0..3");
+  auto okc = [](int c) { return c == 0 || (c >= 1 && c <= 64 && (c & (c - 1)) == 0); };
+  require(okc(cols1) && okc(cols2), "ntt tune: cols must be 0 or a power of two <= 64");
   std::lock_guard<std::mutex> lk(impl_->mu);
   auto& p = impl_->plans[N];
-  const int want = n1_log > 0 ? n1_log : (L + 1) / 2;
   if (!p || p->n1_log != want) p.reset(new NttPlan(N, want));
-  p->radix = radix;
-  p->cols_max = cols_max;
+  p->variant = variant;
+  p->cols1 = cols1;
+  p->cols2 = cols2;
 }
 
 uint64_t* NttEngine::scratch(size_t words, int which) {
@@ -403,102 +257,175 @@ uint64_t* NttEngine::scratch(size_t words, int which) {
 
 namespace {
 
-size_t g_lds_budget = 0;  // bytes of LDS one workgroup may use for a tile
+size_t g_lds_max = 0;  // bytes of LDS one workgroup may use
 
-size_t lds_budget() {
-  if (g_lds_budget == 0) {
+size_t lds_max() {
+  if (g_lds_max == 0) {
     int dev = 0;
-    hipDeviceProp_t prop;
     size_t b = 65536;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-      b = prop.sharedMemPerBlock;
-    g_lds_budget = std::min<size_t>(b, 131072);
+    int v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+            hipSuccess &&
+        v > 0)
+      b = (size_t)v;
+    g_lds_max = std::min<size_t>(b, 160 * 1024);
   }
-  return g_lds_budget;
+  return g_lds_max;
 }
 
-int cols_per_wg(int n_log, int ncols, int cols_max) {
-  int C = cols_max;
-  while (C > 1 && (((size_t)C << n_log) * sizeof(uint64_t)) > lds_budget()) C >>= 1;
-  while (C > ncols) C >>= 1;
+struct PassKernel {
+  const void* fn;
+  int nlog, rmax;
+};
+
+template <int K, int RA, int RB, int RC>
+PassKernel pk() {
+  using P = nttpass::Part<RA, RB, RC>;
+  return PassKernel{(const void*)ntt_pass_kernel<K, RA, RB, RC>, P::NLOG, P::RMAX};
+}
+
+// Stage grouping per pass length (variant 1 = more, shorter groups).  Four-step
+// passes are 6..13 long; whole-transform passes 1..13.
+template <int K>
+PassKernel pass_kernel(int nlog, int variant) {
+  constexpr bool whole = K == kFwdWhole || K == kInvWhole || K == kInvWholeProd;
+  if constexpr (whole) {
+    switch (nlog) {
+      case 1: return pk<K, 1, 0, 0>();
+      case 2: return pk<K, 2, 0, 0>();
+      case 3: return pk<K, 3, 0, 0>();
+      case 4: return pk<K, 4, 0, 0>();
+      case 5: return pk<K, 5, 0, 0>();
+      default: break;
+    }
+  }
+  switch (nlog) {
+    case 6: return pk<K, 3, 3, 0>();
+    case 7: return pk<K, 4, 3, 0>();
+    case 8: return pk<K, 4, 4, 0>();
+    case 9: return variant ? pk<K, 3, 3, 3>() : pk<K, 5, 4, 0>();
+    case 10: return variant ? pk<K, 4, 3, 3>() : pk<K, 5, 5, 0>();
+    case 11: return pk<K, 4, 4, 3>();
+    case 12: return pk<K, 4, 4, 4>();
+    case 13: return pk<K, 5, 4, 4>();
+    default: throw Error(kErrUnsupported, "ntt: pass length out of range");
+  }
+}
+
+// Columns per tile: requested, else about 256 threads; bounded by 512
+// threads, two tiles' LDS per CU and the column count.
+int tile_cols(int want, int nlog, int rmax, int ncols) {
+  int C = want > 0 ? want : std::max(1, 256 >> (nlog - rmax));
+  if (want <= 0) C = std::min(C, 16);
+  while (C > 1 && (C << (nlog - rmax)) > kPassThreads) C >>= 1;
+  while (C > 1 && ntt_tile_lds_bytes(nlog, C) > lds_max() / (want > 0 ? 1 : 2)) C >>= 1;
+  while (C > 1 && C > ncols) C >>= 1;
   return C;
 }
 
-template <bool INV, bool TW, bool PROD, int R>
-void launch_pass_r(hipStream_t st, const PassArgs& a, int batch, size_t lds) {
-  static bool attr_set = false;
-  if (!attr_set && lds > 65536) {
-    hip_check(hipFuncSetAttribute((const void*)ntt_pass_kernel<INV, TW, PROD, R>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget()),
-              "ntt: raise LDS limit");
-    attr_set = true;
+template <int K>
+void launch_pass(hipStream_t st, NttPassDesc d, int nlog, int variant, int want_cols, int ncols,
+                 int batch, Profiler* prof, const char* zone) {
+  ProfZone pz(prof, st, zone);
+  const PassKernel k = pass_kernel<K>(nlog, variant);
+  const int C = tile_cols(want_cols, nlog, k.rmax, ncols);
+  d.c_log = ilog2_exact((uint64_t)C);
+  d.ncols = ncols;
+  const uint32_t tiles = (uint32_t)((ncols + C - 1) / C);
+  const size_t lds = ntt_tile_lds_bytes(nlog, C);
+  require(lds <= lds_max(), "ntt: tile exceeds LDS");
+  if (lds > 65536) {
+    static std::mutex mu;
+    static std::vector<const void*> raised;
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(raised.begin(), raised.end(), k.fn) == raised.end()) {
+      hip_check(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_max()),
+                "ntt: raise LDS limit");
+      raised.push_back(k.fn);
+    }
   }
-  hipLaunchKernelGGL((ntt_pass_kernel<INV, TW, PROD, R>), dim3(a.ncols / a.C, batch),
-                     dim3(kPassThreads), lds, st, a);
+  uint32_t tl = tiles;
+  void* args[] = {&d, &tl};
+  hip_check(hipLaunchKernel(k.fn, dim3(tiles, (unsigned)batch), dim3((unsigned)(C << (nlog - k.rmax))),
+                            args, lds, st),
+            "launch ntt_pass_kernel");
 }
 
-template <bool INV, bool TW, bool PROD>
-void launch_pass(hipStream_t st, PassArgs a, int batch, Profiler* prof, int radix) {
-  ProfZone pz(prof, st, INV ? (TW ? "ntt.inv_pass1" : "ntt.inv_pass2")
-                                  : (TW ? "ntt.fwd_pass1" : "ntt.fwd_pass2"));
-  const size_t lds = ((size_t)a.C << a.n_log) * sizeof(uint64_t);
-  require(a.ncols % a.C == 0, "ntt: tile does not divide the columns");
-  require(lds <= lds_budget(), "ntt: tile exceeds LDS");
-  switch (radix) {
-    case 2: launch_pass_r<INV, TW, PROD, 2>(st, a, batch, lds); break;
-    case 3: launch_pass_r<INV, TW, PROD, 3>(st, a, batch, lds); break;
-    case 5: launch_pass_r<INV, TW, PROD, 5>(st, a, batch, lds); break;
-    default: launch_pass_r<INV, TW, PROD, 4>(st, a, batch, lds); break;
-  }
-  hip_check(hipGetLastError(), "launch ntt_pass_kernel");
+NttPassDesc base_desc(const NttPlan& pl) {
+  NttPassDesc d{};
+  d.N_log = pl.L;
+  d.tw_lo_bits = pl.tw_lo_bits;
+  d.scale_w = pl.ninv;
+  d.scale_wq = pl.ninv_q;
+  return d;
 }
 
 void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t* tmp,
                        uint64_t* out, int batch, Profiler* prof) {
-  const int n1 = 1 << pl.n1_log, n2 = 1 << pl.n2_log;
-  PassArgs a{};
+  if (pl.single()) {
+    // whole transforms as columns: one HBM round trip
+    NttPassDesc a = base_desc(pl);
+    a.in = in; a.out = out;
+    a.tab = pl.n1_neg_fwd.as<uint64_t>();
+    a.in_ps_log = 0; a.in_cs_log = pl.L; a.out_ps_log = 0; a.out_cs_log = pl.L;
+    launch_pass<kFwdWhole>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
+                           "ntt.fwd_single");
+    return;
+  }
   // pass 1: columns i2, length-n1 negacyclic over i1, twiddle psi^(i2(2k1+1))
+  NttPassDesc a = base_desc(pl);
   a.in = in; a.out = tmp;
   a.tab = pl.n1_neg_fwd.as<uint64_t>();
   a.tw_lo = pl.tw_fwd_lo.as<uint64_t>(); a.tw_hi = pl.tw_fwd_hi.as<uint64_t>();
-  a.tw_lo_bits = pl.tw_lo_bits; a.tw_swap = 0;
-  a.n_log = pl.n1_log; a.ncols = n2; a.C = cols_per_wg(pl.n1_log, n2, pl.cols_max);
-  a.in_ps = n2; a.in_cs = 1; a.out_ps = n2; a.out_cs = 1;
-  a.in_brv = 0; a.out_brv = 1; a.N_log = pl.L;
-  launch_pass<false, true, false>(st, a, batch, prof, pl.radix);
+  a.in_ps_log = pl.n2_log; a.in_cs_log = 0; a.out_ps_log = pl.n2_log; a.out_cs_log = 0;
+  launch_pass<kFwdCols>(st, a, pl.n1_log, pl.variant & 1, pl.cols1, 1 << pl.n2_log, batch, prof,
+                     "ntt.fwd_pass1");
   // pass 2: rows k1, length-n2 cyclic over i2, written transposed (natural k)
-  PassArgs b{};
+  NttPassDesc b = base_desc(pl);
   b.in = tmp; b.out = out;
   b.tab = pl.n2_cyc_fwd.as<uint64_t>();
-  b.n_log = pl.n2_log; b.ncols = n1; b.C = cols_per_wg(pl.n2_log, n1, pl.cols_max);
-  b.in_ps = 1; b.in_cs = n2; b.out_ps = n1; b.out_cs = 1;
-  b.in_brv = 0; b.out_brv = 1; b.N_log = pl.L;
-  launch_pass<false, false, false>(st, b, batch, prof, pl.radix);
+  b.in_ps_log = 0; b.in_cs_log = pl.n2_log; b.out_ps_log = pl.n1_log; b.out_cs_log = 0;
+  launch_pass<kFwdRows>(st, b, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2, 1 << pl.n1_log, batch,
+                     prof, "ntt.fwd_pass2");
 }
 
 void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const uint64_t* in2,
                        uint64_t* tmp, uint64_t* out, int batch, Profiler* prof) {
-  const int n1 = 1 << pl.n1_log, n2 = 1 << pl.n2_log;
-  PassArgs a{};
+  if (pl.single()) {
+    NttPassDesc a = base_desc(pl);
+    a.in = in; a.in2 = in2; a.out = out;
+    a.tab = pl.n1_neg_inv.as<uint64_t>();
+    a.in_ps_log = 0; a.in_cs_log = pl.L; a.out_ps_log = 0; a.out_cs_log = pl.L;
+    if (in2)
+      launch_pass<kInvWholeProd>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
+                                 "ntt.inv_single");
+    else
+      launch_pass<kInvWhole>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
+                             "ntt.inv_single");
+    return;
+  }
   // pass I1: columns k1 of the [k2][k1] input, inverse cyclic over k2,
   // twiddle psi^-(i2(2k1+1)) N^-1, rows k1 out
+  NttPassDesc a = base_desc(pl);
   a.in = in; a.in2 = in2; a.out = tmp;
   a.tab = pl.n2_cyc_inv.as<uint64_t>();
   a.tw_lo = pl.tw_inv_lo.as<uint64_t>(); a.tw_hi = pl.tw_inv_hi.as<uint64_t>();
-  a.tw_lo_bits = pl.tw_lo_bits; a.tw_swap = 1;
-  a.n_log = pl.n2_log; a.ncols = n1; a.C = cols_per_wg(pl.n2_log, n1, pl.cols_max);
-  a.in_ps = n1; a.in_cs = 1; a.out_ps = 1; a.out_cs = n2;
-  a.in_brv = 1; a.out_brv = 0; a.N_log = pl.L;
-  if (in2) launch_pass<true, true, true>(st, a, batch, prof, pl.radix);
-  else launch_pass<true, true, false>(st, a, batch, prof, pl.radix);
+  a.in_ps_log = pl.n1_log; a.in_cs_log = 0; a.out_ps_log = 0; a.out_cs_log = pl.n2_log;
+  if (in2)
+    launch_pass<kInvRowsProd>(st, a, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2,
+                              1 << pl.n1_log, batch, prof, "ntt.inv_pass1");
+  else
+    launch_pass<kInvRows>(st, a, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2, 1 << pl.n1_log,
+                          batch, prof, "ntt.inv_pass1");
   // pass I2: columns i2, inverse negacyclic over k1
-  PassArgs b{};
+  NttPassDesc b = base_desc(pl);
   b.in = tmp; b.out = out;
   b.tab = pl.n1_neg_inv.as<uint64_t>();
-  b.n_log = pl.n1_log; b.ncols = n2; b.C = cols_per_wg(pl.n1_log, n2, pl.cols_max);
-  b.in_ps = n2; b.in_cs = 1; b.out_ps = n2; b.out_cs = 1;
-  b.in_brv = 1; b.out_brv = 0; b.N_log = pl.L;
-  launch_pass<true, false, false>(st, b, batch, prof, pl.radix);
+  b.in_ps_log = pl.n2_log; b.in_cs_log = 0; b.out_ps_log = pl.n2_log; b.out_cs_log = 0;
+  launch_pass<kInvCols>(st, b, pl.n1_log, pl.variant & 1, pl.cols1, 1 << pl.n2_log, batch, prof,
+                    "ntt.inv_pass2");
 }
 
 void naive_forward_brv(hipStream_t st, NttPlan& pl, uint64_t* d, int batch) {

@@ -4,14 +4,16 @@
 //   forward(a)[k] = a(psi^(2k+1)) in natural order (psi = 3^((P-1)/2N)),
 //   inverse = forward^-1, poly_mul(a, b) = a*b mod (X^N + 1).
 // Two device backends compute exactly that:
-//   * kFourStep: N = n1*n2 with n1 = 2^ceil(L/2), n2 = 2^floor(L/2).  Pass 1
-//     runs length-n1 negacyclic NTTs down the columns (root psi^n2, twist
-//     merged into the butterflies), multiplies by psi^(i2(2k1+1)) and writes
-//     back; pass 2 runs length-n2 cyclic NTTs along the rows and writes the
-//     result transposed, i.e. in natural order.  Each pass stages a tile of
-//     whole sub-transforms in LDS, so a transform is two coalesced HBM round
-//     trips.  The inverse mirrors it; N^-1 and the pointwise product of a
-//     poly-mul are folded into the first inverse pass.
+//   * kFourStep: for N <= 2^13 one pass per transform (the whole transform in
+//     one workgroup tile, one HBM round trip).  Larger N split as n1*n2 with
+//     n1 = 2^ceil(L/2), n2 = 2^floor(L/2) by default: pass 1 runs length-n1
+//     negacyclic NTTs down the columns (root psi^n2, twist merged into the
+//     butterflies), multiplies by psi^(i2(2k1+1)) and writes back; pass 2
+//     runs length-n2 cyclic NTTs along the rows and writes the result
+//     transposed, i.e. in natural order (two HBM round trips).  Passes hold
+//     their data in registers and exchange through LDS between register
+//     groups (ntt_pass.hpp).  The inverse mirrors it; N^-1 and the pointwise
+//     product of a poly-mul are folded into the first inverse pass.
 //   * kNaive: the textbook per-stage radix-2 form (one launch per butterfly
 //     stage: psi-twist, DIF forward, DIT inverse), the structurally
 //     independent cross-check of the reference's naive tier
@@ -28,6 +30,9 @@
 namespace pcgsq {
 
 enum NttBackend : int { kNttFourStep = 0, kNttNaive = 1 };
+
+// Longest transform run as a single pass (2^13 words = 64 KiB of LDS).
+constexpr int kNttSingleMax = 13;
 
 struct NttPlan;
 
@@ -49,10 +54,12 @@ class NttEngine {
   // Scratch for a batch of transforms (grows on demand).
   uint64_t* scratch(size_t words, int which);
   NttPlan& plan(int N);
-  // Shape of the four-step passes for length N (chosen by the autotuner):
-  // n1 = 2^n1_log column length (0 = ceil(log2 N / 2)), radix = stages per
-  // register group, cols_max = sub-transforms per workgroup.
-  void tune(int N, int n1_log, int radix, int cols_max);
+  // Shape of the passes for length N (chosen by the autotuner): n1 = 2^n1_log
+  // column length (n1_log = log2 N: single pass; 0: default_split), variant
+  // bit 0/1 = alternative register grouping of the n1/n2 passes, cols1/cols2 =
+  // sub-transforms per workgroup tile of those passes (0 = automatic).
+  void tune(int N, int n1_log, int variant, int cols1, int cols2);
+  static int default_split(int L);
   Profiler* prof = nullptr;  // optional per-launch timing
 
  private:

@@ -1,0 +1,306 @@
+// One pass of the negacyclic NTT over F_p, as a device function shared by the
+// per-length launches (ntt.hip) and the persistent batched kernel
+// (ntt_batch.hip).
+//
+// A pass runs C independent sub-transforms ("columns") of length n = 2^NLOG
+// through all NLOG radix-2 butterfly stages.  The stages are cut into up to
+// three register groups of RA, RB, RC stages (forward stage order).  Each
+// thread owns one set of 2^r elements of a column per group: the first group
+// in time loads its set straight from HBM into registers, the last one stores
+// straight to HBM, and consecutive groups exchange through one padded,
+// swizzled LDS tile.  A length-1024 pass (5 + 5 stages) therefore touches LDS
+// once per element in each direction.
+//
+// Forward passes are Cooley-Tukey with merged roots (natural in, bit-reversed
+// positions), values lazily reduced in [0, 4P) (Harvey); inverse passes are
+// Gentleman-Sande in the mirrored order, values in [0, 2P).  The stage table
+// holds (w, floor(w 2^64 / P)) pairs: stage s, butterfly block j uses entry
+// 2^s + j.
+#pragma once
+
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace pcgsq {
+
+// Where a pass reads and writes.  Strides are powers of two (log2, in
+// elements); a column's element at position pos lives at row p = pos, or at
+// p = bitreverse(pos) on the bit-reversed side of the pass (see PassKind).
+struct NttPassDesc {
+  const uint64_t* in;
+  const uint64_t* in2;       // *Prod kinds: the load multiplies in[] by in2[] (mod P)
+  uint64_t* out;
+  const uint64_t* tab;       // stage table [n][2]
+  const uint64_t* tw_lo;     // four-step twiddle psi^(+-e), two-level tables [.][2]
+  const uint64_t* tw_hi;
+  uint64_t scale_w, scale_wq;  // whole-transform inverse: N^-1
+  int tw_lo_bits;
+  int c_log;                 // C = 2^c_log columns per tile
+  int in_ps_log, in_cs_log, out_ps_log, out_cs_log;
+  int N_log;                 // whole transform length (batch stride, twiddle modulus 2N)
+  int ncols;                 // valid columns (a ragged last tile skips the rest)
+};
+
+// The kinds of pass, fixed at compile time so that the per-element load and
+// store code is straight-line:
+//   kFwdCols  four-step pass 1: natural rows in, bit-reversed positions out
+//             with the twiddle psi^(c (2p + 1)); lanes along the columns.
+//   kFwdRows  four-step pass 2: contiguous sub-transforms in (lanes along the
+//             positions), transposed store.
+//   kInvRows / kInvRowsProd  inverse pass 1 (optionally multiplying two
+//             forward-domain inputs on load), twiddle psi^-(p (2c + 1)) N^-1.
+//   kInvCols  inverse pass 2.
+//   kFwdWhole / kInvWhole / kInvWholeProd  a whole transform per column
+//             (one pass; the inverse scales by N^-1).
+enum PassKind : int {
+  kFwdCols = 0, kFwdRows, kInvRows, kInvRowsProd, kInvCols, kFwdWhole, kInvWhole, kInvWholeProd
+};
+
+namespace nttpass {
+
+__device__ __forceinline__ uint64_t red2p(uint64_t x) { return x >= kTwoP ? x - kTwoP : x; }
+__device__ __forceinline__ uint64_t red_full(uint64_t x) {
+  x = x >= kTwoP ? x - kTwoP : x;
+  return x >= kP ? x - kP : x;
+}
+
+__device__ __forceinline__ uint32_t brv(uint32_t x, int bits) {
+  return bits ? __brev(x) >> (32 - bits) : 0u;
+}
+
+struct Pair {
+  uint64_t w, wq;
+};
+__device__ __forceinline__ Pair ld_pair(const uint64_t* tab, uint32_t i) {
+  const ulonglong2 t = reinterpret_cast<const ulonglong2*>(tab)[i];
+  return Pair{t.x, t.y};
+}
+
+// Padding words per column so that column c starts on bank slot c * 32 / C.
+__host__ __device__ constexpr int lds_pad(int C) { return C >= 2 ? (C >= 32 ? 1 : 32 / C) : 0; }
+
+// LDS word of (column c, position pos): columns are contiguous, the low five
+// position bits are XORed with the next five.
+template <int NLOG>
+__device__ __forceinline__ uint32_t lds_at(uint32_t c, uint32_t pos, uint32_t col_stride) {
+  uint32_t x = pos;
+  if constexpr (NLOG >= 6) {
+    constexpr uint32_t k = NLOG - 5 < 5 ? NLOG - 5 : 5;
+    x ^= (pos >> 5) & ((1u << k) - 1);
+  }
+  return c * col_stride + x;
+}
+
+// r butterfly stages [S0, S0 + R) on one set held in registers.  Hh is the
+// set's block index (its butterfly blocks in stage S0 + q are (Hh << q) | j).
+template <bool INV, int S0, int R>
+__device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_t Hh,
+                                                  const uint64_t* __restrict__ tab) {
+  constexpr int M = 1 << R;
+  if constexpr (S0 == 0) Hh = 0;
+  if constexpr (!INV) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int bit = R - 1 - q;
+      // keep each stage's twiddle loads in that stage (bounds register use)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        if ((w >> bit) & 1) continue;
+        const int w2 = w | (1 << bit);
+        const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
+        const uint64_t U = red2p(v[w]);
+        const uint64_t T = shoup_lazy(v[w2], t.w, t.wq);
+        v[w] = U + T;
+        v[w2] = U - T + kTwoP;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+      const int bit = R - 1 - q;
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        if ((w >> bit) & 1) continue;
+        const int w2 = w | (1 << bit);
+        const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
+        const uint64_t U = v[w], V = v[w2];  // both < 2P
+        v[w] = red2p(U + V);
+        v[w2] = shoup_lazy(U - V + kTwoP, t.w, t.wq);
+      }
+    }
+  }
+}
+
+template <int K>
+struct KindTraits {
+  static constexpr bool INV = K == kInvRows || K == kInvRowsProd || K == kInvCols ||
+                              K == kInvWhole || K == kInvWholeProd;
+  static constexpr bool IN_BRV = INV;
+  static constexpr bool OUT_BRV = !INV;
+  static constexpr bool PROD = K == kInvRowsProd || K == kInvWholeProd;
+  static constexpr int STORE = (K == kFwdCols || K == kInvRows || K == kInvRowsProd) ? 1
+                               : (K == kInvWhole || K == kInvWholeProd)             ? 2
+                                                                                    : 0;
+  static constexpr bool TW_SWAP = K == kInvRows || K == kInvRowsProd;
+  // lanes along the columns (unit column stride in HBM) or along the positions
+  static constexpr bool LOAD_CFAST = K == kFwdCols || K == kInvRows || K == kInvRowsProd ||
+                                     K == kInvCols;
+  static constexpr bool STORE_CFAST = K == kFwdCols || K == kFwdRows || K == kInvCols;
+};
+
+// Role of a register group in the pass (kCopy: move the set, no stages).
+enum : int { kFromHbm = 1, kToHbm = 2, kCopy = 4 };
+
+// Group `S0, R` of a pass over a tile: for each set owned by this thread,
+// gather (from HBM or LDS), run the stages, scatter (to LDS or HBM).
+template <int K, int NLOG, int S0, int R, int RMAX, int ROLE>
+__device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
+                                          uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
+                                          const uint64_t* __restrict__ inb,
+                                          const uint64_t* __restrict__ in2b,
+                                          uint64_t* __restrict__ outb) {
+  constexpr int M = 1 << R;
+  constexpr int TMIN = NLOG - S0 - R;         // log2 of the set stride
+  constexpr int SLOG = NLOG - R;              // log2 of sets per column
+  constexpr int PER = 1 << (RMAX - R);        // sets per thread
+  using Tr = KindTraits<K>;
+  const uint32_t col_stride = (1u << NLOG) + lds_pad((int)C);
+  // lanes run along the columns when the HBM side of this group has unit
+  // column stride, else along the positions
+  constexpr bool cfast = (ROLE & kFromHbm) ? Tr::LOAD_CFAST
+                         : (ROLE & kToHbm) ? Tr::STORE_CFAST
+                                           : true;
+#pragma unroll 1
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t g = t + (uint32_t)k * T;
+    uint32_t c, sigma;
+    if constexpr (cfast) {
+      c = g & (C - 1);
+      sigma = g >> d.c_log;
+    } else {
+      sigma = g & ((1u << SLOG) - 1);
+      c = g >> SLOG;
+    }
+    if (c0 + c >= (uint32_t)d.ncols) continue;
+    const uint32_t L = sigma & ((1u << TMIN) - 1);
+    const uint32_t Hh = sigma >> TMIN;
+    const uint32_t base = (Hh << (TMIN + R)) | L;
+    uint64_t v[M];
+    if constexpr (ROLE & kFromHbm) {
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        if ((w & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+        const uint32_t pos = base | ((uint32_t)w << TMIN);
+        const uint32_t p = Tr::IN_BRV ? brv(pos, NLOG) : pos;
+        const uint32_t off = (p << d.in_ps_log) + (c << d.in_cs_log);
+        v[w] = inb[off];
+        if constexpr (Tr::PROD) v[w] = fmul(v[w], in2b[off]);
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < M; ++w)
+        v[w] = lds[lds_at<NLOG>(c, base | ((uint32_t)w << TMIN), col_stride)];
+    }
+    if constexpr (!(ROLE & kCopy)) group_butterflies<Tr::INV, S0, R>(v, Hh, d.tab);
+    if constexpr (ROLE & kToHbm) {
+      const uint32_t two_n_mask = (2u << d.N_log) - 1;
+      const uint32_t lo_mask = (1u << d.tw_lo_bits) - 1;
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        // bound the twiddle loads in flight (each is 4 VGPRs)
+        if ((w & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+        const uint32_t pos = base | ((uint32_t)w << TMIN);
+        const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
+        uint64_t x = v[w];
+        if constexpr (Tr::STORE == 1) {
+          const uint32_t cc = c0 + c;
+          const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
+          const Pair a = ld_pair(d.tw_lo, ex & lo_mask);
+          const Pair b = ld_pair(d.tw_hi, ex >> d.tw_lo_bits);
+          x = shoup_lazy(x, a.w, a.wq);
+          x = shoup_lazy(x, b.w, b.wq);
+        } else if constexpr (Tr::STORE == 2) {
+          x = shoup_lazy(x, d.scale_w, d.scale_wq);
+        }
+        outb[(p << d.out_ps_log) + (c << d.out_cs_log)] = red_full(x);
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < M; ++w)
+        lds[lds_at<NLOG>(c, base | ((uint32_t)w << TMIN), col_stride)] = v[w];
+    }
+  }
+}
+
+// Stage layout of a pass: up to three groups RA, RB, RC (forward order).
+template <int RA, int RB, int RC>
+struct Part {
+  static constexpr int NLOG = RA + RB + RC;
+  static constexpr int G = RC ? 3 : (RB ? 2 : 1);
+  static constexpr int RMAX = RA > RB ? (RA > RC ? RA : RC) : (RB > RC ? RB : RC);
+  // threads a tile of C columns needs
+  static __host__ __device__ constexpr int threads(int C) { return C << (NLOG - RMAX); }
+};
+
+// One tile (C columns starting at column tile `ct` of transform `b`).
+// Threads t >= Part::threads(C) idle but take part in the barriers.
+template <int K, int RA, int RB, int RC>
+__device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __restrict__ lds,
+                                          size_t b, uint32_t ct) {
+  using P = Part<RA, RB, RC>;
+  constexpr int NLOG = P::NLOG, RMAX = P::RMAX;
+  const uint32_t C = 1u << d.c_log;
+  const uint32_t T = (uint32_t)P::threads((int)C);
+  const uint32_t t = threadIdx.x;
+  const bool active = t < T;
+  const uint32_t c0 = ct << d.c_log;
+  const uint64_t* inb = d.in + (b << d.N_log) + ((size_t)c0 << d.in_cs_log);
+  const uint64_t* in2b = KindTraits<K>::PROD ? d.in2 + (b << d.N_log) + ((size_t)c0 << d.in_cs_log) : nullptr;
+  uint64_t* outb = d.out + (b << d.N_log) + ((size_t)c0 << d.out_cs_log);
+  if constexpr (P::G == 1) {
+    // one group: stage through LDS so that an in-place pass reads every
+    // element before any bit-reversed store lands
+    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    __syncthreads();
+    if (active)
+      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy>(d, lds, t, T, C, c0, inb, in2b, outb);
+  } else if constexpr (P::G == 2) {
+    constexpr int SA = 0, SB = RA;
+    if constexpr (!KindTraits<K>::INV) {
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    } else {
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    }
+  } else {
+    constexpr int SA = 0, SB = RA, SC = RA + RB;
+    if constexpr (!KindTraits<K>::INV) {
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    } else {
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0>(d, lds, t, T, C, c0, inb, in2b, outb);
+      __syncthreads();
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    }
+  }
+}
+
+}  // namespace nttpass
+
+// LDS bytes of a tile of C columns of length 2^nlog.
+inline size_t ntt_tile_lds_bytes(int nlog, int C) {
+  return (size_t)C * ((size_t(1) << nlog) + (size_t)nttpass::lds_pad(C)) * sizeof(uint64_t);
+}
+
+}  // namespace pcgsq

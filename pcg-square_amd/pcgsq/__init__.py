@@ -92,7 +92,7 @@ def lib() -> C.CDLL:
         getattr(L, f).argtypes = [vp, u64p, C.c_int, C.c_int, C.c_int]
     for f in ("pcgsq_ntt_forward_dev", "pcgsq_ntt_inverse_dev"):
         getattr(L, f).argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
-    L.pcgsq_ntt_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.pcgsq_ntt_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.pcgsq_aes128_expand_key.argtypes = [u8p, u8p]
     L.pcgsq_aes128_ecb_encrypt.argtypes = [vp, u8p, u8p, u8p, C.c_int]
     L.pcgsq_aes_tree_round_keys.argtypes = [u8p, C.c_int, u8p]
@@ -308,9 +308,12 @@ class Context:
         _check(lib().pcgsq_ntt_inverse(self.h, _p(x2), x2.shape[0], x2.shape[1], backend))
         return x
 
-    def ntt_tune(self, N: int, n1_log: int = 0, radix: int = 4, cols_max: int = 16) -> None:
-        """Shape of the four-step passes for length N (results do not change)."""
-        _check(lib().pcgsq_ntt_tune(self.h, N, n1_log, radix, cols_max))
+    def ntt_tune(self, N: int, n1_log: int = 0, variant: int = 0, cols1: int = 0,
+                 cols2: int = 0) -> None:
+        """Shape of the NTT passes for length N (results do not change): split
+        n1 = 2^n1_log (log2 N = one pass), register-grouping variant bits and
+        columns per tile of the n1 / n2 passes (0 = automatic)."""
+        _check(lib().pcgsq_ntt_tune(self.h, N, n1_log, variant, cols1, cols2))
 
     def poly_mul_dev(self, out_ptr: int, a_ptr: int, b_ptr: int, batch: int, N: int,
                      backend=NTT_FOURSTEP):

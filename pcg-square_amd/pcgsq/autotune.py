@@ -1,16 +1,18 @@
-"""Autotuner for the four-step NTT pass shape.
+"""Autotuner for the NTT pass shapes.
 
 For each transform length N it times forward + inverse of a device-resident
-batch for every candidate shape -- column/row split (n1 = 2^n1_log), stages
-per register group (radix 2^r) and sub-transforms per workgroup -- with the
-library's own per-launch event timing, and keeps the fastest.  Shapes never
-change results (every candidate is checked bit-exact against the default
-shape before it is timed).
+batch for candidate shapes -- one pass or a column/row split (n1 = 2^n1_log),
+the register grouping of each pass (variant bits) and the sub-transforms per
+workgroup tile of each pass (cols1, cols2) -- with the library's own
+per-launch event timing, and keeps the fastest.  Shapes never change results:
+every candidate is checked bit-exact against the default shape before it is
+timed.  Splits and groupings are searched exhaustively, tile widths by
+coordinate descent from the best of those.
 
     python pcg-square_amd/pcgsq/autotune.py --out tuning.json
 
-The cache maps device name -> {N: [n1_log, radix, cols_max, us_per_transform]}
-and is applied by pcgsq.Context(apply_tuning=True).
+The cache maps the device's gfx name -> {N: [n1_log, variant, cols1, cols2,
+us_per_transform]} and is applied by pcgsq.Context(apply_tuning=True).
 """
 from __future__ import annotations
 
@@ -26,12 +28,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_CACHE = os.path.join(HERE, "ntt_tuning.json")
 
 
-def candidates(N: int):
+SINGLE_MAX = 13          # longest one-pass transform
+PASS_MIN = 6             # shortest four-step pass
+ALT_GROUPING = (9, 10)   # pass lengths with a second register grouping
+
+
+def splits(N: int):
     L = N.bit_length() - 1
+    out = [L] if L <= SINGLE_MAX else []
     base = (L + 1) // 2
-    splits = sorted({s for s in (base - 1, base, base + 1) if 1 <= s < L and max(s, L - s) <= 12})
-    for s, r, c in itertools.product(splits, (3, 4, 5), (4, 8, 16)):
-        yield s, r, c
+    for s in (base - 1, base, base + 1):
+        if s < L and PASS_MIN <= s <= SINGLE_MAX and PASS_MIN <= L - s <= SINGLE_MAX:
+            out.append(s)
+    return sorted(set(out))
+
+
+def variants(L: int, s: int):
+    bits = [0]
+    if s in ALT_GROUPING:
+        bits.append(1)
+    if s < L and (L - s) in ALT_GROUPING:
+        bits += [b | 2 for b in bits]
+    return bits
 
 
 def time_shape(ctx, buf, batch, N, iters=3) -> float:
@@ -54,26 +72,39 @@ def tune(ctx, Ns, elems=1 << 24, log=print) -> dict:
     res = {}
     rng = np.random.default_rng(1)
     for N in Ns:
+        L = N.bit_length() - 1
         batch = max(1, elems // N)
         a = rng.integers(0, pcgsq.P, (batch, N), dtype=np.uint64)
-        ctx.ntt_tune(N)                      # default shape as the reference result
+        ctx.ntt_tune(N)                      # default shape gives the reference result
         ref = ctx.ntt_forward(a[:1])
         buf = ctx.to_device(a)
-        best = None
-        for s, r, c in candidates(N):
-            ctx.ntt_tune(N, s, r, c)
+        seen = {}
+
+        def run(shape):
+            if shape in seen:
+                return seen[shape]
+            ctx.ntt_tune(N, *shape)
             if not np.array_equal(ctx.ntt_forward(a[:1]), ref):
-                raise RuntimeError(f"shape {(s, r, c)} changed the result at N={N}")
-            ms = time_shape(ctx, buf, batch, N)
-            us = ms * 1e3 / batch
-            log(f"N=2^{N.bit_length() - 1} n1_log={s} radix={r} cols={c}: {us:.2f} us/transform "
-                f"(fwd+inv)")
-            if best is None or us < best[3]:
-                best = [s, r, c, us]
+                raise RuntimeError(f"shape {shape} changed the result at N={N}")
+            us = time_shape(ctx, buf, batch, N) * 1e3 / batch
+            log(f"N=2^{L} n1_log={shape[0]} variant={shape[1]} cols={shape[2]},{shape[3]}: "
+                f"{us:.2f} us/transform (fwd+inv)")
+            seen[shape] = us
+            return us
+
+        best = min(((s, v, 0, 0) for s in splits(N) for v in variants(L, s)), key=run)
+        for slot in (2, 3):
+            if slot == 3 and best[0] == L:
+                break                        # one pass: no second tile width
+            for c in (1, 2, 4, 8, 16):
+                cand = list(best)
+                cand[slot] = c
+                if run(tuple(cand)) < seen[best]:
+                    best = tuple(cand)
         buf.close()
-        ctx.ntt_tune(N, *best[:3])
-        res[str(N)] = best
-        log(f"best N=2^{N.bit_length() - 1}: {best}")
+        ctx.ntt_tune(N, *best)
+        res[str(N)] = list(best) + [round(seen[best], 3)]
+        log(f"best N=2^{L}: {res[str(N)]}")
     return res
 
 
@@ -88,15 +119,20 @@ def apply(ctx, path: str = DEFAULT_CACHE) -> int:
     entry = cache.get(dev) or cache.get(dev.split(" (")[-1].split(",")[0])
     if not entry:
         return 0
-    for N, (s, r, c, _us) in entry.items():
-        ctx.ntt_tune(int(N), int(s), int(r), int(c))
-    return len(entry)
+    n = 0
+    for N, shape in entry.items():
+        if len(shape) != 5:
+            continue                          # an entry of an older layout
+        s, v, c1, c2, _us = shape
+        ctx.ntt_tune(int(N), int(s), int(v), int(c1), int(c2))
+        n += 1
+    return n
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_CACHE)
-    ap.add_argument("--logN", default="12,14,16,18,20,22")
+    ap.add_argument("--logN", default="10,12,13,14,16,18,20,22")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     import pcgsq

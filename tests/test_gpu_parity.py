@@ -133,11 +133,9 @@ def test_out_hash_bitexact_fixture(ctx, pq):
 
 
 @pytest.mark.parametrize("backend", [0, 1])
-@pytest.mark.parametrize("logN", [2, 3, 4, 5, 7, 10, 11, 12, 13, 14, 16])
+@pytest.mark.parametrize("logN", [1, 2, 3, 4, 5, 7, 10, 11, 12, 13, 14, 16])
 def test_ntt_forward_inverse_match_oracle(ctx, pq, backend, logN):
     N = 1 << logN
-    if backend == 0 and logN < 2:
-        pytest.skip("four-step needs N >= 4")
     rng = np.random.default_rng(logN + 100 * backend)
     batch = 3
     a = rng.integers(0, P, (batch, N), dtype=np.uint64)
@@ -147,8 +145,31 @@ def test_ntt_forward_inverse_match_oracle(ctx, pq, backend, logN):
     assert np.array_equal(ctx.ntt_inverse(fa, backend), a)
 
 
+@pytest.mark.parametrize("logN", [6, 9, 10, 13, 14, 15, 18, 19, 21])
+def test_ntt_every_tunable_shape_matches_oracle(ctx, pq, logN):
+    """Every split / register grouping / tile width the autotuner may pick
+    gives the oracle's forward transform and round-trips."""
+    from pcgsq import autotune as at
+    N = 1 << logN
+    rng = np.random.default_rng(logN + 7)
+    a = rng.integers(0, P, (2, N), dtype=np.uint64)
+    exp = [O.ntt_forward(a[k]) for k in range(2)]
+    tiles = [(0, 0), (1, 1), (2, 16), (16, 2), (8, 8)]
+    try:
+        for s in at.splits(N):
+            for v in at.variants(logN, s):
+                for c1, c2 in tiles:
+                    ctx.ntt_tune(N, s, v, c1, c2)
+                    fa = ctx.ntt_forward(a)
+                    for k in range(2):
+                        assert np.array_equal(fa[k], exp[k]), (s, v, c1, c2)
+                    assert np.array_equal(ctx.ntt_inverse(fa), a), (s, v, c1, c2)
+    finally:
+        ctx.ntt_tune(N)
+
+
 @pytest.mark.parametrize("backend", [0, 1])
-@pytest.mark.parametrize("logN", [2, 6, 9, 12, 15, 17])
+@pytest.mark.parametrize("logN", [1, 2, 6, 9, 12, 15, 17])
 def test_poly_mul_matches_oracle(ctx, pq, backend, logN):
     N = 1 << logN
     rng = np.random.default_rng(logN)
