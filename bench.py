@@ -76,6 +76,55 @@ def cpu_baseline(N: int, c: int, t: int, threads: int) -> dict:
                        f"{full:.1f} s")}
 
 
+def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
+    """Batched NTT of uneven lengths: 256 transforms, lengths drawn uniformly
+    from {2^12, 2^14, 2^16, 2^18, 2^20}, uniform coefficients in [0, P),
+    forward then inverse of every transform in one persistent launch (graph
+    replay), autotuned; against one launch per transform (the plain form)."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    logN = [int(12 + 2 * x) for x in rng.integers(0, 5, 256)]
+    with pcgsq.NttBatch(ctx, logN) as b:
+        b.load(rng.integers(0, pcgsq.P, b.total, dtype=np.uint64))
+        shape = b.autotune()
+        b.run(graph=True)
+        b.finish()
+        ctx.profile(True)
+        for _ in range(reps):
+            b.run(graph=True)
+        b.finish()
+        z = ctx.profile_read()
+        ctx.profile(False)
+        ms = z["ntt.batch"][1] / z["ntt.batch"][0]
+        info = b.info()
+        pa, _, _, _ = b.dev()
+
+        def plain():
+            for i, L in enumerate(logN):
+                ptr = pa + 8 * int(b.offsets[i])
+                ctx.ntt_forward_dev(ptr, 1, 1 << L)
+                ctx.ntt_inverse_dev(ptr, 1, 1 << L)
+
+        plain()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plain()
+        ctx.synchronize()
+        ms_plain = (time.perf_counter() - t0) * 1e3 / reps
+    gbps = info["alg_bytes"] / (ms * 1e-3) / 1e9
+    return {"workload": "256 forward+inverse NTTs, lengths uniform over 2^12..2^20 (x4 steps), "
+                        "one persistent launch from per-XCD work lists, graph replay",
+            "lengths": {f"2^{L}": logN.count(L) for L in sorted(set(logN))},
+            "ms": ms, "alg_bytes": info["alg_bytes"], "GBps": gbps,
+            "hbm_frac": gbps * 1e9 / HBM_PEAK, "work_items": info["items"],
+            "grid": info["grid"], "tuned": {"wgs_per_cu": shape["wgs_per_cu"],
+                                            "splits": {f"2^{k}": v for k, v in
+                                                       shape["splits"].items()}},
+            "one_launch_per_transform_ms": ms_plain,
+            "speedup_vs_one_launch_per_transform": ms_plain / ms}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +171,7 @@ def main() -> int:
         ctx.profile(False)
         ceiling = ctx.probe_chacha8()
         copy_bps = ctx.probe_copy(1 << 30)
+        ntt_batch = bench_ntt_batch(ctx, pcgsq)
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
     n_leaf, ms_leaf = zones["dpf.leaf_convert"]
@@ -176,6 +226,7 @@ def main() -> int:
         "tree_prg_rate_Gblocks_per_s": tree_blocks / (tree_ms * 1e-3) / 1e9,
         "chacha8_ceiling_Gblocks_per_s": ceiling / 1e9,
         "hbm_copy_GBps": copy_bps / 1e9,
+        "batched_ntt": ntt_batch,
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)

@@ -176,6 +176,46 @@ int pcgsq_ntt_inverse_dev(pcgsq_ctx* ctx, uint64_t* dev_a, int batch, int N,
  * 0 = automatic).  Results do not depend on the shape. */
 int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int variant, int cols1, int cols2);
 
+/* ---- Batched NTT of uneven lengths ------------------------------------------
+ * No counterpart in the reference (its GPU NTT tiers take one length per
+ * call: poly_mul_cuda.h:9-31, ntt_batch_bench.cu).  For callers that hold
+ * many transforms of different lengths at once: `count` transforms of
+ * lengths 2^logN[i] (2^10..2^22) in one persistent launch that drains a
+ * device work list (pass-major tiles with per-pass completion counters).
+ * ops: PCGSQ_NTT_BATCH_FORWARD (a -> f), PCGSQ_NTT_BATCH_INVERSE (a -> r) or
+ * both (a -> f -> r).  Transform i occupies words [off_i, off_i + 2^logN[i])
+ * of the batch buffers, off_i = sum of the earlier lengths.  Results equal
+ * pcgsq_ntt_forward / pcgsq_ntt_inverse per transform, bit for bit. */
+typedef struct pcgsq_ntt_batch pcgsq_ntt_batch;
+#define PCGSQ_NTT_BATCH_FORWARD 1
+#define PCGSQ_NTT_BATCH_INVERSE 2
+int pcgsq_ntt_batch_create(pcgsq_ctx* ctx, const int* logN, int count, int ops,
+                           pcgsq_ntt_batch** out);
+int pcgsq_ntt_batch_destroy(pcgsq_ntt_batch* b);
+/* Shape (set by the autotuner): split of length 2^logN (n1_log = logN: one
+ * pass, allowed up to 2^12; else a column pass of 2^7..2^11 and a row pass of
+ * 2^6..2^11; 0: default), resident
+ * workgroups per CU (0 = as many as fit) and the dependency spin limit
+ * (0 = default); logN = 0 changes only the last two. */
+int pcgsq_ntt_batch_tune(pcgsq_ntt_batch* b, int logN, int n1_log, int wgs_per_cu,
+                         uint32_t spin_limit);
+/* Device buffers (inputs a, forward f, inverse r) and their length in words. */
+int pcgsq_ntt_batch_dev(pcgsq_ntt_batch* b, uint64_t** a, uint64_t** f, uint64_t** r,
+                        size_t* total_words);
+/* Copy the concatenated inputs from the host (synchronizes). */
+int pcgsq_ntt_batch_load(pcgsq_ntt_batch* b, const uint64_t* host);
+/* Enqueue the whole batch on the context stream (use_graph: replay a
+ * captured HIP graph of the launch). */
+int pcgsq_ntt_batch_run(pcgsq_ntt_batch* b, int use_graph);
+/* Wait for the batch; PCGSQ_EDEVICE if a dependency wait hit its limit. */
+int pcgsq_ntt_batch_finish(pcgsq_ntt_batch* b);
+/* Copy f and/or r to the host (either may be NULL; synchronizes). */
+int pcgsq_ntt_batch_read(pcgsq_ntt_batch* b, uint64_t* f_host, uint64_t* r_host);
+/* Work items, grid, LDS bytes per workgroup and the algorithmic HBM bytes of
+ * one run (16 * N * passes per transform and direction). */
+int pcgsq_ntt_batch_info(pcgsq_ntt_batch* b, int* nitems, int* grid, size_t* lds_bytes,
+                         double* alg_bytes);
+
 /* ---- AES (aes_gpu.h) -------------------------------------------------------- */
 /* pcg_cuda::aes128_expand_key_host (aes_gpu.h:13). */
 int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]);

@@ -71,14 +71,14 @@ __global__ __launch_bounds__(kTpb) void beaver_cw_kernel(const uint64_t* __restr
 // leaf is read exactly once.
 template <bool FROM_LEAVES>
 __global__ __launch_bounds__(kTpb) void gather_g_kernel(GatherArgs a) {
-  const int set = blockIdx.y;
+  const int bb = blockIdx.y;                    // block within the party
+  const int party = a.first_party + blockIdx.z;
+  const int set = blockIdx.z * a.sets_per_party + bb;
   const size_t q = (size_t)blockIdx.x * kTpb + threadIdx.x;
   if (q >= (size_t)a.N) return;
   const size_t D = (size_t)1 << a.D_log;
   const int bin_log = a.D_log - 1;
   const size_t bin = D >> 1;
-  const int party = a.first_party + set / a.sets_per_party;
-  const int bb = set % a.sets_per_party;
   const uint64_t* ct = FROM_LEAVES ? nullptr : a.ct + a.set_src_stride * set;
   const Blk* lv = FROM_LEAVES ? a.leaves + a.set_src_stride * set : nullptr;
   const uint64_t* CW = a.CW + a.cw_stride * bb;
@@ -121,7 +121,8 @@ __global__ __launch_bounds__(kTpb) void gather_g_kernel(GatherArgs a) {
 }
 
 __global__ __launch_bounds__(kTpb) void leaf_sums_kernel(const Blk* __restrict__ leaves, int B,
-                                                         int D_log, int iters, int nparts,
+                                                         int D_log, int iters, int chunk_log,
+                                                         int nparts,
                                                          uint64_t* __restrict__ part) {
   __shared__ uint64_t red[16];
   const size_t total = (size_t)B << D_log;
@@ -140,8 +141,7 @@ __global__ __launch_bounds__(kTpb) void leaf_sums_kernel(const Blk* __restrict__
                         red, &first)) {
     const size_t f = base + (size_t)first;
     const size_t T = f >> D_log;
-    const size_t chunk = (size_t)iters * kTpb;
-    const size_t pidx = (D_log >= 8) ? ((f & (((size_t)1 << D_log) - 1)) / chunk) : 0;
+    const size_t pidx = (D_log >= 8) ? ((f & (((size_t)1 << D_log) - 1)) >> chunk_log) : 0;
     if (T < (size_t)B) {
       part[(T * nparts + pidx) * 2] = v[0];
       part[(T * nparts + pidx) * 2 + 1] = v[1];
@@ -183,13 +183,16 @@ void launch_beaver_cw(hipStream_t st, const uint64_t* partials, int ninst, int n
 void launch_gather_g(hipStream_t st, const GatherArgs& a) {
   require(a.N > 0 && a.t > 0 && a.B > 0 && a.nsets > 0 && a.sets_per_party > 0,
           "gather: bad shape");
-  require(a.nsets <= 65535, "gather: too many sets");
+  require(a.sets_per_party <= 65535 && a.nsets % a.sets_per_party == 0 &&
+              a.nsets / a.sets_per_party <= 2,
+          "gather: sets must be whole parties of at most 65535 blocks");
   ProfZone pz(a.prof, st, "out.gather");
   const int grid = (a.N + kTpb - 1) / kTpb;
+  const dim3 g3(grid, a.sets_per_party, a.nsets / a.sets_per_party);
   if (a.leaves)
-    hipLaunchKernelGGL(gather_g_kernel<true>, dim3(grid, a.nsets), dim3(kTpb), 0, st, a);
+    hipLaunchKernelGGL(gather_g_kernel<true>, g3, dim3(kTpb), 0, st, a);
   else
-    hipLaunchKernelGGL(gather_g_kernel<false>, dim3(grid, a.nsets), dim3(kTpb), 0, st, a);
+    hipLaunchKernelGGL(gather_g_kernel<false>, g3, dim3(kTpb), 0, st, a);
   hip_check(hipGetLastError(), "launch gather_g_kernel");
 }
 
@@ -199,8 +202,9 @@ void launch_leaf_sums(hipStream_t st, const Blk* leaves, int B, int D_log, uint6
   const size_t total = (size_t)B << D_log;
   const size_t grid = (total + (size_t)iters * kTpb - 1) / ((size_t)iters * kTpb);
   require(grid < (1ULL << 31), "leaf sums: launch too large");
+  const int chunk_log = ilog2_exact((uint64_t)iters) + ilog2_exact((uint64_t)kTpb);
   hipLaunchKernelGGL(leaf_sums_kernel, dim3((unsigned)grid), dim3(kTpb), 0, st, leaves, B, D_log,
-                     iters, nparts, partials);
+                     iters, chunk_log, nparts, partials);
   hip_check(hipGetLastError(), "launch leaf_sums_kernel");
   launch_fold_sums(st, partials, B, nparts, sumC, sumT);
 }

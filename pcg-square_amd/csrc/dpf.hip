@@ -25,6 +25,15 @@ namespace {
 
 constexpr int kTpb = 256;
 
+// Tiles of kTpb parents per workgroup: a power of two (so a parent's partial
+// slot is a shift, not a division), at most iters_max, at most S / kTpb.
+int level_iters(size_t S, int iters_max) {
+  if (S < (size_t)kTpb) return 1;
+  size_t it = 1;
+  while (it * 2 <= std::min<size_t>(S / kTpb, (size_t)std::max(1, iters_max))) it *= 2;
+  return (int)it;
+}
+
 struct LevelArgs {
   const Blk* Fp;     // parents [ntrees][S]
   const Blk* Hp;     // parents' hashes [ntrees][S][nk]
@@ -38,6 +47,7 @@ struct LevelArgs {
   const uint8_t* aes_rks;
   int peer_shift, npeer;
   int ntrees, S_log, level, iters, nparts;
+  int chunk_log;     // log2(iters * threads): parents per workgroup (a power of two)
 };
 
 enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
@@ -46,7 +56,8 @@ template <int W>
 __device__ __forceinline__ void load_cw(const LevelArgs& a, int T, Blk* CW) {
   constexpr int NK = (1 << W) - 1;
   const Blk* mine = a.cw + ((size_t)(a.level - 1) * a.ntrees + T) * NK;
-  const int pt = (T + a.peer_shift) % a.npeer;
+  int pt = T + a.peer_shift;  // T < npeer and peer_shift < npeer
+  if (pt >= a.npeer) pt -= a.npeer;
   const Blk* peer = a.peer + ((size_t)(a.level - 1) * a.npeer + pt) * NK;
   Blk fold{0, 0};
 #pragma unroll
@@ -140,8 +151,7 @@ __global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
                           red, &first)) {
       const size_t f = base + (size_t)first;
       const int T = (int)(f >> a.S_log);
-      const size_t chunk = (size_t)a.iters * kTpb;
-      const size_t pidx = (a.S_log >= 8) ? ((f & (S - 1)) / chunk) : 0;
+      const size_t pidx = (a.S_log >= 8) ? ((f & (S - 1)) >> a.chunk_log) : 0;
       if (T < a.ntrees) {
         uint64_t* dst = a.part + ((size_t)T * a.nparts + pidx) * 2;
         dst[0] = v[0];
@@ -283,7 +293,8 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     a.ntrees = r.ntrees;
     a.S_log = S_log;
     a.level = i;
-    a.iters = (S >= (size_t)kTpb) ? (int)std::min<size_t>(S / kTpb, (size_t)r.iters_max) : 1;
+    a.iters = level_iters(S, r.iters_max);
+    a.chunk_log = ilog2_exact((uint64_t)a.iters) + ilog2_exact((uint64_t)kTpb);
     if (i < levels) {
       Blk *Fn, *Hn;
       frontier(cur ^ 1, S * M, Fn, Hn);
@@ -312,8 +323,7 @@ int dpf_leaf_nparts(int n, int w, int iters_max) {
   const int levels = n / w;
   const size_t S = (size_t)1 << (w * (levels - 1));
   if (S < (size_t)kTpb) return 1;
-  const size_t iters = std::min<size_t>(S / kTpb, (size_t)iters_max);
-  return (int)(S / (iters * kTpb));
+  return (int)(S / ((size_t)level_iters(S, iters_max) * kTpb));
 }
 
 void dpf_expand(hipStream_t st, const DpfRun& r, DpfScratch& s) {

@@ -7,6 +7,7 @@
 #include "device_common.hpp"
 #include "ntt.hpp"
 #include "ntt_pass.hpp"
+#include "ntt_plan.hpp"
 
 namespace pcgsq {
 
@@ -15,11 +16,6 @@ namespace {
 constexpr int kPassThreads = 512;
 constexpr int kEwThreads = 256;
 
-inline int brv_bits(uint64_t x, int bits) {
-  uint64_t r = 0;
-  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1ULL) << (bits - 1 - i);
-  return (int)r;
-}
 
 __device__ __forceinline__ int brv_dev(int x, int bits) {
   return (int)(__brev((unsigned)x) >> (32 - bits));
@@ -92,113 +88,9 @@ __global__ void pointwise_kernel(uint64_t* out, const uint64_t* a, const uint64_
   out[i] = fmul(a[i], b[i]);
 }
 
-std::vector<uint64_t> with_companions(const std::vector<uint64_t>& w) {
-  std::vector<uint64_t> t(2 * w.size());
-  for (size_t i = 0; i < w.size(); ++i) {
-    t[2 * i] = w[i];
-    t[2 * i + 1] = shoup_companion(w[i]);
-  }
-  return t;
-}
-
-void upload(DevBuf& buf, const std::vector<uint64_t>& v, const char* what) {
-  buf.ensure(v.size() * sizeof(uint64_t), what);
-  hip_check(hipMemcpy(buf.ptr, v.data(), v.size() * sizeof(uint64_t), hipMemcpyHostToDevice),
-            what);
-}
-
 unsigned grid_for(size_t n) { return (unsigned)((n + kEwThreads - 1) / kEwThreads); }
 
 }  // namespace
-
-struct NttPlan {
-  int N = 0, L = 0, n1_log = 0, n2_log = 0, tw_lo_bits = 0;
-  int variant = 0;           // bit 0: alternative stage grouping of the n1 passes, bit 1: n2
-  int cols1 = 0, cols2 = 0;  // columns per tile of the n1 / n2 passes (0 = automatic)
-  uint64_t psi = 0, ninv = 0, ninv_q = 0;
-  DevBuf n1_neg_fwd, n1_neg_inv, n2_cyc_fwd, n2_cyc_inv;
-  DevBuf tw_fwd_lo, tw_fwd_hi, tw_inv_lo, tw_inv_hi;
-  bool naive_ready = false;
-  DevBuf nv_w, nv_winv, nv_twf, nv_twi;
-
-  bool single() const { return n2_log == 0; }
-
-  // n1_log == L: one pass (whole transform in one tile); otherwise the
-  // four-step split N = n1 * n2.
-  NttPlan(int N_, int n1_log_) : N(N_) {
-    L = ilog2_exact((uint64_t)N);
-    n1_log = n1_log_;
-    n2_log = L - n1_log;
-    psi = hpow(3, (kPrime - 1) / (2 * (uint64_t)N));
-    const uint64_t psi_inv = hinv(psi);
-    ninv = hinv((uint64_t)N);
-    ninv_q = shoup_companion(ninv);
-    const int n1 = 1 << n1_log, n2 = 1 << n2_log;
-    // length-n1 negacyclic sub-transform, root psi1 = psi^n2 (order 2*n1)
-    const uint64_t psi1 = hpow(psi, (uint64_t)n2), psi1_inv = hinv(psi1);
-    std::vector<uint64_t> f(n1, 0), fi(n1, 0);
-    for (int k = 1; k < n1; ++k) {
-      f[k] = hpow(psi1, (uint64_t)brv_bits(k, n1_log));
-      fi[k] = hpow(psi1_inv, (uint64_t)brv_bits(k, n1_log));
-    }
-    upload(n1_neg_fwd, with_companions(f), "ntt tables");
-    upload(n1_neg_inv, with_companions(fi), "ntt tables");
-    if (single()) return;
-    // length-n2 cyclic sub-transform, root w2 = psi^(2 n1) (order n2)
-    const uint64_t w2 = hpow(psi, 2 * (uint64_t)n1), w2_inv = hinv(w2);
-    std::vector<uint64_t> g(n2, 0), gi(n2, 0);
-    for (int k = 1; k < n2; ++k) {
-      int s = 0;
-      while ((2 << s) <= k) ++s;
-      const int m = 1 << s, i = k - m;
-      const uint64_t e = (uint64_t)brv_bits(i, s) * (uint64_t)(n2 / (2 * m));
-      g[k] = hpow(w2, e);
-      gi[k] = hpow(w2_inv, e);
-    }
-    upload(n2_cyc_fwd, with_companions(g), "ntt tables");
-    upload(n2_cyc_inv, with_companions(gi), "ntt tables");
-    // two-level psi^(+-e), e < 2N; the inverse high table carries N^-1
-    tw_lo_bits = (L + 2) / 2;
-    const int hi_bits = L + 1 - tw_lo_bits;
-    std::vector<uint64_t> lo(1 << tw_lo_bits), hi(1 << hi_bits), loi(1 << tw_lo_bits),
-        hii(1 << hi_bits);
-    for (size_t x = 0; x < lo.size(); ++x) {
-      lo[x] = hpow(psi, x);
-      loi[x] = hpow(psi_inv, x);
-    }
-    for (size_t y = 0; y < hi.size(); ++y) {
-      hi[y] = hpow(psi, y << tw_lo_bits);
-      hii[y] = hmul(hpow(psi_inv, y << tw_lo_bits), ninv);
-    }
-    upload(tw_fwd_lo, with_companions(lo), "ntt twiddles");
-    upload(tw_fwd_hi, with_companions(hi), "ntt twiddles");
-    upload(tw_inv_lo, with_companions(loi), "ntt twiddles");
-    upload(tw_inv_hi, with_companions(hii), "ntt twiddles");
-  }
-
-  void ensure_naive() {
-    if (naive_ready) return;
-    const uint64_t omega = hmul(psi, psi), omega_inv = hinv(omega);
-    const uint64_t psi_inv = hinv(psi);
-    std::vector<uint64_t> w(std::max(N / 2, 1)), wi(std::max(N / 2, 1)), tf(N), ti(N);
-    w[0] = wi[0] = 1;
-    for (int k = 1; k < N / 2; ++k) {
-      w[k] = hmul(w[k - 1], omega);
-      wi[k] = hmul(wi[k - 1], omega_inv);
-    }
-    tf[0] = 1;
-    ti[0] = ninv;
-    for (int k = 1; k < N; ++k) {
-      tf[k] = hmul(tf[k - 1], psi);
-      ti[k] = hmul(ti[k - 1], psi_inv);
-    }
-    upload(nv_w, with_companions(w), "naive tables");
-    upload(nv_winv, with_companions(wi), "naive tables");
-    upload(nv_twf, with_companions(tf), "naive tables");
-    upload(nv_twi, with_companions(ti), "naive tables");
-    naive_ready = true;
-  }
-};
 
 struct NttEngine::Impl {
   std::mutex mu;
@@ -353,79 +245,41 @@ void launch_pass(hipStream_t st, NttPassDesc d, int nlog, int variant, int want_
             "launch ntt_pass_kernel");
 }
 
-NttPassDesc base_desc(const NttPlan& pl) {
-  NttPassDesc d{};
-  d.N_log = pl.L;
-  d.tw_lo_bits = pl.tw_lo_bits;
-  d.scale_w = pl.ninv;
-  d.scale_wq = pl.ninv_q;
-  return d;
+void launch_spec(hipStream_t st, PassSpec ps, int batch, Profiler* prof, const char* zone) {
+  const int grid_y = ps.kind == kFwdWhole || ps.kind == kInvWhole || ps.kind == kInvWholeProd
+                         ? 1 : batch;
+  switch (ps.kind) {
+    case kFwdCols: return launch_pass<kFwdCols>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kFwdRows: return launch_pass<kFwdRows>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kInvRows: return launch_pass<kInvRows>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kInvRowsProd: return launch_pass<kInvRowsProd>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kInvCols: return launch_pass<kInvCols>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kFwdWhole: return launch_pass<kFwdWhole>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kInvWhole: return launch_pass<kInvWhole>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    case kInvWholeProd: return launch_pass<kInvWholeProd>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
+    default: throw Error(kErrInternal, "ntt: no such pass");
+  }
 }
 
 void four_step_forward(hipStream_t st, NttPlan& pl, const uint64_t* in, uint64_t* tmp,
                        uint64_t* out, int batch, Profiler* prof) {
   if (pl.single()) {
     // whole transforms as columns: one HBM round trip
-    NttPassDesc a = base_desc(pl);
-    a.in = in; a.out = out;
-    a.tab = pl.n1_neg_fwd.as<uint64_t>();
-    a.in_ps_log = 0; a.in_cs_log = pl.L; a.out_ps_log = 0; a.out_cs_log = pl.L;
-    launch_pass<kFwdWhole>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
-                           "ntt.fwd_single");
+    launch_spec(st, plan_pass(pl, kPhFwd1, in, nullptr, out, batch), batch, prof, "ntt.fwd_single");
     return;
   }
-  // pass 1: columns i2, length-n1 negacyclic over i1, twiddle psi^(i2(2k1+1))
-  NttPassDesc a = base_desc(pl);
-  a.in = in; a.out = tmp;
-  a.tab = pl.n1_neg_fwd.as<uint64_t>();
-  a.tw_lo = pl.tw_fwd_lo.as<uint64_t>(); a.tw_hi = pl.tw_fwd_hi.as<uint64_t>();
-  a.in_ps_log = pl.n2_log; a.in_cs_log = 0; a.out_ps_log = pl.n2_log; a.out_cs_log = 0;
-  launch_pass<kFwdCols>(st, a, pl.n1_log, pl.variant & 1, pl.cols1, 1 << pl.n2_log, batch, prof,
-                     "ntt.fwd_pass1");
-  // pass 2: rows k1, length-n2 cyclic over i2, written transposed (natural k)
-  NttPassDesc b = base_desc(pl);
-  b.in = tmp; b.out = out;
-  b.tab = pl.n2_cyc_fwd.as<uint64_t>();
-  b.in_ps_log = 0; b.in_cs_log = pl.n2_log; b.out_ps_log = pl.n1_log; b.out_cs_log = 0;
-  launch_pass<kFwdRows>(st, b, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2, 1 << pl.n1_log, batch,
-                     prof, "ntt.fwd_pass2");
+  launch_spec(st, plan_pass(pl, kPhFwd1, in, nullptr, tmp, batch), batch, prof, "ntt.fwd_pass1");
+  launch_spec(st, plan_pass(pl, kPhFwd2, tmp, nullptr, out, batch), batch, prof, "ntt.fwd_pass2");
 }
 
 void four_step_inverse(hipStream_t st, NttPlan& pl, const uint64_t* in, const uint64_t* in2,
                        uint64_t* tmp, uint64_t* out, int batch, Profiler* prof) {
   if (pl.single()) {
-    NttPassDesc a = base_desc(pl);
-    a.in = in; a.in2 = in2; a.out = out;
-    a.tab = pl.n1_neg_inv.as<uint64_t>();
-    a.in_ps_log = 0; a.in_cs_log = pl.L; a.out_ps_log = 0; a.out_cs_log = pl.L;
-    if (in2)
-      launch_pass<kInvWholeProd>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
-                                 "ntt.inv_single");
-    else
-      launch_pass<kInvWhole>(st, a, pl.L, pl.variant & 1, pl.cols1, batch, 1, prof,
-                             "ntt.inv_single");
+    launch_spec(st, plan_pass(pl, kPhInv1, in, in2, out, batch), batch, prof, "ntt.inv_single");
     return;
   }
-  // pass I1: columns k1 of the [k2][k1] input, inverse cyclic over k2,
-  // twiddle psi^-(i2(2k1+1)) N^-1, rows k1 out
-  NttPassDesc a = base_desc(pl);
-  a.in = in; a.in2 = in2; a.out = tmp;
-  a.tab = pl.n2_cyc_inv.as<uint64_t>();
-  a.tw_lo = pl.tw_inv_lo.as<uint64_t>(); a.tw_hi = pl.tw_inv_hi.as<uint64_t>();
-  a.in_ps_log = pl.n1_log; a.in_cs_log = 0; a.out_ps_log = 0; a.out_cs_log = pl.n2_log;
-  if (in2)
-    launch_pass<kInvRowsProd>(st, a, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2,
-                              1 << pl.n1_log, batch, prof, "ntt.inv_pass1");
-  else
-    launch_pass<kInvRows>(st, a, pl.n2_log, (pl.variant >> 1) & 1, pl.cols2, 1 << pl.n1_log,
-                          batch, prof, "ntt.inv_pass1");
-  // pass I2: columns i2, inverse negacyclic over k1
-  NttPassDesc b = base_desc(pl);
-  b.in = tmp; b.out = out;
-  b.tab = pl.n1_neg_inv.as<uint64_t>();
-  b.in_ps_log = pl.n2_log; b.in_cs_log = 0; b.out_ps_log = pl.n2_log; b.out_cs_log = 0;
-  launch_pass<kInvCols>(st, b, pl.n1_log, pl.variant & 1, pl.cols1, 1 << pl.n2_log, batch, prof,
-                    "ntt.inv_pass2");
+  launch_spec(st, plan_pass(pl, kPhInv1, in, in2, tmp, batch), batch, prof, "ntt.inv_pass1");
+  launch_spec(st, plan_pass(pl, kPhInv2, tmp, nullptr, out, batch), batch, prof, "ntt.inv_pass2");
 }
 
 void naive_forward_brv(hipStream_t st, NttPlan& pl, uint64_t* d, int batch) {

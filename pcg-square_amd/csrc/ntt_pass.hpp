@@ -69,11 +69,19 @@ __device__ __forceinline__ uint32_t brv(uint32_t x, int bits) {
   return bits ? __brev(x) >> (32 - bits) : 0u;
 }
 
+// Device memory reached through a pointer the compiler cannot trace to a
+// kernel argument (a work-list descriptor) would otherwise be accessed with
+// flat instructions; the pass names its buffers as global memory explicitly.
+#define PCGSQ_GLOBAL __attribute__((address_space(1)))
+typedef const PCGSQ_GLOBAL uint64_t* gcptr;
+typedef PCGSQ_GLOBAL uint64_t* gptr;
+
 struct Pair {
   uint64_t w, wq;
 };
-__device__ __forceinline__ Pair ld_pair(const uint64_t* tab, uint32_t i) {
-  const ulonglong2 t = reinterpret_cast<const ulonglong2*>(tab)[i];
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Pair ld_pair(gcptr tab, uint32_t i) {
+  const u64x2 t = reinterpret_cast<const PCGSQ_GLOBAL u64x2*>(tab)[i];
   return Pair{t.x, t.y};
 }
 
@@ -96,7 +104,7 @@ __device__ __forceinline__ uint32_t lds_at(uint32_t c, uint32_t pos, uint32_t co
 // set's block index (its butterfly blocks in stage S0 + q are (Hh << q) | j).
 template <bool INV, int S0, int R>
 __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_t Hh,
-                                                  const uint64_t* __restrict__ tab) {
+                                                  gcptr __restrict__ tab) {
   constexpr int M = 1 << R;
   if constexpr (S0 == 0) Hh = 0;
   if constexpr (!INV) {
@@ -159,9 +167,8 @@ enum : int { kFromHbm = 1, kToHbm = 2, kCopy = 4 };
 template <int K, int NLOG, int S0, int R, int RMAX, int ROLE>
 __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
                                           uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
-                                          const uint64_t* __restrict__ inb,
-                                          const uint64_t* __restrict__ in2b,
-                                          uint64_t* __restrict__ outb) {
+                                          gcptr __restrict__ inb, gcptr __restrict__ in2b,
+                                          gptr __restrict__ outb) {
   constexpr int M = 1 << R;
   constexpr int TMIN = NLOG - S0 - R;         // log2 of the set stride
   constexpr int SLOG = NLOG - R;              // log2 of sets per column
@@ -204,7 +211,7 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
       for (int w = 0; w < M; ++w)
         v[w] = lds[lds_at<NLOG>(c, base | ((uint32_t)w << TMIN), col_stride)];
     }
-    if constexpr (!(ROLE & kCopy)) group_butterflies<Tr::INV, S0, R>(v, Hh, d.tab);
+    if constexpr (!(ROLE & kCopy)) group_butterflies<Tr::INV, S0, R>(v, Hh, (gcptr)d.tab);
     if constexpr (ROLE & kToHbm) {
       const uint32_t two_n_mask = (2u << d.N_log) - 1;
       const uint32_t lo_mask = (1u << d.tw_lo_bits) - 1;
@@ -218,8 +225,8 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
         if constexpr (Tr::STORE == 1) {
           const uint32_t cc = c0 + c;
           const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
-          const Pair a = ld_pair(d.tw_lo, ex & lo_mask);
-          const Pair b = ld_pair(d.tw_hi, ex >> d.tw_lo_bits);
+          const Pair a = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
+          const Pair b = ld_pair((gcptr)d.tw_hi, ex >> d.tw_lo_bits);
           x = shoup_lazy(x, a.w, a.wq);
           x = shoup_lazy(x, b.w, b.wq);
         } else if constexpr (Tr::STORE == 2) {
@@ -257,9 +264,10 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
   const uint32_t t = threadIdx.x;
   const bool active = t < T;
   const uint32_t c0 = ct << d.c_log;
-  const uint64_t* inb = d.in + (b << d.N_log) + ((size_t)c0 << d.in_cs_log);
-  const uint64_t* in2b = KindTraits<K>::PROD ? d.in2 + (b << d.N_log) + ((size_t)c0 << d.in_cs_log) : nullptr;
-  uint64_t* outb = d.out + (b << d.N_log) + ((size_t)c0 << d.out_cs_log);
+  gcptr inb = (gcptr)(d.in + (b << d.N_log) + ((size_t)c0 << d.in_cs_log));
+  gcptr in2b = KindTraits<K>::PROD ? (gcptr)(d.in2 + (b << d.N_log) + ((size_t)c0 << d.in_cs_log))
+                                   : nullptr;
+  gptr outb = (gptr)(d.out + (b << d.N_log) + ((size_t)c0 << d.out_cs_log));
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every
     // element before any bit-reversed store lands

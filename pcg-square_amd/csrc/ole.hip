@@ -52,10 +52,12 @@ constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
 
 __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t* pay, int c, int t,
                                int N) {
-  // e: [2][c][N] zero-filled beforehand; pos/pay: [2][c][t]
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 2 * c * t) return;
-  const int poly = i / t;
+  // e: [2][c][N] zero-filled beforehand; pos/pay: [2][c][t]; one y-block row
+  // per polynomial
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t) return;
+  const int poly = blockIdx.y;
+  const int i = poly * t + k;
   e[(size_t)poly * N + pos[i]] = pay[i];
 }
 
@@ -328,7 +330,7 @@ void plan_step4(pcgsq_ole_plan* pl) {
   hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ne_words + 255) / 256)), dim3(256), 0, st, E,
                      ne_words);
   const int ne = 2 * d.c * d.t;
-  hipLaunchKernelGGL(build_e_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, E,
+  hipLaunchKernelGGL(build_e_kernel, dim3((d.t + 255) / 256, 2 * d.c), dim3(256), 0, st, E,
                      pl->noise.as<uint64_t>(), pl->noise.as<uint64_t>() + ne, d.c, d.t, d.N);
   hip_check(hipGetLastError(), "launch build_e");
   pl->ctx->ntt.prof = &pl->ctx->prof;

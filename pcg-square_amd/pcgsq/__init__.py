@@ -110,6 +110,17 @@ def lib() -> C.CDLL:
     L.pcgsq_ole_g.argtypes = [vp, u64p]
     L.pcgsq_ole_expand.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_uint64, u64p, u64p,
                                    u64p, u64p]
+    L.pcgsq_ntt_batch_create.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(vp)]
+    L.pcgsq_ntt_batch_destroy.argtypes = [vp]
+    L.pcgsq_ntt_batch_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint32]
+    L.pcgsq_ntt_batch_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                      C.POINTER(C.c_size_t)]
+    L.pcgsq_ntt_batch_load.argtypes = [vp, u64p]
+    L.pcgsq_ntt_batch_run.argtypes = [vp, C.c_int]
+    L.pcgsq_ntt_batch_finish.argtypes = [vp]
+    L.pcgsq_ntt_batch_read.argtypes = [vp, u64p, u64p]
+    L.pcgsq_ntt_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
     L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
     L.pcgsq_dealer_u64.restype = C.c_uint64
     _lib = L
@@ -438,6 +449,119 @@ class OlePlan:
 
     def outputs_dev(self) -> int:
         return lib().pcgsq_ole_outputs_dev(self.h)
+
+
+NTT_BATCH_FORWARD, NTT_BATCH_INVERSE = 1, 2
+
+
+class NttBatch:
+    """Transforms of uneven lengths (2^10..2^22) in one persistent launch
+    (pcgsq_ntt_batch_*).  Transform i occupies words offsets[i] ..
+    offsets[i] + 2^logN[i] of the concatenated buffers."""
+
+    def __init__(self, ctx: Context, logN, ops: int = NTT_BATCH_FORWARD | NTT_BATCH_INVERSE):
+        self.ctx = ctx
+        self.logN = [int(x) for x in logN]
+        self.ops = ops
+        arr = (C.c_int * len(self.logN))(*self.logN)
+        h = vp()
+        _check(lib().pcgsq_ntt_batch_create(ctx.h, arr, len(self.logN), ops, C.byref(h)))
+        self.h = h
+        self.offsets = np.concatenate([[0], np.cumsum([1 << x for x in self.logN])]).astype(np.int64)
+        self.total = int(self.offsets[-1])
+
+    def close(self):
+        if self.h:
+            lib().pcgsq_ntt_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, logN: int = 0, n1_log: int = 0, wgs_per_cu: int = 0, spin_limit: int = 0):
+        _check(lib().pcgsq_ntt_batch_tune(self.h, logN, n1_log, wgs_per_cu, spin_limit))
+
+    def load(self, concat: np.ndarray) -> None:
+        a = _u64(concat)
+        assert a.size == self.total
+        _check(lib().pcgsq_ntt_batch_load(self.h, _p(a)))
+
+    def run(self, graph: bool = False) -> None:
+        _check(lib().pcgsq_ntt_batch_run(self.h, int(graph)))
+
+    def finish(self) -> None:
+        _check(lib().pcgsq_ntt_batch_finish(self.h))
+
+    def read(self):
+        f = np.empty(self.total, dtype=np.uint64)
+        r = np.empty(self.total, dtype=np.uint64)
+        _check(lib().pcgsq_ntt_batch_read(self.h, _p(f), _p(r)))
+        return f, r
+
+    def dev(self):
+        a, f, r, n = vp(), vp(), vp(), C.c_size_t()
+        _check(lib().pcgsq_ntt_batch_dev(self.h, C.byref(a), C.byref(f), C.byref(r), C.byref(n)))
+        return a.value, f.value, r.value, n.value
+
+    def info(self) -> dict:
+        n, g, lds, b = C.c_int(), C.c_int(), C.c_size_t(), C.c_double()
+        _check(lib().pcgsq_ntt_batch_info(self.h, C.byref(n), C.byref(g), C.byref(lds),
+                                          C.byref(b)))
+        return {"items": n.value, "grid": g.value, "lds_bytes": lds.value,
+                "alg_bytes": b.value}
+
+    def autotune(self, reps: int = 3, log=None) -> dict:
+        """Coordinate search over the split of every length in the batch and
+        the resident workgroups per CU, timing whole replays of the captured
+        launch on the loaded inputs; keeps the fastest shape.  Shapes never
+        change results."""
+        def timed():
+            self.run(graph=True)
+            self.finish()
+            self.ctx.profile(True)
+            for _ in range(reps):
+                self.run(graph=True)
+            self.finish()
+            z = self.ctx.profile_read()
+            self.ctx.profile(False)
+            return z["ntt.batch"][1] / z["ntt.batch"][0]
+
+        best = {"wgs_per_cu": 0, "splits": {}, "ms": timed()}
+        for L in sorted(set(self.logN), reverse=True):
+            cands = [s for s in range(7, 12) if 6 <= L - s <= 11] + ([L] if L <= 12 else [])
+            keep = 0
+            for sp in cands:
+                self.tune(L, sp, best["wgs_per_cu"])
+                ms = timed()
+                if log:
+                    log(f"batch 2^{L} split {sp}: {ms:.3f} ms")
+                if ms < best["ms"]:
+                    best["ms"], keep = ms, sp
+            self.tune(L, keep, best["wgs_per_cu"])
+            if keep:
+                best["splits"][L] = keep
+        for w in (2, 3, 4):
+            self.tune(0, 0, w)
+            ms = timed()
+            if log:
+                log(f"batch wgs_per_cu {w}: {ms:.3f} ms")
+            if ms < best["ms"]:
+                best["ms"], best["wgs_per_cu"] = ms, w
+        self.tune(0, 0, best["wgs_per_cu"])
+        return best
+
+    def split(self, arr: np.ndarray):
+        """Per-transform views of a concatenated array."""
+        return [arr[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.logN))]
 
 
 def ole_expand(ctx: Context, N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0):
