@@ -216,6 +216,44 @@ int pcgsq_ntt_batch_read(pcgsq_ntt_batch* b, uint64_t* f_host, uint64_t* r_host)
 int pcgsq_ntt_batch_info(pcgsq_ntt_batch* b, int* nitems, int* grid, size_t* lds_bytes,
                          double* alg_bytes);
 
+/* ---- Batched DPF keys of uneven domains -------------------------------------
+ * No single counterpart in the reference: DPF.Gen / DPF.Eval of
+ * half_tree_dpf/dpf.cpp:302-398 (the gen/eval primitives behind the fused
+ * path) for `count` key pairs with domains 2^logn[i] (2^10..2^30), w = 1, and
+ * the output conversion of pcg_ole_impl.h:432-446.  Gen is the dealer form:
+ * roots Delta_s ^ W and the path-only correction words, which equal the
+ * fused protocol's exchange result; then both parties' whole domains are
+ * expanded sync-free in one persistent launch and converted to field shares
+ * y_s(j) = (-1)^s (C_s(j) + tau_s(j) cw_out), y_0 + y_1 = beta [j == alpha].
+ * Outputs: party s, key i at words [s * total + off_i, + 2^logn[i]),
+ * off_i = sum of the earlier domains. */
+typedef struct pcgsq_dpf_batch pcgsq_dpf_batch;
+int pcgsq_dpf_batch_create(pcgsq_ctx* ctx, const int* logn, int count, int prg,
+                           pcgsq_dpf_batch** out);
+int pcgsq_dpf_batch_destroy(pcgsq_dpf_batch* b);
+/* Gen from the dealer's seeded draws (alpha, beta, Delta, Delta_0 share). */
+int pcgsq_dpf_batch_deal(pcgsq_dpf_batch* b, uint64_t seed);
+/* Gen from given points: alpha[i] < 2^logn[i], beta[i] < P, Delta with lsb 1,
+ * Delta_0 the first party's share of Delta. */
+int pcgsq_dpf_batch_gen(pcgsq_dpf_batch* b, const uint64_t* alpha, const uint64_t* beta,
+                        const pcgsq_blk* delta, const pcgsq_blk* delta0);
+/* The dealt points (either pointer may be NULL). */
+int pcgsq_dpf_batch_points(pcgsq_dpf_batch* b, uint64_t* alpha, uint64_t* beta);
+/* The keys: roots [2][count], correction words (key i's n words after the
+ * earlier keys'), output words [count]; any pointer may be NULL. */
+int pcgsq_dpf_batch_keys(pcgsq_dpf_batch* b, pcgsq_blk* roots, pcgsq_blk* cw, uint64_t* cw_out);
+/* Enqueue the full-domain eval of every key of both parties (use_graph:
+ * replay a captured HIP graph of the launch). */
+int pcgsq_dpf_batch_run(pcgsq_dpf_batch* b, int use_graph);
+/* Copy the outputs of party 0 / party 1 (either may be NULL; synchronizes). */
+int pcgsq_dpf_batch_read(pcgsq_dpf_batch* b, uint64_t* y0, uint64_t* y1);
+/* Device pointer of the outputs [2][total] and total. */
+int pcgsq_dpf_batch_dev(pcgsq_dpf_batch* b, uint64_t** y, size_t* total_words);
+/* Work items, grid, and the ChaCha8/AES tree blocks and out-hash blocks of
+ * one run (both parties). */
+int pcgsq_dpf_batch_info(pcgsq_dpf_batch* b, int* nitems, int* grid, double* tree_blocks,
+                         double* out_blocks);
+
 /* ---- AES (aes_gpu.h) -------------------------------------------------------- */
 /* pcg_cuda::aes128_expand_key_host (aes_gpu.h:13). */
 int pcgsq_aes128_expand_key(const uint8_t key[16], uint8_t round_keys[176]);

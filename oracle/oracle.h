@@ -133,6 +133,16 @@ void orc_dpf_path_cw(int n, int w, int prg, const uint8_t* rks, orc_blk root0,
                      orc_blk root1, const orc_blk* dlt0, const orc_blk* dlt1,
                      uint64_t alpha, orc_blk* cw_out);
 
+/* Dealer Gen of a w = 1 key pair from alpha, beta, Delta (lsb 1), Delta_0
+ * share and public W: roots, path-only CW[n], output word cw_out (see
+ * orc_dpf.c).  Eval of party s's whole domain into y[2^n] (field shares,
+ * y_0 + y_1 = beta * [j == alpha]). */
+void orc_dpf_key_gen(int n, int prg, const uint8_t* rks, uint64_t alpha, uint64_t beta,
+                     orc_blk delta, orc_blk d0, orc_blk W, orc_blk* root0, orc_blk* root1,
+                     orc_blk* cw, uint64_t* cw_out);
+void orc_dpf_key_eval_y(int n, int prg, const uint8_t* rks, orc_blk root, const orc_blk* cw,
+                        uint64_t cw_out, int party, uint64_t* y);
+
 /* DltSft for a dealer: value_{i,k} = [digit_i(alpha) == k] * Delta
  * (dpf.cpp:104-130, indicator over bits n - w(i+1) + c). */
 int orc_dpf_digit(int n, int w, int level, uint64_t alpha);

@@ -81,6 +81,12 @@ def lib() -> C.CDLL:
         L.orc_dpf_batch_full_eval_both.restype = C.c_int
         L.orc_dpf_eval.restype = Blk
         L.orc_dpf_eval.argtypes = [C.c_int, C.c_int, C.c_int, u8p, Blk, C.c_void_p, C.c_uint64]
+        L.orc_public_W.restype = Blk
+        L.orc_dpf_key_gen.argtypes = [C.c_int, C.c_int, u8p, C.c_uint64, C.c_uint64, Blk, Blk,
+                                      Blk, C.POINTER(Blk), C.POINTER(Blk), C.c_void_p,
+                                      C.POINTER(C.c_uint64)]
+        L.orc_dpf_key_eval_y.argtypes = [C.c_int, C.c_int, u8p, Blk, C.c_void_p, C.c_uint64,
+                                         C.c_int, C.c_void_p]
         L.orc_dpf_path_cw.argtypes = [C.c_int, C.c_int, C.c_int, u8p, Blk, Blk,
                                       C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         L.orc_dpf_digit.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64]
@@ -275,6 +281,34 @@ def dpf_path_cw(n, w, prg, root0, root1, dlt0, dlt1, alpha, rks=None) -> np.ndar
                           Blk(int(root1[0]), int(root1[1])), dlt0.ctypes.data,
                           dlt1.ctypes.data, alpha, out.ctypes.data)
     return out
+
+
+def public_W() -> tuple[int, int]:
+    """The public W block of the dealt expansion (the F_RAND_SEED constant)."""
+    b = lib().orc_public_W()
+    return (b.lo, b.hi)
+
+
+def dpf_key_gen(n, prg, alpha, beta, delta, d0, W=None, rks=None):
+    """Dealer Gen of a w = 1 key pair: (root0, root1, cw[n][2], cw_out)."""
+    W = public_W() if W is None else W
+    r0, r1, co = Blk(), Blk(), C.c_uint64()
+    cw = np.empty((n, 2), dtype=np.uint64)
+    r = None if rks is None else _p(np.ascontiguousarray(rks, dtype=np.uint8), u8p)
+    lib().orc_dpf_key_gen(n, prg, r, int(alpha), int(beta), Blk(int(delta[0]), int(delta[1])),
+                          Blk(int(d0[0]), int(d0[1])), Blk(int(W[0]), int(W[1])), C.byref(r0),
+                          C.byref(r1), cw.ctypes.data, C.byref(co))
+    return (r0.lo, r0.hi), (r1.lo, r1.hi), cw, co.value
+
+
+def dpf_key_eval_y(n, prg, root, cw, cw_out, party, rks=None) -> np.ndarray:
+    """Party `party`'s converted full domain y[2^n] from its key."""
+    cw = np.ascontiguousarray(cw, dtype=np.uint64).reshape(-1, 2)
+    y = np.empty(1 << n, dtype=np.uint64)
+    r = None if rks is None else _p(np.ascontiguousarray(rks, dtype=np.uint8), u8p)
+    lib().orc_dpf_key_eval_y(n, prg, r, Blk(int(root[0]), int(root[1])), cw.ctypes.data,
+                             int(cw_out), party, y.ctypes.data)
+    return y
 
 
 # ---- output layer ----------------------------------------------------------------

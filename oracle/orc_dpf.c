@@ -220,3 +220,73 @@ void orc_dpf_path_cw(int n, int w, int prg, const uint8_t* rks, orc_blk root0,
         }
     }
 }
+
+/* ---- dealer Gen + sync-free full-domain eval (batched DPF keys) --------- */
+/* Gen from a dealer who knows alpha and both seeds (w = 1).  Roots follow the
+ * fused protocol's setup, root_s = Delta_s ^ W (dpf.cpp:338): Delta_0 = d0,
+ * Delta_1 = d0 ^ delta.  Each level's CW is the path-only value
+ * H(s0) ^ H(s1) ^ [bit_i(alpha) == 0] * delta -- the value the fused exchange
+ * of dpf.cpp:358-385 reconstructs (off-path nodes cancel) -- and the path
+ * advances with the children of dpf.cpp:391-397.  The output word makes the
+ * converted leaves y_s(j) = (-1)^s (C_s(j) + tau_s(j) * cw_out) (the
+ * conversion of pcg_ole_impl.h:432-446) sum to beta * [j == alpha]:
+ * cw_out = (beta - C_0(alpha) + C_1(alpha)) * (tau_0(alpha) - tau_1(alpha)). */
+void orc_dpf_key_gen(int n, int prg, const uint8_t* rks, uint64_t alpha, uint64_t beta,
+                     orc_blk delta, orc_blk d0, orc_blk W, orc_blk* root0, orc_blk* root1,
+                     orc_blk* cw, uint64_t* cw_out) {
+    orc_blk s[2], h[2];
+    s[0] = bx(d0, W);
+    s[1] = bx(bx(d0, delta), W);
+    *root0 = s[0];
+    *root1 = s[1];
+    for (int i = 0; i < n; ++i) {
+        tree_hash(prg, rks, &h[0], &s[0], 1, 1);
+        tree_hash(prg, rks, &h[1], &s[1], 1, 1);
+        const int bit = (int)((alpha >> (n - 1 - i)) & 1);
+        orc_blk c = bx(h[0], h[1]);
+        if (bit == 0) c = bx(c, delta);
+        cw[i] = c;
+        for (int p = 0; p < 2; ++p) {
+            const int t = (int)(s[p].lo & 1);
+            orc_blk nx = bit ? bx(h[p], s[p]) : h[p];
+            if (t) nx = bx(nx, c);
+            s[p] = nx;
+        }
+    }
+    const uint64_t C0 = orc_fmod(orc_out_hash_lo64(s[0]));
+    const uint64_t C1 = orc_fmod(orc_out_hash_lo64(s[1]));
+    const uint64_t v = orc_fsub(orc_fadd(orc_fmod(beta), C1), C0);
+    *cw_out = (s[0].lo & 1) ? v : orc_fneg(v);
+}
+
+/* Party s's full domain from its key (w = 1): every leaf as DPF.Eval of
+ * dpf.cpp:302-327 would compute it (level by level, CW[i] known), then
+ * y(j) = (-1)^s (C(j) + tau(j) * cw_out) with C = lo64(H'(leaf)) mod P. */
+void orc_dpf_key_eval_y(int n, int prg, const uint8_t* rks, orc_blk root, const orc_blk* cw,
+                        uint64_t cw_out, int party, uint64_t* y) {
+    const size_t D = (size_t)1 << n;
+    orc_blk* cur = (orc_blk*)malloc(D * sizeof(orc_blk));
+    orc_blk* nxt = (orc_blk*)malloc(D * sizeof(orc_blk));
+    cur[0] = root;
+    for (int i = 0; i < n; ++i) {
+        const size_t sz = (size_t)1 << i;
+        for (size_t j = 0; j < sz; ++j) {
+            orc_blk h;
+            tree_hash(prg, rks, &h, &cur[j], 1, 1);
+            orc_blk c0 = h;
+            if (cur[j].lo & 1) c0 = bx(c0, cw[i]);
+            nxt[2 * j] = c0;
+            nxt[2 * j + 1] = bx(c0, cur[j]);
+        }
+        orc_blk* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (size_t j = 0; j < D; ++j) {
+        uint64_t v = orc_fmod(orc_out_hash_lo64(cur[j]));
+        if (cur[j].lo & 1) v = orc_fadd(v, cw_out);
+        y[j] = party ? orc_fneg(v) : v;
+    }
+    free(cur);
+    free(nxt);
+}

@@ -121,6 +121,17 @@ def lib() -> C.CDLL:
     L.pcgsq_ntt_batch_read.argtypes = [vp, u64p, u64p]
     L.pcgsq_ntt_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+    L.pcgsq_dpf_batch_create.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(vp)]
+    L.pcgsq_dpf_batch_destroy.argtypes = [vp]
+    L.pcgsq_dpf_batch_deal.argtypes = [vp, C.c_uint64]
+    L.pcgsq_dpf_batch_gen.argtypes = [vp, u64p, u64p, vp, vp]
+    L.pcgsq_dpf_batch_points.argtypes = [vp, u64p, u64p]
+    L.pcgsq_dpf_batch_keys.argtypes = [vp, vp, vp, u64p]
+    L.pcgsq_dpf_batch_run.argtypes = [vp, C.c_int]
+    L.pcgsq_dpf_batch_read.argtypes = [vp, u64p, u64p]
+    L.pcgsq_dpf_batch_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.pcgsq_dpf_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
     L.pcgsq_dealer_u64.restype = C.c_uint64
     _lib = L
@@ -562,6 +573,87 @@ class NttBatch:
     def split(self, arr: np.ndarray):
         """Per-transform views of a concatenated array."""
         return [arr[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.logN))]
+
+
+class DpfBatch:
+    """DPF key pairs of uneven domains (2^10..2^30, w = 1): dealer Gen and
+    both parties' converted full domains in one persistent launch
+    (pcgsq_dpf_batch_*)."""
+
+    def __init__(self, ctx: Context, logn, prg: int = PRG_CHACHA8):
+        self.ctx = ctx
+        self.logn = [int(x) for x in logn]
+        self.prg = prg
+        arr = (C.c_int * len(self.logn))(*self.logn)
+        h = vp()
+        _check(lib().pcgsq_dpf_batch_create(ctx.h, arr, len(self.logn), prg, C.byref(h)))
+        self.h = h
+        self.offsets = np.concatenate([[0], np.cumsum([1 << x for x in self.logn])]).astype(np.int64)
+        self.total = int(self.offsets[-1])
+        self.cw_offsets = np.concatenate([[0], np.cumsum(self.logn)]).astype(np.int64)
+
+    def close(self):
+        if self.h:
+            lib().pcgsq_dpf_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def deal(self, seed: int) -> None:
+        _check(lib().pcgsq_dpf_batch_deal(self.h, seed))
+
+    def gen(self, alpha, beta, delta, delta0) -> None:
+        a, b = _u64(alpha), _u64(beta)
+        d, d0 = _blocks(delta), _blocks(delta0)
+        _check(lib().pcgsq_dpf_batch_gen(self.h, _p(a), _p(b), d.ctypes.data, d0.ctypes.data))
+
+    def points(self):
+        a = np.empty(len(self.logn), dtype=np.uint64)
+        b = np.empty(len(self.logn), dtype=np.uint64)
+        _check(lib().pcgsq_dpf_batch_points(self.h, _p(a), _p(b)))
+        return a, b
+
+    def keys(self):
+        K = len(self.logn)
+        roots = np.empty((2, K, 2), dtype=np.uint64)
+        cw = np.empty((int(self.cw_offsets[-1]), 2), dtype=np.uint64)
+        co = np.empty(K, dtype=np.uint64)
+        _check(lib().pcgsq_dpf_batch_keys(self.h, roots.ctypes.data, cw.ctypes.data, _p(co)))
+        return roots, cw, co
+
+    def run(self, graph: bool = False) -> None:
+        _check(lib().pcgsq_dpf_batch_run(self.h, int(graph)))
+
+    def read(self):
+        y0 = np.empty(self.total, dtype=np.uint64)
+        y1 = np.empty(self.total, dtype=np.uint64)
+        _check(lib().pcgsq_dpf_batch_read(self.h, _p(y0), _p(y1)))
+        return y0, y1
+
+    def dev(self):
+        y, n = vp(), C.c_size_t()
+        _check(lib().pcgsq_dpf_batch_dev(self.h, C.byref(y), C.byref(n)))
+        return y.value, n.value
+
+    def info(self) -> dict:
+        n, g, tb, ob = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        _check(lib().pcgsq_dpf_batch_info(self.h, C.byref(n), C.byref(g), C.byref(tb),
+                                          C.byref(ob)))
+        return {"items": n.value, "grid": g.value, "tree_blocks": tb.value,
+                "out_blocks": ob.value}
+
+    def key_slice(self, arr, i):
+        return arr[self.offsets[i]:self.offsets[i + 1]]
 
 
 def ole_expand(ctx: Context, N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0):

@@ -133,3 +133,108 @@ def test_ntt_batch_spin_limit_reports_error(ctx, pq):
         b.tune(0, 0, 0, 0)                  # default limit again: runs clean
         b.run()
         b.finish()
+
+
+# ---- batched DPF keys (dealer Gen + sync-free eval + leaf convert) -------------
+
+def _rks(prg):
+    return O.aes_tree_round_keys(1) if prg == O.PRG_AES else None
+
+
+def _check_point_function(b, y0, y1, alpha, beta):
+    s = y0 + y1                                   # both < P < 2^62: no overflow
+    s = np.where(s >= np.uint64(P), s - np.uint64(P), s)
+    nz = np.flatnonzero(s)
+    exp = (b.offsets[:-1] + alpha.astype(np.int64))
+    assert np.array_equal(nz, exp), "y0 + y1 is not a point function at the keys' alphas"
+    assert np.array_equal(s[nz], beta)
+
+
+@pytest.mark.parametrize("prg", [O.PRG_CHACHA8, O.PRG_AES])
+def test_dpf_batch_matches_oracle(ctx, pq, prg):
+    rng = np.random.default_rng(10 + prg)
+    logn = [int(x) for x in rng.choice([10, 11, 12, 13, 14, 15, 16], 20)]
+    K = len(logn)
+    alpha = np.array([rng.integers(0, 1 << n) for n in logn], dtype=np.uint64)
+    beta = rng.integers(1, P, K, dtype=np.uint64)
+    delta = rng.integers(0, 2**63, (K, 2), dtype=np.uint64)
+    delta[:, 0] |= np.uint64(1)
+    d0 = rng.integers(0, 2**63, (K, 2), dtype=np.uint64)
+    with pq.DpfBatch(ctx, logn, prg) as b:
+        b.gen(alpha, beta, delta, d0)
+        roots, cw, cw_out = b.keys()
+        b.run()
+        y0, y1 = b.read()
+        _check_point_function(b, y0, y1, alpha, beta)
+        rks = _rks(prg)
+        for k, n in enumerate(logn):
+            r0, r1, ecw, eco = O.dpf_key_gen(n, prg, int(alpha[k]), int(beta[k]), delta[k], d0[k],
+                                             rks=rks)
+            assert tuple(int(v) for v in roots[0, k]) == r0 and tuple(int(v) for v in roots[1, k]) == r1
+            assert np.array_equal(cw[b.cw_offsets[k]:b.cw_offsets[k + 1]], ecw)
+            assert int(cw_out[k]) == eco
+            if k % 3 == 0:
+                assert np.array_equal(b.key_slice(y0, k), O.dpf_key_eval_y(n, prg, r0, ecw, eco, 0, rks=rks))
+                assert np.array_equal(b.key_slice(y1, k), O.dpf_key_eval_y(n, prg, r1, ecw, eco, 1, rks=rks))
+
+
+def test_dpf_batch_dealer_spec(ctx, pq):
+    """deal(seed) draws alpha, beta, Delta, Delta_0 by the documented dealer
+    spec (domains 10-13) and generates the same keys as gen() on them."""
+    logn = [10, 12, 11]
+    seed = 77
+    alpha = np.array([pq.dealer_u64(seed, 10, k, 0) & ((1 << n) - 1) for k, n in enumerate(logn)],
+                     dtype=np.uint64)
+    beta = np.array([pq.dealer_u64(seed, 11, k, 0) % (P - 1) + 1 for k in range(3)], np.uint64)
+    delta = np.array([[pq.dealer_u64(seed, 12, k, 0) | 1, pq.dealer_u64(seed, 12, k, 1)]
+                      for k in range(3)], np.uint64)
+    d0 = np.array([[pq.dealer_u64(seed, 13, k, 0) | 1, pq.dealer_u64(seed, 13, k, 1)]
+                   for k in range(3)], np.uint64)
+    with pq.DpfBatch(ctx, logn) as b:
+        b.deal(seed)
+        a2, b2 = b.points()
+        assert np.array_equal(a2, alpha) and np.array_equal(b2, beta)
+        k1 = b.keys()
+        b.gen(alpha, beta, delta, d0)
+        k2 = b.keys()
+        for u, v in zip(k1, k2):
+            assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("prg", [O.PRG_CHACHA8, O.PRG_AES])
+def test_dpf_batch_config4_workload(ctx, pq, prg):
+    """4096 key pairs, domains 2^10..2^18 (n uniform from mt19937_64), dealt
+    points: both parties' converted domains form beta * e_alpha for every key
+    (graph replay), and a sample of keys equals the oracle."""
+    m = O.MT64(4096 + prg)
+    logn = [10 + int(x) for x in m.take(4096, 9)]
+    with pq.DpfBatch(ctx, logn, prg) as b:
+        b.deal(5)
+        b.run(graph=True)
+        y0, y1 = b.read()
+        alpha, beta = b.points()
+        _check_point_function(b, y0, y1, alpha, beta)
+        roots, cw, cw_out = b.keys()
+        rks = _rks(prg)
+        order = np.argsort(logn)
+        for k in list(order[:3]) + [int(order[-1])]:
+            n = logn[k]
+            ecw = cw[b.cw_offsets[k]:b.cw_offsets[k + 1]]
+            assert np.array_equal(b.key_slice(y0, k),
+                                  O.dpf_key_eval_y(n, prg, roots[0, k], ecw, cw_out[k], 0, rks=rks))
+            assert np.array_equal(b.key_slice(y1, k),
+                                  O.dpf_key_eval_y(n, prg, roots[1, k], ecw, cw_out[k], 1, rks=rks))
+
+
+def test_dpf_batch_graph_replays_new_keys(ctx, pq):
+    """The captured eval replays with keys regenerated in between (every
+    replay checked as a point function)."""
+    rng = np.random.default_rng(12)
+    logn = [int(x) for x in rng.integers(10, 15, 40)]
+    with pq.DpfBatch(ctx, logn) as b:
+        for rep in range(3):
+            b.deal(100 + rep)
+            b.run(graph=True)
+            y0, y1 = b.read()
+            alpha, beta = b.points()
+            _check_point_function(b, y0, y1, alpha, beta)

@@ -158,6 +158,46 @@ def test_dpf_point_function_and_eval(w, prg):
         assert np.array_equal(pcw, cwb)
 
 
+@pytest.mark.parametrize("prg", [O.PRG_CHACHA8, O.PRG_AES])
+@pytest.mark.parametrize("n", [1, 4, 7])
+def test_dpf_dealer_gen_and_key_eval(prg, n):
+    """Dealer Gen (path-only CW) + sync-free full eval of the batched DPF keys:
+    the CWs equal the fused protocol's on the same roots and DltSft shares,
+    the key eval reproduces the fused leaves (through the point eval of every
+    j), and the converted shares sum to beta at alpha and 0 elsewhere."""
+    rks = O.aes_tree_round_keys(1) if prg == O.PRG_AES else None
+    rng = np.random.default_rng(40 + n)
+    for _ in range(4):
+        alpha = int(rng.integers(0, 1 << n))
+        beta = int(rng.integers(1, O.P))
+        delta = [int(rng.integers(0, 2**63)) | 1, int(rng.integers(0, 2**63))]
+        d0 = [int(rng.integers(0, 2**63)) | 1, int(rng.integers(0, 2**63))]
+        r0, r1, cw, cw_out = O.dpf_key_gen(n, prg, alpha, beta, delta, d0, rks=rks)
+        # the fused protocol from the same roots with DltSft shares
+        dl = np.zeros((1, n, 1, 2), dtype=np.uint64)
+        for i in range(n):
+            if ((alpha >> (n - 1 - i)) & 1) == 0:
+                dl[0, i, 0] = delta
+        sh = rng.integers(0, 2**63, (1, n, 1, 2), dtype=np.uint64)
+        l0, l1, fcw = O.dpf_full_eval_both(n, 1, prg, np.array([r0], np.uint64),
+                                           np.array([r1], np.uint64), sh, sh ^ dl, rks=rks,
+                                           want_cw=True)
+        fcw = fcw.reshape(n, 2, 2)
+        assert np.array_equal(fcw[:, 0], cw)              # CW_{i,0}
+        assert np.array_equal(fcw[:, 1], cw)              # fold word = CW_{i,0} at w = 1
+        y0 = O.dpf_key_eval_y(n, prg, r0, cw, cw_out, 0, rks=rks)
+        y1 = O.dpf_key_eval_y(n, prg, r1, cw, cw_out, 1, rks=rks)
+        full_cw = np.repeat(cw, 2, axis=0)                 # [i][k], k = 0, 1
+        for j in range(1 << n):
+            leaf0 = O.dpf_eval(n, 1, prg, r0, full_cw, j, rks=rks)
+            assert leaf0 == tuple(int(v) for v in l0[j])
+            C = int(O.out_hash(np.array([leaf0], np.uint64))[0][0]) % O.P
+            e0 = (C + (cw_out if leaf0[0] & 1 else 0)) % O.P
+            assert int(y0[j]) == e0
+            total = (int(y0[j]) + int(y1[j])) % O.P
+            assert total == (beta if j == alpha else 0)
+
+
 def test_tree_prg_chacha8_is_keystream_chunks():
     rng = np.random.default_rng(9)
     par = rng.integers(0, 2**63, (4, 2), dtype=np.uint64)
