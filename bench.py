@@ -125,6 +125,97 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
             "speedup_vs_one_launch_per_transform": ms_plain / ms}
 
 
+def bench_dpf_batch(ctx, pcgsq, ceiling_blocks: float, reps: int = 5) -> dict:
+    """Batched DPF keys: 4096 key pairs, domains 2^10..2^18 (log-uniform),
+    dealer Gen with random special points and payloads, both parties' full
+    domains with the fused leaf convert in one persistent launch (graph
+    replay); ChaCha8 and AES tree PRGs (ChaCha8 out-hash in both)."""
+    import numpy as np
+    rng = np.random.default_rng(4096)
+    logn = [int(x) for x in rng.integers(10, 19, 4096)]
+    out = {"workload": "4096 DPF key pairs, domains 2^10..2^18, dealer Gen + sync-free "
+                       "full-domain eval of both parties + leaf convert, one persistent launch",
+           "leaves_both_parties": 2 * sum(1 << n for n in logn)}
+    for prg, name in ((pcgsq.PRG_CHACHA8, "chacha8"), (pcgsq.PRG_AES, "aes")):
+        with pcgsq.DpfBatch(ctx, logn, prg) as b:
+            b.deal(5)
+            b.run(graph=True)
+            ctx.synchronize()
+            ctx.profile(True)
+            for _ in range(reps):
+                b.run(graph=True)
+            ctx.synchronize()
+            z = ctx.profile_read()
+            ctx.profile(False)
+            ms = z["dpf.batch"][1] / z["dpf.batch"][0]
+            info = b.info()
+        r = {"ms": ms, "tree_blocks": info["tree_blocks"], "out_hash_blocks": info["out_blocks"],
+             "leaves_per_s": info["out_blocks"] / (ms * 1e-3)}
+        if prg == pcgsq.PRG_CHACHA8:
+            blocks = info["tree_blocks"] + info["out_blocks"]
+            ops = blocks * CHACHA8_OPS / (ms * 1e-3)
+            r.update({"chacha8_blocks_per_s": blocks / (ms * 1e-3), "Tops_per_s": ops / 1e12,
+                      "frac_of_chacha8_ceiling": blocks / (ms * 1e-3) / ceiling_blocks})
+        else:
+            r.update({"aes_blocks_per_s": info["tree_blocks"] / (ms * 1e-3),
+                      "note": "AES tree blocks (T-table AES-128 in LDS) + ChaCha8 out-hash blocks; "
+                              "no algorithmic op model for AES"})
+        out[name] = r
+    return out
+
+
+def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
+    """Served PCG-OLE: Poisson arrivals (rate = 75% of the measured
+    one-at-a-time capacity), N uniform over {2^16, 2^18, 2^20, 2^22}, c=4,
+    t=16, a fresh seed per request; a 4-slot continuous-batching server
+    against one request per launch (1 slot) on the same arrival trace."""
+    from pcgsq import serving
+    sizes = [1 << 16, 1 << 18, 1 << 20, 1 << 22]
+    service = {}
+    with pcgsq.Server(0, 0, 1) as one:
+        for N in sizes:
+            one.reserve(N, 4, 16, 1)
+            lat = []
+            for k in range(4):
+                rid = one.submit(N, 4, 16, 1000 + k)
+                one.wait(rid)
+                a, s, d = one.timing(rid)
+                lat.append(d - s)
+                one.release(rid)
+            service[N] = min(lat[1:])              # the first run builds the plan
+    mean_service = sum(service.values()) / len(sizes)
+    rate = 0.75 / mean_service
+    res = {"workload": "Poisson arrivals, N uniform over 2^16..2^22 (x4 steps), c=4, t=16, "
+                       "fresh seed per request", "rate_per_s": rate, "duration_s": duration_s,
+           "service_ms_alone": {f"2^{N.bit_length() - 1}": v * 1e3 for N, v in service.items()}}
+    for slots, key in ((4, "continuous_batching_4_slots"), (1, "one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots) as srv:
+            for N in sizes:                         # one plan per shape up front
+                srv.reserve(N, 4, 16, 1)
+            recs, wall = serving.run_poisson(srv, rate, duration_s, sizes, 4, 16, seed=7)
+            st = srv.stats()
+        summ = serving.summarize(recs, wall)
+        summ["peak_device_bytes"] = st["peak"]
+        res[key] = summ
+    # saturation: the same 64-request mix submitted at once (capacity, OLEs/s)
+    import numpy as np
+    rng = np.random.default_rng(11)
+    mix = [int(sizes[i]) for i in rng.integers(0, len(sizes), 64)]
+    for slots, key in ((4, "saturated_4_slots"), (1, "saturated_one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots) as srv:
+            for N in sizes:
+                srv.reserve(N, 4, 16, 1)
+            t0 = pcgsq.Server.clock()
+            ids = [srv.submit(N, 4, 16, 5000 + k) for k, N in enumerate(mix)]
+            for rid in ids:
+                srv.wait(rid)
+            wall = max(srv.timing(rid)[2] for rid in ids) - t0
+            for rid in ids:
+                srv.release(rid)
+        res[key] = {"requests": len(mix), "wall_s": wall, "OLEs_per_s": sum(mix) / wall}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +263,8 @@ def main() -> int:
         ceiling = ctx.probe_chacha8()
         copy_bps = ctx.probe_copy(1 << 30)
         ntt_batch = bench_ntt_batch(ctx, pcgsq)
+        dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling)
+    served = bench_served(pcgsq)
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
     n_leaf, ms_leaf = zones["dpf.leaf_convert"]
@@ -227,6 +320,8 @@ def main() -> int:
         "chacha8_ceiling_Gblocks_per_s": ceiling / 1e9,
         "hbm_copy_GBps": copy_bps / 1e9,
         "batched_ntt": ntt_batch,
+        "batched_dpf": dpf_batch,
+        "served": served,
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)

@@ -287,9 +287,12 @@ int pcgsq_ole_plan_create(pcgsq_ctx* ctx, const pcgsq_params* p,
 int pcgsq_ole_plan_destroy(pcgsq_ole_plan* plan);
 /* Device bytes the plan holds. */
 size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* plan);
+/* Device bytes a plan of these parameters and budget would take, including
+ * its context's NTT scratch (0 if the parameters are invalid). */
+size_t pcgsq_ole_plan_estimate(const pcgsq_params* p, size_t hbm_budget_bytes);
 /* Deal the per-expansion inputs from `seed` (the seeded stand-in for the
- * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199)
- * and place them on the device. */
+ * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199):
+ * the dealer runs on the device, asynchronously on the context stream. */
 int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed);
 /* Run the whole expansion for both parties on the device-resident inputs
  * (DPF batches, output layer with Beaver opens, step 4), asynchronously on
@@ -311,6 +314,48 @@ int pcgsq_ole_expand(pcgsq_ctx* ctx, const pcgsq_params* p, uint64_t seed,
  * pcgsq_ole_run that buffer holds NTT-domain values instead. */
 int pcgsq_ole_run_blocks(pcgsq_ole_plan* plan);
 int pcgsq_ole_g(pcgsq_ole_plan* plan, uint64_t* g_host);
+
+/* ---- Serving: continuous batching of expansion requests -------------------
+ * No counterpart in the reference (its benchmarks run one expansion per
+ * process).  A server on `device` runs up to `slots` requests at once, each
+ * on a pooled plan with its own stream; it admits waiting requests, oldest
+ * first, as soon as a slot is free and an idle plan of the request's shape
+ * exists or the device-memory budget (0 = 70% of free memory) holds a new one
+ * (idle plans are evicted least recently used first); running requests
+ * overlap on the device.  Seeds are dealt on the device and outputs stay
+ * there until copied or released.  A request's result does not depend on
+ * what else runs. */
+typedef struct pcgsq_server pcgsq_server;
+enum { PCGSQ_REQ_QUEUED = 0, PCGSQ_REQ_RUNNING = 1, PCGSQ_REQ_DONE = 2, PCGSQ_REQ_FAILED = 3 };
+int pcgsq_server_create(int device, size_t hbm_budget_bytes, int slots, pcgsq_server** out);
+int pcgsq_server_destroy(pcgsq_server* s);
+/* Queue one expansion (PCGSQ_ENOMEM if it can never fit the budget). */
+int pcgsq_server_submit(pcgsq_server* s, const pcgsq_params* p, uint64_t seed,
+                        uint64_t pub_seed, uint64_t* id);
+int pcgsq_server_poll(pcgsq_server* s, uint64_t id, int* state);
+/* Block until the request is done (timeout_s <= 0: no limit). */
+int pcgsq_server_wait(pcgsq_server* s, uint64_t id, double timeout_s);
+/* Copy x0, z0, x1, z1 of a finished request (any pointer may be NULL). */
+int pcgsq_server_outputs(pcgsq_server* s, uint64_t id, uint64_t* x0, uint64_t* z0,
+                         uint64_t* x1, uint64_t* z1);
+/* Device pointer of a finished request's outputs [4][N]. */
+int pcgsq_server_outputs_dev(pcgsq_server* s, uint64_t id, const uint64_t** out);
+/* Drop a finished request and its device outputs. */
+int pcgsq_server_release(pcgsq_server* s, uint64_t id);
+/* Submit, admission and completion times (seconds of pcgsq_server_clock). */
+int pcgsq_server_timing(pcgsq_server* s, uint64_t id, double* t_submit, double* t_start,
+                        double* t_done);
+/* Budget, bytes in use, peak bytes in use, completed and queued requests. */
+int pcgsq_server_stats(pcgsq_server* s, size_t* budget, size_t* used, size_t* peak,
+                       long* completed, int* queued);
+/* Create plans of this shape ahead of requests until `count` exist (they
+ * stay until evicted for memory). */
+int pcgsq_server_reserve(pcgsq_server* s, const pcgsq_params* p, uint64_t pub_seed, int count);
+/* Device bytes the server charges a request of these parameters (its plan
+ * under the server's per-plan budget, plus the outputs). */
+int pcgsq_server_charge(pcgsq_server* s, const pcgsq_params* p, size_t* bytes);
+/* The server's monotonic clock (seconds). */
+double pcgsq_server_clock(void);
 
 /* Dealer value U(seed, domain, a, b) of the documented dealer spec. */
 uint64_t pcgsq_dealer_u64(uint64_t seed, uint32_t domain, uint64_t a,

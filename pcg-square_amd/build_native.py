@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "pcgsq", "libpcgsq.so")
-SOURCES = ["dpf.hip", "convert.hip", "ntt.hip", "ntt_batch.hip", "dpf_batch.hip", "ole.hip", "capi.hip"]
+SOURCES = ["dpf.hip", "convert.hip", "ntt.hip", "ntt_batch.hip", "dpf_batch.hip", "ole.hip", "serve.hip", "capi.hip"]
 ARCH = os.environ.get("PCGSQ_ARCH", "gfx950")
 
 CFLAGS = [

@@ -142,6 +142,10 @@ void NttEngine::tune(int N, int n1_log, int variant, int cols1, int cols2) {
   p->cols2 = cols2;
 }
 
+void NttEngine::trim() {
+  for (DevBuf& b : impl_->scratch) b.release();
+}
+
 uint64_t* NttEngine::scratch(size_t words, int which) {
   impl_->scratch[which].ensure(words * sizeof(uint64_t), "hipMalloc ntt scratch");
   return impl_->scratch[which].as<uint64_t>();

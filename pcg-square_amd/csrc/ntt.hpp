@@ -51,8 +51,9 @@ class NttEngine {
   // out = inverse(x .* y) with x, y in the forward domain (device buffers).
   void inverse_of_product(hipStream_t st, uint64_t* out, const uint64_t* x, const uint64_t* y,
                           int batch, int N);
-  // Scratch for a batch of transforms (grows on demand).
+  // Scratch for a batch of transforms (grows on demand); trim() frees it.
   uint64_t* scratch(size_t words, int which);
+  void trim();
   NttPlan& plan(int N);
   // Shape of the passes for length N (chosen by the autotuner): n1 = 2^n1_log
   // column length (n1_log = log2 N: single pass; 0: default_split), variant

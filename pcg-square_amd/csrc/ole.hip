@@ -41,14 +41,95 @@ namespace pcgsq {
 
 namespace {
 
-uint64_t sm64(uint64_t z) {
+PCGSQ_HD uint64_t sm64(uint64_t z) {
   z += 0x9e3779b97f4a7c15ULL;
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
 
+PCGSQ_HD uint64_t dealer_hd(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
+  return sm64(sm64(sm64(seed ^ ((uint64_t)domain * 0xD6E8FEB86659FD93ULL)) ^ a) ^ b);
+}
+
 constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
+
+// The dealer of the spec above, on the device: the seed is all a request
+// carries to the GPU (t and bin are powers of two since t | N).  Noise: one
+// thread per (i, k), one grid row per party s.
+__global__ void deal_noise_kernel(uint64_t* __restrict__ noise, uint64_t seed, int c, int t_log,
+                                  uint64_t bin) {
+  const int s = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;  // i*t + k
+  const int ct = c << t_log;
+  if (r >= ct) return;
+  const int k = r & ((1 << t_log) - 1);
+  const int o = s * ct + r;
+  noise[o] = (uint64_t)k * bin + (dealer_hd(seed, 1, (uint64_t)s, (uint64_t)r) & (bin - 1));
+  noise[2 * ct + o] = dealer_hd(seed, 2, (uint64_t)s, (uint64_t)r) % (kP - 1) + 1;
+}
+
+// a_0 = 1 and a_i = public_a[i-1] (pcg_ole_impl.h:455-458); grid row i.
+__global__ void public_a_kernel(uint64_t* __restrict__ a, uint64_t pub_seed, int N) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  a[(size_t)i * N + j] =
+      i == 0 ? (j == 0 ? 1u : 0u) : dealer_hd(pub_seed, 9, (uint64_t)(i - 1), (uint64_t)j) % kP;
+}
+
+// Per instance g = blk*B + idx: both roots, both DltSft shares, beta and the
+// Beaver shares.  One thread per instance idx = kk*t + ll; grid rows ii, jj.
+__global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __restrict__ roots,
+                                 Blk* __restrict__ dlt, uint64_t* __restrict__ bt,
+                                 uint64_t seed, int c, int t_log, int levels, int nk, int w,
+                                 int D_log, uint64_t bin, int ninst) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = 1 << t_log, B = t << t_log;
+  if (idx >= B) return;
+  const int ii = blockIdx.y, jj = blockIdx.z;
+  const int blk = ii * c + jj;
+  const int g = blk * B + idx;
+  const int kk = idx >> t_log, ll = idx & (t - 1);
+  const uint64_t* pos = noise;
+  const uint64_t* pay = noise + (size_t)2 * c * t;
+  const uint64_t p0 = pos[(size_t)ii * t + kk], p1 = pos[((size_t)c + jj) * t + ll];
+  const uint64_t alpha = (p0 & (bin - 1)) + (p1 & (bin - 1));
+  const uint64_t G = (uint64_t)g;
+  Blk d0{dealer_hd(seed, 3, 0, 2 * G), dealer_hd(seed, 3, 0, 2 * G + 1)};
+  Blk d1{dealer_hd(seed, 3, 1, 2 * G), dealer_hd(seed, 3, 1, 2 * G + 1)};
+  d0.lo |= 1ULL;   // dpf.cpp:78-82
+  d1.lo &= ~1ULL;
+  const Blk Delta{d0.lo ^ d1.lo, d0.hi ^ d1.hi};
+  const size_t ni = (size_t)ninst;
+  roots[g] = Blk{d0.lo ^ kWlo, d0.hi ^ kWhi};
+  roots[ni + g] = Blk{d1.lo ^ kWlo, d1.hi ^ kWhi};
+  for (int i = 0; i < levels; ++i) {
+    const int digit = (int)((alpha >> (D_log - w * (i + 1))) & (uint64_t)((1 << w) - 1));
+    for (int k = 0; k < nk; ++k) {
+      const uint64_t e = (uint64_t)(i * nk + k);
+      Blk s0{dealer_hd(seed, 4, G, 2 * e), dealer_hd(seed, 4, G, 2 * e + 1)};
+      dlt[((size_t)g * levels + i) * nk + k] = s0;
+      if (digit == k) { s0.lo ^= Delta.lo; s0.hi ^= Delta.hi; }
+      dlt[((ni + g) * levels + i) * nk + k] = s0;
+    }
+  }
+  const uint64_t pay0 = pay[(size_t)ii * t + kk], pay1 = pay[((size_t)c + jj) * t + ll];
+  const uint64_t beta = fmul(pay0, pay1);
+  uint64_t* BETA = bt;
+  uint64_t* R = bt + 2 * ni;
+  uint64_t* S = bt + 4 * ni;
+  uint64_t* RS = bt + 6 * ni;
+  BETA[g] = dealer_hd(seed, 5, G, 0) % kP;
+  BETA[ni + g] = fsub(beta, BETA[g]);
+  R[g] = dealer_hd(seed, 6, G, 0) % kP;
+  R[ni + g] = dealer_hd(seed, 6, G, 1) % kP;
+  S[g] = dealer_hd(seed, 7, G, 0) % kP;
+  S[ni + g] = dealer_hd(seed, 7, G, 1) % kP;
+  const uint64_t rs = fmul(fadd(R[g], R[ni + g]), fadd(S[g], S[ni + g]));
+  RS[g] = dealer_hd(seed, 8, G, 0) % kP;
+  RS[ni + g] = fsub(rs, RS[g]);
+}
 
 __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t* pay, int c, int t,
                                int N) {
@@ -93,7 +174,7 @@ int ilog(uint64_t x) { return ilog2_exact(x); }
 }  // namespace
 
 uint64_t dealer_u64(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
-  return sm64(sm64(sm64(seed ^ ((uint64_t)domain * 0xD6E8FEB86659FD93ULL)) ^ a) ^ b);
+  return dealer_hd(seed, domain, a, b);
 }
 
 struct OleDims {
@@ -153,13 +234,39 @@ size_t plan_group_bytes(const OleDims& d, int gb) {
   return fh + ct;
 }
 
-void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
-  const OleDims& d = pl->d;
+// DPF blocks per group: as many as the budget holds, dividing c^2 (equal
+// groups keep every launch the same shape).
+int plan_group(const OleDims& d, size_t budget) {
   if (budget == 0) budget = (size_t)96 << 30;
   int gb = d.nblk;
   while (gb > 1 && plan_group_bytes(d, gb) > budget) --gb;
-  // equal groups keep every launch the same shape
   while (d.nblk % gb) --gb;
+  return gb;
+}
+
+// Device bytes a plan (with its DPF scratch) and its context's NTT scratch
+// take, without allocating anything.
+size_t plan_estimate(const OleDims& d, size_t budget) {
+  const int gb = plan_group(d, budget);
+  const size_t gtrees = (size_t)2 * gb * d.B;
+  const size_t trees_all = (size_t)2 * d.ninst;
+  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  const size_t nfw = (size_t)2 * d.nblk + 2 * d.c;
+  size_t b = plan_group_bytes(d, gb);
+  b += (size_t)d.levels * gtrees * d.nk * sizeof(Blk);            // dpf cw shares
+  b += trees_all * sizeof(Blk) + trees_all * d.levels * d.nk * sizeof(Blk);
+  b += (size_t)8 * d.ninst * sizeof(uint64_t) + (size_t)4 * d.c * d.t * sizeof(uint64_t);
+  b += (size_t)d.c * d.N * sizeof(uint64_t);                       // ahat
+  b += gtrees * nparts * 2 * sizeof(uint64_t) + (size_t)d.ninst * sizeof(uint64_t);
+  b += 2 * nfw * d.N * sizeof(uint64_t);                           // fw + NTT scratch
+  b += (size_t)4 * d.N * sizeof(uint64_t);                         // outputs
+  return b + ((size_t)1 << 20);
+}
+
+void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
+  const OleDims& d = pl->d;
+  if (budget == 0) budget = (size_t)96 << 30;
+  const int gb = plan_group(d, budget);
   require(plan_group_bytes(d, gb) <= budget || gb == 1, "ole: HBM budget too small", kErrNoMemory);
   pl->group_blocks = gb;
   const size_t trees_all = (size_t)2 * d.ninst;
@@ -187,83 +294,26 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
     hip_check(hipMemcpy(pl->aes_t0.ptr, t0, sizeof(t0), hipMemcpyHostToDevice), "ole aes");
     hip_check(hipMemcpy(pl->aes_rks.ptr, rks.data(), rks.size(), hipMemcpyHostToDevice), "ole aes");
   }
-  // ahat: a_0 = 1, a_i = public_a[i-1]
-  std::vector<uint64_t> a((size_t)d.c * d.N, 0);
-  a[0] = 1;
-  for (int i = 1; i < d.c; ++i)
-    for (int j = 0; j < d.N; ++j)
-      a[(size_t)i * d.N + j] = dealer_u64(pl->pub_seed, 9, (uint64_t)(i - 1), (uint64_t)j) % kPrime;
-  hip_check(hipMemcpy(pl->ahat.ptr, a.data(), a.size() * 8, hipMemcpyHostToDevice), "ole ahat");
+  // ahat: a_0 = 1, a_i = public_a[i-1], generated on the device
+  hipLaunchKernelGGL(public_a_kernel, dim3((unsigned)((d.N + 255) / 256), d.c), dim3(256), 0,
+                     pl->ctx->stream, pl->ahat.as<uint64_t>(), pl->pub_seed, d.N);
+  hip_check(hipGetLastError(), "launch public_a");
   pl->ctx->ntt.forward(pl->ctx->stream, pl->ahat.as<uint64_t>(), d.c, d.N, kNttFourStep);
   hip_check(hipStreamSynchronize(pl->ctx->stream), "ole ahat");
 }
 
 void plan_deal(pcgsq_ole_plan* pl, uint64_t seed) {
   const OleDims& d = pl->d;
-  const uint64_t bin = (uint64_t)d.bin;
-  const int nk = d.nk, levels = d.levels;
-  std::vector<uint64_t> noise((size_t)2 * 2 * d.c * d.t);   // [pos|pay][s][c][t]
-  uint64_t* pos = noise.data();
-  uint64_t* pay = noise.data() + (size_t)2 * d.c * d.t;
-  for (int s = 0; s < 2; ++s)
-    for (int i = 0; i < d.c; ++i)
-      for (int k = 0; k < d.t; ++k) {
-        const size_t o = ((size_t)s * d.c + i) * d.t + k;
-        pos[o] = (uint64_t)k * bin + dealer_u64(seed, 1, (uint64_t)s, (uint64_t)(i * d.t + k)) % bin;
-        pay[o] = dealer_u64(seed, 2, (uint64_t)s, (uint64_t)(i * d.t + k)) % (kPrime - 1) + 1;
-      }
-  const size_t ni = (size_t)d.ninst;
-  std::vector<Blk> roots(2 * ni);
-  std::vector<Blk> dlt(2 * ni * levels * nk);
-  std::vector<uint64_t> bt((size_t)4 * 2 * ni);  // [beta|r|s|rs][party][inst]
-  for (int ii = 0; ii < d.c; ++ii)
-    for (int jj = 0; jj < d.c; ++jj) {
-      const int blk = ii * d.c + jj;
-      for (int idx = 0; idx < d.B; ++idx) {
-        const int kk = idx / d.t, ll = idx % d.t;
-        const uint64_t g = (uint64_t)blk * d.B + idx;
-        const uint64_t p0 = pos[(size_t)ii * d.t + kk], p1 = pos[((size_t)d.c + jj) * d.t + ll];
-        const uint64_t alpha = p0 % bin + p1 % bin;
-        Blk d0{dealer_u64(seed, 3, 0, 2 * g), dealer_u64(seed, 3, 0, 2 * g + 1)};
-        Blk d1{dealer_u64(seed, 3, 1, 2 * g), dealer_u64(seed, 3, 1, 2 * g + 1)};
-        d0.lo |= 1ULL;   // dpf.cpp:78-82
-        d1.lo &= ~1ULL;
-        const Blk Delta{d0.lo ^ d1.lo, d0.hi ^ d1.hi};
-        roots[g] = Blk{d0.lo ^ kWlo, d0.hi ^ kWhi};
-        roots[ni + g] = Blk{d1.lo ^ kWlo, d1.hi ^ kWhi};
-        for (int i = 0; i < levels; ++i) {
-          const int digit = (int)((alpha >> (d.D_log - d.w * (i + 1))) & (uint64_t)((1 << d.w) - 1));
-          for (int k = 0; k < nk; ++k) {
-            const uint64_t e = (uint64_t)(i * nk + k);
-            Blk s0{dealer_u64(seed, 4, g, 2 * e), dealer_u64(seed, 4, g, 2 * e + 1)};
-            dlt[(g * levels + i) * nk + k] = s0;
-            if (digit == k) { s0.lo ^= Delta.lo; s0.hi ^= Delta.hi; }
-            dlt[((ni + g) * levels + i) * nk + k] = s0;
-          }
-        }
-        const uint64_t pay0 = pay[(size_t)ii * d.t + kk], pay1 = pay[((size_t)d.c + jj) * d.t + ll];
-        const uint64_t beta = hmul(pay0, pay1);
-        uint64_t* BETA = bt.data();
-        uint64_t* R = bt.data() + 2 * ni;
-        uint64_t* S = bt.data() + 4 * ni;
-        uint64_t* RS = bt.data() + 6 * ni;
-        BETA[g] = dealer_u64(seed, 5, g, 0) % kPrime;
-        BETA[ni + g] = (beta + kPrime - BETA[g]) % kPrime;
-        R[g] = dealer_u64(seed, 6, g, 0) % kPrime;
-        R[ni + g] = dealer_u64(seed, 6, g, 1) % kPrime;
-        S[g] = dealer_u64(seed, 7, g, 0) % kPrime;
-        S[ni + g] = dealer_u64(seed, 7, g, 1) % kPrime;
-        const uint64_t rs = hmul((R[g] + R[ni + g]) % kPrime, (S[g] + S[ni + g]) % kPrime);
-        RS[g] = dealer_u64(seed, 8, g, 0) % kPrime;
-        RS[ni + g] = (rs + kPrime - RS[g]) % kPrime;
-      }
-    }
   hipStream_t st = pl->ctx->stream;
-  hip_check(hipMemcpyAsync(pl->noise.ptr, noise.data(), noise.size() * 8, hipMemcpyHostToDevice, st), "deal");
-  hip_check(hipMemcpyAsync(pl->roots.ptr, roots.data(), roots.size() * sizeof(Blk), hipMemcpyHostToDevice, st), "deal");
-  hip_check(hipMemcpyAsync(pl->dlt.ptr, dlt.data(), dlt.size() * sizeof(Blk), hipMemcpyHostToDevice, st), "deal");
-  hip_check(hipMemcpyAsync(pl->bt.ptr, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, st), "deal");
-  hip_check(hipStreamSynchronize(st), "deal");
+  const int t_log = ilog((uint64_t)d.t);
+  require(t_log >= 0, "ole: t must be a power of two");
+  hipLaunchKernelGGL(deal_noise_kernel, dim3((unsigned)((d.c * d.t + 255) / 256), 2), dim3(256),
+                     0, st, pl->noise.as<uint64_t>(), seed, d.c, t_log, (uint64_t)d.bin);
+  hipLaunchKernelGGL(deal_inst_kernel, dim3((unsigned)((d.B + 255) / 256), d.c, d.c), dim3(256),
+                     0, st, pl->noise.as<uint64_t>(), pl->roots.as<Blk>(), pl->dlt.as<Blk>(),
+                     pl->bt.as<uint64_t>(), seed, d.c, t_log, d.levels, d.nk, d.w, d.D_log,
+                     (uint64_t)d.bin, d.ninst);
+  hip_check(hipGetLastError(), "launch dealer");
   pl->dealt = true;
 }
 
@@ -367,6 +417,15 @@ int pcgsq_ole_plan_destroy(pcgsq_ole_plan* plan) {
     hip_check(hipStreamSynchronize(plan->ctx->stream), "ole destroy");
     delete plan;
   });
+}
+
+size_t pcgsq_ole_plan_estimate(const pcgsq_params* p, size_t hbm_budget_bytes) {
+  size_t r = 0;
+  const int rc = guarded([&] {
+    require(p, "ole_plan_estimate: null params");
+    r = plan_estimate(make_dims(*p), hbm_budget_bytes);
+  });
+  return rc == PCGSQ_OK ? r : 0;
 }
 
 size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* pl) {

@@ -132,6 +132,25 @@ def lib() -> C.CDLL:
     L.pcgsq_dpf_batch_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.pcgsq_dpf_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pcgsq_ole_plan_estimate.argtypes = [C.POINTER(Params), C.c_size_t]
+    L.pcgsq_ole_plan_estimate.restype = C.c_size_t
+    L.pcgsq_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(vp)]
+    L.pcgsq_server_destroy.argtypes = [vp]
+    L.pcgsq_server_submit.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]
+    L.pcgsq_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
+    L.pcgsq_server_wait.argtypes = [vp, C.c_uint64, C.c_double]
+    L.pcgsq_server_outputs.argtypes = [vp, C.c_uint64, u64p, u64p, u64p, u64p]
+    L.pcgsq_server_outputs_dev.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.pcgsq_server_release.argtypes = [vp, C.c_uint64]
+    L.pcgsq_server_timing.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pcgsq_server_stats.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                     C.POINTER(C.c_size_t), C.POINTER(C.c_long),
+                                     C.POINTER(C.c_int)]
+    L.pcgsq_server_clock.restype = C.c_double
+    L.pcgsq_server_charge.argtypes = [vp, C.POINTER(Params), C.POINTER(C.c_size_t)]
+    L.pcgsq_server_reserve.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_int]
     L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
     L.pcgsq_dealer_u64.restype = C.c_uint64
     _lib = L
@@ -654,6 +673,98 @@ class DpfBatch:
 
     def key_slice(self, arr, i):
         return arr[self.offsets[i]:self.offsets[i + 1]]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
+
+
+class Server:
+    """Continuous-batching server of PCG-OLE expansion requests on one GPU
+    (pcgsq_server_*): `slots` concurrent execution slots, admission by
+    device-memory budget, outputs kept on the device until read/released."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4):
+        h = vp()
+        _check(lib().pcgsq_server_create(device, hbm_budget, slots, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(lib().pcgsq_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, N: int, c: int, t: int, seed: int, pub_seed: int = 0, w: int = 1,
+               prg: int = PRG_CHACHA8) -> int:
+        p = Params(N, c, t, w, prg)
+        rid = C.c_uint64()
+        _check(lib().pcgsq_server_submit(self.h, C.byref(p), seed, pub_seed, C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        st = C.c_int()
+        _check(lib().pcgsq_server_poll(self.h, rid, C.byref(st)))
+        return st.value
+
+    def wait(self, rid: int, timeout: float = 0.0) -> None:
+        _check(lib().pcgsq_server_wait(self.h, rid, timeout))
+
+    def outputs(self, rid: int, N: int):
+        x0, z0, x1, z1 = (np.empty(N, dtype=np.uint64) for _ in range(4))
+        _check(lib().pcgsq_server_outputs(self.h, rid, _p(x0), _p(z0), _p(x1), _p(z1)))
+        return x0, z0, x1, z1
+
+    def outputs_dev(self, rid: int) -> int:
+        p = vp()
+        _check(lib().pcgsq_server_outputs_dev(self.h, rid, C.byref(p)))
+        return p.value
+
+    def release(self, rid: int) -> None:
+        _check(lib().pcgsq_server_release(self.h, rid))
+
+    def timing(self, rid: int):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().pcgsq_server_timing(self.h, rid, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def stats(self) -> dict:
+        b, u, pk, done, q = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_long(), C.c_int()
+        _check(lib().pcgsq_server_stats(self.h, C.byref(b), C.byref(u), C.byref(pk),
+                                        C.byref(done), C.byref(q)))
+        return {"budget": b.value, "used": u.value, "peak": pk.value, "completed": done.value,
+                "queued": q.value}
+
+    def reserve(self, N: int, c: int, t: int, count: int = 1, pub_seed: int = 0, w: int = 1,
+                prg: int = PRG_CHACHA8) -> None:
+        """Create plans of this shape ahead of requests."""
+        _check(lib().pcgsq_server_reserve(self.h, C.byref(Params(N, c, t, w, prg)), pub_seed,
+                                          count))
+
+    def charge(self, N: int, c: int, t: int, w: int = 1, prg: int = PRG_CHACHA8) -> int:
+        """Device bytes this server charges such a request."""
+        b = C.c_size_t()
+        _check(lib().pcgsq_server_charge(self.h, C.byref(Params(N, c, t, w, prg)), C.byref(b)))
+        return b.value
+
+    @staticmethod
+    def clock() -> float:
+        return lib().pcgsq_server_clock()
+
+
+def ole_plan_estimate(N: int, c: int, t: int, w: int = 1, prg: int = PRG_CHACHA8,
+                      hbm_budget: int = 0) -> int:
+    return lib().pcgsq_ole_plan_estimate(C.byref(Params(N, c, t, w, prg)), hbm_budget)
 
 
 def ole_expand(ctx: Context, N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0):

@@ -1,0 +1,129 @@
+"""The expansion server (pcgsq_server_*): continuous batching with admission
+by device-memory budget.  Needs an MI355X.  Every served result is compared
+with the oracle (small N) or with the same request run alone (large N), bit
+for bit, and the OLE relation x0 * x1 = z0 + z1 is checked."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def pq():
+    import pcgsq
+    if not pcgsq.device_available():
+        pytest.fail("no gfx950 device visible (the GPU tests must run on an MI355X)")
+    return pcgsq
+
+
+def _digest(outs):
+    return [O.fnv1a64(v) for v in outs]
+
+
+def test_served_small_mix_matches_oracle(pq):
+    reqs = [(1 << 12, 2, 16, 11), (1 << 13, 2, 16, 12), (1 << 12, 3, 16, 13),
+            (1 << 14, 2, 16, 14), (1 << 12, 2, 16, 15), (1 << 13, 3, 16, 16)]
+    with pq.Server(0, 0, 3) as srv:
+        ids = [srv.submit(N, c, t, seed) for (N, c, t, seed) in reqs]
+        for rid, (N, c, t, seed) in zip(ids, reqs):
+            srv.wait(rid, 60.0)
+            got = srv.outputs(rid, N)
+            exp = O.pcg_ole_expand(N, c, t, 1, O.PRG_CHACHA8, seed=seed, pub_seed=0, threads=8)
+            for g, e, name in zip(got, exp, ("x0", "z0", "x1", "z1")):
+                assert np.array_equal(g, e), f"request N={N} c={c} seed={seed}: {name}"
+            srv.release(rid)
+        assert srv.stats()["completed"] == len(reqs)
+
+
+def test_served_result_same_alone_and_in_mix(pq):
+    """Requests of 2^16..2^22 submitted together overlap on a 4-slot server;
+    each result equals the same request run alone (1 slot, nothing else
+    running), and the OLE relation holds for each."""
+    mix = [(1 << 20, 101), (1 << 16, 102), (1 << 18, 103), (1 << 22, 104), (1 << 16, 105),
+           (1 << 20, 106), (1 << 18, 107), (1 << 16, 108)]
+    mixed = {}
+    with pq.Server(0, 0, 4) as srv:
+        ids = [srv.submit(N, 4, 16, seed) for (N, seed) in mix]
+        overlapped = False
+        for rid, (N, seed) in zip(ids, mix):
+            srv.wait(rid, 120.0)
+            mixed[seed] = _digest(srv.outputs(rid, N))
+        times = [srv.timing(rid) for rid in ids]
+        # some request started before another finished: they overlapped
+        for i, (_, si, di) in enumerate(times):
+            for j, (_, sj, dj) in enumerate(times):
+                if i != j and si < dj and sj < di:
+                    overlapped = True
+        assert overlapped, "no two requests ran at the same time"
+        for rid in ids:
+            srv.release(rid)
+    with pq.Server(0, 0, 1) as alone, pq.Context(0) as ctx:
+        for N, seed in mix:
+            rid = alone.submit(N, 4, 16, seed)
+            alone.wait(rid, 120.0)
+            outs = alone.outputs(rid, N)
+            assert _digest(outs) == mixed[seed], f"N={N} seed={seed}: differs alone vs in the mix"
+            x0, z0, x1, z1 = outs
+            if N <= (1 << 20):
+                prod = ctx.poly_mul(x0[None, :], x1[None, :])[0]
+                s = z0 + z1
+                s = np.where(s >= np.uint64(P), s - np.uint64(P), s)
+                assert np.array_equal(prod, s), f"N={N} seed={seed}: x0*x1 != z0+z1"
+            alone.release(rid)
+
+
+def test_served_admission_respects_memory_budget(pq):
+    """A budget that holds one 2^18 request at a time: four slots, but the
+    requests run one after another and the peak stays within the budget."""
+    budget = 8 << 30
+    for _ in range(8):                       # the per-plan budget follows the total
+        with pq.Server(0, budget, 4) as probe:
+            charge = probe.charge(1 << 18, 4, 16)
+        if charge <= budget < 2 * charge:
+            break
+        budget = charge + charge // 2
+    assert charge <= budget < 2 * charge
+    with pq.Server(0, budget, 4) as srv:
+        ids = [srv.submit(1 << 18, 4, 16, 200 + k) for k in range(4)]
+        for rid in ids:
+            srv.wait(rid, 120.0)
+        st = srv.stats()
+        assert st["peak"] <= budget
+        times = sorted(srv.timing(rid)[1:] for rid in ids)
+        for (s0, d0), (s1, d1) in zip(times, times[1:]):
+            assert s1 >= d0 - 1e-3, "two requests ran together despite the budget"
+        for rid in ids:
+            srv.release(rid)
+
+
+def test_served_rejects_request_beyond_budget(pq):
+    with pq.Server(0, 1 << 30, 2) as srv:
+        with pytest.raises(pq.PcgsqError):
+            srv.submit(1 << 22, 4, 16, 1)
+
+
+def test_served_poisson_run(pq):
+    """A short open-loop Poisson run over the four ring dimensions: every
+    request completes and each output of the smallest size equals the oracle."""
+    from pcgsq import serving
+    checked = []
+
+    def keep(rid, N, seed):
+        if N == (1 << 16) and len(checked) < 2:
+            checked.append((seed, srv.outputs(rid, N)))
+
+    with pq.Server(0, 0, 4) as srv:
+        recs, wall = serving.run_poisson(srv, rate=20.0, duration_s=1.0,
+                                         sizes=[1 << 16, 1 << 18, 1 << 20, 1 << 22], c=4,
+                                         t=16, seed=3, keep=keep)
+    assert len(recs) > 5
+    assert all(r["latency_ms"] > 0 for r in recs)
+    for seed, outs in checked:
+        exp = O.pcg_ole_expand(1 << 16, 4, 16, 1, O.PRG_CHACHA8, seed=seed, pub_seed=0,
+                               threads=16)
+        for g, e in zip(outs, exp):
+            assert np.array_equal(g, e)
