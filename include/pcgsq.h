@@ -357,6 +357,17 @@ int pcgsq_server_charge(pcgsq_server* s, const pcgsq_params* p, size_t* bytes);
 /* The server's monotonic clock (seconds). */
 double pcgsq_server_clock(void);
 
+/* pcg_ole::generate_triples (CPU_baseline/pcg_ole_2pc/triple.h:45-110) for
+ * both parties: the plan's expansion for seed1 and for seed2 (the reference's
+ * two gen_and_expand calls), ring triple shares a_0 = x_{0,1}, b_0 = x_{0,2},
+ * a_1 = x_{1,2}, b_1 = x_{1,1}, c_s = a_s b_s + z_{s,1} + z_{s,2}, and their
+ * forward NTTs: N Z_p triple shares per party with
+ * c0[i] + c1[i] = (a0[i] + a1[i]) (b0[i] + b1[i]).  Outputs [N] (any may be
+ * NULL); synchronizes. */
+int pcgsq_triples_generate(pcgsq_ole_plan* plan, uint64_t seed1, uint64_t seed2, uint64_t* a0,
+                           uint64_t* b0, uint64_t* c0, uint64_t* a1, uint64_t* b1,
+                           uint64_t* c1);
+
 /* Dealer value U(seed, domain, a, b) of the documented dealer spec. */
 uint64_t pcgsq_dealer_u64(uint64_t seed, uint32_t domain, uint64_t a,
                           uint64_t b);

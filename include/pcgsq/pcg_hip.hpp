@@ -261,6 +261,20 @@ class OleExpander {
   struct Output {
     std::vector<uint64_t> x0, z0, x1, z1;   // z0 + z1 = x0 * x1 in R_p
   };
+  // pcg_ole::generate_triples (pcg_ole_2pc/triple.h:45-110) for both parties:
+  // N Z_p triple shares each, from the expansions of seed1 and seed2.
+  struct Triples {
+    std::vector<uint64_t> a0, b0, c0, a1, b1, c1;  // c0 + c1 = (a0 + a1)(b0 + b1) per slot
+  };
+  Triples triples(uint64_t seed1, uint64_t seed2) {
+    Triples t;
+    for (auto* v : {&t.a0, &t.b0, &t.c0, &t.a1, &t.b1, &t.c1}) v->resize(N_);
+    std::lock_guard<std::mutex> lk(detail::mu());
+    detail::check(pcgsq_triples_generate(plan_, seed1, seed2, t.a0.data(), t.b0.data(),
+                                         t.c0.data(), t.a1.data(), t.b1.data(), t.c1.data()));
+    return t;
+  }
+
   Output expand(uint64_t seed) {
     Output o;
     o.x0.resize(N_); o.z0.resize(N_); o.x1.resize(N_); o.z1.resize(N_);

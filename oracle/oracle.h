@@ -205,6 +205,12 @@ orc_blk orc_public_W(void);
 int orc_pcg_ole_expand(const orc_params* p, uint64_t seed, uint64_t pub_seed,
                        uint64_t* x0, uint64_t* z0, uint64_t* x1, uint64_t* z1,
                        int threads);
+/* generate_triples (pcg_ole_2pc/triple.h:45-110): N Z_p triple shares per
+ * party from the OLEs of seed1 and seed2: (a_s[i], b_s[i], c_s[i]) with
+ * c_0 + c_1 = (a_0 + a_1)(b_0 + b_1) per slot. */
+int orc_generate_triples(const orc_params* p, uint64_t seed1, uint64_t seed2, uint64_t pub_seed,
+                         uint64_t* a0, uint64_t* b0, uint64_t* c0, uint64_t* a1,
+                         uint64_t* b1, uint64_t* c1, int threads);
 /* Same, but only blocks [blk_begin, blk_end) of the c*c DPF blocks and no
  * step 4 (for bounded CPU timing samples).  g_red0/1 may be NULL. */
 int orc_pcg_ole_blocks(const orc_params* p, uint64_t seed, int blk_begin,

@@ -101,6 +101,9 @@ def lib() -> C.CDLL:
         L.orc_pcg_ole_expand.argtypes = [C.POINTER(Params), C.c_uint64, C.c_uint64,
                                          u64p, u64p, u64p, u64p, C.c_int]
         L.orc_pcg_ole_expand.restype = C.c_int
+        L.orc_generate_triples.argtypes = [C.POINTER(Params), C.c_uint64, C.c_uint64, C.c_uint64,
+                                           u64p, u64p, u64p, u64p, u64p, u64p, C.c_int]
+        L.orc_generate_triples.restype = C.c_int
         L.orc_pcg_ole_blocks.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_int,
                                          u64p, u64p, C.c_int]
         L.orc_pcg_ole_blocks.restype = C.c_int
@@ -350,6 +353,17 @@ def pcg_ole_expand(N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0, threads=8)
     if rc:
         raise ValueError("orc_pcg_ole_expand: invalid parameters")
     return x0, z0, x1, z1
+
+
+def generate_triples(N, c, t, seed1, seed2, w=1, prg=PRG_CHACHA8, pub_seed=0, threads=8):
+    """(a0, b0, c0, a1, b1, c1), each [N]: the triple shares of triple.h."""
+    p = Params(N, c, t, w, prg)
+    out = [np.empty(N, dtype=np.uint64) for _ in range(6)]
+    rc = lib().orc_generate_triples(C.byref(p), seed1, seed2, pub_seed,
+                                    *[_p(o) for o in out], threads)
+    if rc:
+        raise ValueError("orc_generate_triples: invalid parameters")
+    return tuple(out)
 
 
 def pcg_ole_blocks(N, c, t, w=1, prg=PRG_CHACHA8, seed=1, blk_begin=0, blk_end=1,

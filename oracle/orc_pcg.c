@@ -479,3 +479,42 @@ uint64_t orc_dpf_real_bench_checksum(int n, int B, int t, int w, int threads) {
     free(roots); free(dlt); free(sumC); free(sumT); free(gpart);
     return cks;
 }
+
+/* generate_triples (CPU_baseline/pcg_ole_2pc/triple.h:45-110): two OLEs from
+ * independent seeds, ring triple shares
+ *   a_0 = x_{0,1}, b_0 = x_{0,2}, c_0 = a_0 b_0 + z_{0,1} + z_{0,2},
+ *   a_1 = x_{1,2}, b_1 = x_{1,1}, c_1 = a_1 b_1 + z_{1,1} + z_{1,2},
+ * then the forward NTT of a, b and c of each party: slot i is one Z_p triple
+ * share.  Outputs are [N] each. */
+int orc_generate_triples(const orc_params* p, uint64_t seed1, uint64_t seed2, uint64_t pub_seed,
+                         uint64_t* a0, uint64_t* b0, uint64_t* c0, uint64_t* a1,
+                         uint64_t* b1, uint64_t* c1, int threads) {
+    const int N = p->N;
+    const size_t n = (size_t)N;
+    uint64_t* m = (uint64_t*)malloc(8 * n * sizeof(uint64_t));
+    if (!m) return -1;
+    uint64_t *x01 = m, *z01 = m + n, *x11 = m + 2 * n, *z11 = m + 3 * n;
+    uint64_t *x02 = m + 4 * n, *z02 = m + 5 * n, *x12 = m + 6 * n, *z12 = m + 7 * n;
+    if (orc_pcg_ole_expand(p, seed1, pub_seed, x01, z01, x11, z11, threads) ||
+        orc_pcg_ole_expand(p, seed2, pub_seed, x02, z02, x12, z12, threads)) {
+        free(m);
+        return -1;
+    }
+    memcpy(a0, x01, n * 8);
+    memcpy(b0, x02, n * 8);
+    memcpy(a1, x12, n * 8);
+    memcpy(b1, x11, n * 8);
+    orc_poly_mul(c0, a0, b0, N);
+    orc_poly_mul(c1, a1, b1, N);
+    for (size_t i = 0; i < n; ++i) {
+        c0[i] = orc_fadd(orc_fadd(c0[i], z01[i]), z02[i]);
+        c1[i] = orc_fadd(orc_fadd(c1[i], z11[i]), z12[i]);
+    }
+    orc_ntt* t = orc_ntt_new(N);
+    if (!t) { free(m); return -1; }
+    uint64_t* v[6] = {a0, b0, c0, a1, b1, c1};
+    for (int k = 0; k < 6; ++k) orc_ntt_forward(t, v[k]);
+    orc_ntt_free(t);
+    free(m);
+    return 0;
+}

@@ -69,6 +69,35 @@ __global__ void deal_noise_kernel(uint64_t* __restrict__ noise, uint64_t seed, i
   noise[2 * ct + o] = dealer_hd(seed, 2, (uint64_t)s, (uint64_t)r) % (kP - 1) + 1;
 }
 
+// generate_triples (CPU_baseline/pcg_ole_2pc/triple.h:45-110) from the
+// outputs [x0, z0, x1, z1] of two OLEs: a_0 = x_{0,1}, b_0 = x_{0,2},
+// a_1 = x_{1,2}, b_1 = x_{1,1} and z_s = z_{s,1} + z_{s,2}, into
+// T = [a0, b0, a1, b1, z0, z1][N].
+__global__ void triple_compose_kernel(const uint64_t* __restrict__ o1,
+                                      const uint64_t* __restrict__ o2,
+                                      uint64_t* __restrict__ T, int N) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N) return;
+  const size_t n = (size_t)N;
+  T[i] = o1[i];                                  // a0 = x_{0,1}
+  T[n + i] = o2[i];                              // b0 = x_{0,2}
+  T[2 * n + i] = o2[2 * n + i];                  // a1 = x_{1,2}
+  T[3 * n + i] = o1[2 * n + i];                  // b1 = x_{1,1}
+  T[4 * n + i] = fadd(o1[n + i], o2[n + i]);     // z_{0,1} + z_{0,2}
+  T[5 * n + i] = fadd(o1[3 * n + i], o2[3 * n + i]);
+}
+
+// With every row of T forward-transformed, slot i of the ring triple's c_s =
+// a_s b_s + z_s is A_s[i] B_s[i] + Z_s[i] (the ring product is slot-wise in
+// the NTT domain), written over Z_s.
+__global__ void triple_finish_kernel(uint64_t* __restrict__ T, int N) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N) return;
+  const size_t n = (size_t)N;
+  T[4 * n + i] = fadd(fmul(T[i], T[n + i]), T[4 * n + i]);
+  T[5 * n + i] = fadd(fmul(T[2 * n + i], T[3 * n + i]), T[5 * n + i]);
+}
+
 // a_0 = 1 and a_i = public_a[i-1] (pcg_ole_impl.h:455-458); grid row i.
 __global__ void public_a_kernel(uint64_t* __restrict__ a, uint64_t pub_seed, int N) {
   const int i = blockIdx.y;
@@ -222,6 +251,7 @@ struct pcgsq_ole_plan {
   DevBuf ahat;
   // working set
   DevBuf ct, part, cw, fw, out;
+  DevBuf tri;   // triple generation: first OLE's outputs [4][N] + shares [6][N]
 };
 
 namespace {
@@ -432,7 +462,7 @@ size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* pl) {
   if (!pl) return 0;
   size_t b = 0;
   for (const DevBuf* x : {&pl->roots, &pl->dlt, &pl->bt, &pl->noise, &pl->aes_t0, &pl->aes_rks,
-                          &pl->ahat, &pl->ct, &pl->part, &pl->cw, &pl->fw, &pl->out,
+                          &pl->ahat, &pl->ct, &pl->part, &pl->cw, &pl->fw, &pl->out, &pl->tri,
                           &pl->dpf.fh[0], &pl->dpf.fh[1], &pl->dpf.cw, &pl->dpf.peer})
     b += x->bytes;
   return b;
@@ -480,6 +510,42 @@ int pcgsq_ole_outputs(pcgsq_ole_plan* plan, uint64_t* x0, uint64_t* z0, uint64_t
 
 const uint64_t* pcgsq_ole_outputs_dev(pcgsq_ole_plan* plan) {
   return plan ? plan->out.as<uint64_t>() : nullptr;
+}
+
+int pcgsq_triples_generate(pcgsq_ole_plan* plan, uint64_t seed1, uint64_t seed2, uint64_t* a0,
+                           uint64_t* b0, uint64_t* c0, uint64_t* a1, uint64_t* b1,
+                           uint64_t* c1) {
+  return guarded([&] {
+    require(plan, "triples_generate: null plan");
+    CtxGuard g(plan->ctx);
+    const OleDims& d = plan->d;
+    const size_t n = (size_t)d.N;
+    hipStream_t st = plan->ctx->stream;
+    plan->tri.ensure(10 * n * sizeof(uint64_t), "hipMalloc triples");
+    uint64_t* O1 = plan->tri.as<uint64_t>();
+    uint64_t* T = O1 + 4 * n;
+    ProfZone pz(&plan->ctx->prof, st, "triples");
+    plan_deal(plan, seed1);
+    plan_run_blocks(plan);
+    plan_step4(plan);
+    hip_check(hipMemcpyAsync(O1, plan->out.ptr, 4 * n * 8, hipMemcpyDeviceToDevice, st),
+              "triples copy");
+    plan_deal(plan, seed2);
+    plan_run_blocks(plan);
+    plan_step4(plan);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(triple_compose_kernel, grid, dim3(256), 0, st, O1,
+                       plan->out.as<uint64_t>(), T, d.N);
+    plan->ctx->ntt.forward(st, T, 6, d.N, kNttFourStep);
+    hipLaunchKernelGGL(triple_finish_kernel, grid, dim3(256), 0, st, T, d.N);
+    hip_check(hipGetLastError(), "launch triples");
+    uint64_t* dst[6] = {a0, b0, a1, b1, c0, c1};
+    for (int k = 0; k < 6; ++k)
+      if (dst[k])
+        hip_check(hipMemcpyAsync(dst[k], T + k * n, n * 8, hipMemcpyDeviceToHost, st),
+                  "triples out");
+    hip_check(hipStreamSynchronize(st), "triples");
+  });
 }
 
 int pcgsq_ole_g(pcgsq_ole_plan* plan, uint64_t* g_host) {

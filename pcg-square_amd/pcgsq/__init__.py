@@ -132,6 +132,8 @@ def lib() -> C.CDLL:
     L.pcgsq_dpf_batch_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.pcgsq_dpf_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pcgsq_triples_generate.argtypes = [vp, C.c_uint64, C.c_uint64, u64p, u64p, u64p, u64p,
+                                         u64p, u64p]
     L.pcgsq_ole_plan_estimate.argtypes = [C.POINTER(Params), C.c_size_t]
     L.pcgsq_ole_plan_estimate.restype = C.c_size_t
     L.pcgsq_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(vp)]
@@ -479,6 +481,15 @@ class OlePlan:
 
     def outputs_dev(self) -> int:
         return lib().pcgsq_ole_outputs_dev(self.h)
+
+    def triples(self, seed1: int, seed2: int):
+        """generate_triples (triple.h): (a0, b0, c0, a1, b1, c1), N Z_p triple
+        shares per party from the expansions of seed1 and seed2."""
+        out = [np.empty(self.N, dtype=np.uint64) for _ in range(6)]
+        a0, b0, c0, a1, b1, c1 = out
+        _check(lib().pcgsq_triples_generate(self.h, seed1, seed2, _p(a0), _p(b0), _p(c0),
+                                            _p(a1), _p(b1), _p(c1)))
+        return a0, b0, c0, a1, b1, c1
 
 
 NTT_BATCH_FORWARD, NTT_BATCH_INVERSE = 1, 2

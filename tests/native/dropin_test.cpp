@@ -11,6 +11,7 @@
 //   3. poly_mul patterns of GPU_baseline/test/poly_mul_cuda.cpp:56-71 and the
 //      alias case out == lhs (:121-135), against a schoolbook product
 //   4. OleExpander: x0 * x1 == z0 + z1 at N = 2^12
+//   5. OleExpander::triples: every slot c0 + c1 == (a0 + a1)(b0 + b1)
 // Exit 0 = PASS, 1 = FAIL, 77 = no device (the reference's skip code).
 #include <stdio.h>
 #include <string.h>
@@ -158,6 +159,15 @@ int main() {
     for (int i = 0; i < N; ++i) ok &= prod[i] == (o.z0[i] + o.z1[i]) % P;
     CHECK(ok, "x0 * x1 == z0 + z1");
     printf("OLE relation N=%d: %s\n", N, ok ? "holds" : "FAILS");
+    // 5. generate_triples (triple.h:45-110): every slot a Z_p triple
+    auto tr = ole.triples(5, 6);
+    bool tok = true;
+    for (int i = 0; i < N; ++i) {
+      const uint64_t a = (tr.a0[i] + tr.a1[i]) % P, b = (tr.b0[i] + tr.b1[i]) % P;
+      tok &= mulm(a, b) == (tr.c0[i] + tr.c1[i]) % P;
+    }
+    CHECK(tok, "triples: c0 + c1 == (a0 + a1)(b0 + b1)");
+    printf("triples N=%d: %s\n", N, tok ? "hold" : "FAIL");
   }
   printf("%s\n", fails ? "RESULT: FAIL" : "RESULT: PASS");
   return fails ? 1 : 0;

@@ -263,3 +263,37 @@ def test_ole_relation_full_size(ctx, pq):
     s = (z0 + z1) % np.uint64(P)   # z < P < 2^62: no uint64 overflow
     assert np.array_equal(prod, s)
     assert int((x0 != 0).sum()) > N // 2
+
+
+@pytest.mark.parametrize("N,c", [(1 << 12, 2), (1 << 13, 3)])
+def test_triples_match_oracle(ctx, pq, N, c):
+    """generate_triples (triple.h:45-110) on the device equals the oracle's."""
+    with pq.OlePlan(ctx, N, c, 16) as pl:
+        got = pl.triples(21, 22)
+    exp = O.generate_triples(N, c, 16, 21, 22, threads=8)
+    for g, e, name in zip(got, exp, ("a0", "b0", "c0", "a1", "b1", "c1")):
+        assert np.array_equal(g, e), name
+
+
+def test_triples_full_size_are_triples(ctx, pq):
+    """N = 2^20, c = 4, t = 16: every one of the N slots is a Z_p
+    multiplication triple, c0 + c1 = (a0 + a1)(b0 + b1)."""
+    N = 1 << 20
+    with pq.OlePlan(ctx, N, 4, 16) as pl:
+        a0, b0, c0, a1, b1, c1 = pl.triples(31, 32)
+    Pn = np.uint64(P)
+
+    def add(x, y):
+        s = x + y
+        return np.where(s >= Pn, s - Pn, s)
+    a, b, cc = add(a0, a1), add(b0, b1), add(c0, c1)
+    idx = np.random.default_rng(0).choice(N, 4096, replace=False)
+    for i in idx:
+        assert (int(a[i]) * int(b[i])) % P == int(cc[i])
+    # all slots at once: a*b mod P through 31-bit limbs in float-free numpy
+    lo = np.uint64((1 << 31) - 1)
+    al, ah = a & lo, a >> np.uint64(31)
+    bl, bh = b & lo, b >> np.uint64(31)
+    prod = [int(x) for x in ((al * bl).astype(object) + ((al * bh + ah * bl).astype(object) << 31)
+                             + ((ah * bh).astype(object) << 62))]
+    assert all(p % P == int(v) for p, v in zip(prod, cc))

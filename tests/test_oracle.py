@@ -233,6 +233,26 @@ def test_ole_relation(N, c, t, prg):
     assert int((x0 != 0).sum()) > N // 2            # x is dense (a_i e_i)
 
 
+@pytest.mark.parametrize("N,c", [(1 << 12, 2), (1 << 13, 3)])
+def test_generate_triples_are_triples(N, c):
+    """triple.h:45-110 restated: every slot is a Z_p multiplication triple
+    (c0 + c1 = (a0 + a1)(b0 + b1)), the a/b shares are the forward NTTs of the
+    right OLE outputs, and a planted error in one c share is caught."""
+    a0, b0, c0, a1, b1, c1 = O.generate_triples(N, c, 16, 3, 4, threads=8)
+    a = (a0.astype(object) + a1.astype(object)) % O.P
+    b = (b0.astype(object) + b1.astype(object)) % O.P
+    cc = (c0.astype(object) + c1.astype(object)) % O.P
+    assert ((a * b) % O.P == cc).all()
+    x01, _, x11, _ = O.pcg_ole_expand(N, c, 16, seed=3, threads=8)
+    x02, _, x12, _ = O.pcg_ole_expand(N, c, 16, seed=4, threads=8)
+    assert np.array_equal(a0, O.ntt_forward(x01)) and np.array_equal(b0, O.ntt_forward(x02))
+    assert np.array_equal(a1, O.ntt_forward(x12)) and np.array_equal(b1, O.ntt_forward(x11))
+    bad = c0.copy()
+    bad[N // 3] = (int(bad[N // 3]) + 1) % O.P
+    cc = (bad.astype(object) + c1.astype(object)) % O.P
+    assert not ((a * b) % O.P == cc).all()
+
+
 def test_ole_relation_fails_on_planted_error():
     # a planted mistake in the oracle's output must be caught by the relation
     N = 1 << 10
