@@ -39,22 +39,23 @@ def percentile(xs, q):
     return s[k]
 
 
-def pmc_traffic(path: str):
-    """HBM bytes per launch of the dominant kernel from a committed
-    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE), if present."""
+def pmc_traffic(path: str, key: str = "hbm_bytes_per_launch"):
+    """HBM bytes per launch of one kernel from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE, the line `key = bytes`), if present."""
     try:
         txt = open(path).read()
     except OSError:
         return None
-    m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    m = re.search(r"^" + re.escape(key) + r"\s*=\s*([0-9.eE+]+)", txt, re.M)
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(N: int, c: int, t: int, threads: int) -> dict:
+def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) -> dict:
     """The CPU oracle (a C restatement of CPU_baseline's local compute) on a
     bounded sample: one DPF block of the c^2 (both parties, DPF + output
-    layer) and the c + 2c^2 poly-muls of step 4, each with `threads` OpenMP
-    threads; the full expansion is c^2 blocks + step 4."""
+    layer) and the c + 2c^2 poly-muls of step 4 (or `polymul_sample` of them,
+    scaled), each with `threads` OpenMP threads; the full expansion is c^2
+    blocks + step 4."""
     import numpy as np
 
     import oracle as O
@@ -62,16 +63,18 @@ def cpu_baseline(N: int, c: int, t: int, threads: int) -> dict:
     O.pcg_ole_blocks(N, c, t, 1, O.PRG_CHACHA8, seed=1, blk_begin=0, blk_end=1, threads=threads)
     t_block = time.perf_counter() - t0
     npm = c + 2 * c * c
+    ns = polymul_sample or npm
     rng = np.random.default_rng(0)
-    a = rng.integers(0, O.P, (npm, N), dtype=np.uint64)
-    b = rng.integers(0, O.P, (npm, N), dtype=np.uint64)
+    a = rng.integers(0, O.P, (ns, N), dtype=np.uint64)
+    b = rng.integers(0, O.P, (ns, N), dtype=np.uint64)
     t0 = time.perf_counter()
     O.poly_mul_batch(a, b, threads=threads)
-    t_step4 = time.perf_counter() - t0
+    t_step4 = (time.perf_counter() - t0) * npm / ns
     full = c * c * t_block + t_step4
+    pm = f"{npm} poly-muls" if ns == npm else f"{npm} poly-muls (timed {ns}, scaled)"
     return {"value": N / full, "unit": "OLEs/s", "cores": threads, "kind": "port",
             "sample": (f"oracle (C, u128 %P, per-node ChaCha8) on {threads} host threads: 1 of "
-                       f"{c * c} DPF blocks both parties ({t_block:.2f} s) + {npm} poly-muls of "
+                       f"{c * c} DPF blocks both parties ({t_block:.2f} s) + {pm} of "
                        f"step 4 ({t_step4:.2f} s); expansion = {c * c} x block + step 4 = "
                        f"{full:.1f} s")}
 
@@ -122,7 +125,12 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
                                             "splits": {f"2^{k}": v for k, v in
                                                        shape["splits"].items()}},
             "one_launch_per_transform_ms": ms_plain,
-            "speedup_vs_one_launch_per_transform": ms_plain / ms}
+            "speedup_vs_one_launch_per_transform": ms_plain / ms,
+            "roofline": {"bound": "hbm", "kernel": "ntt_batch_kernel", "achieved": gbps,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": gbps * 1e9 / HBM_PEAK,
+                         "traffic": pmc_traffic(os.path.join(
+                             ROOT, "profiles", "ntt_batch_256_uneven_2e12_2e20.md")),
+                         "avg_launch_ms": ms}}
 
 
 def bench_dpf_batch(ctx, pcgsq, ceiling_blocks: float, reps: int = 5) -> dict:
@@ -188,23 +196,33 @@ def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     res = {"workload": "Poisson arrivals, N uniform over 2^16..2^22 (x4 steps), c=4, t=16, "
                        "fresh seed per request", "rate_per_s": rate, "duration_s": duration_s,
            "service_ms_alone": {f"2^{N.bit_length() - 1}": v * 1e3 for N, v in service.items()}}
-    for slots, key in ((4, "continuous_batching_4_slots"), (1, "one_request_per_launch")):
-        with pcgsq.Server(0, 0, slots) as srv:
-            for N in sizes:                         # one plan per shape up front
-                srv.reserve(N, 4, 16, 1)
+    # plans up front: two per shape for the 4-slot server (requests of one
+    # shape overlap on them), one per shape for one request per launch; the
+    # 4-slot server's work window lets one 2^22 request run beside smaller
+    # ones but starts the large requests in arrival order
+    window = (1 << 22) + (1 << 19)
+    res["window_N"] = window
+    for slots, per_shape, key in ((4, 2, "continuous_batching_4_slots"),
+                                  (1, 1, "one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots, window=window if slots > 1 else 0) as srv:
+            for N in sizes:
+                srv.reserve(N, 4, 16, per_shape)
             recs, wall = serving.run_poisson(srv, rate, duration_s, sizes, 4, 16, seed=7)
-            st = srv.stats()
+            st, pl = srv.stats(), srv.plans()
         summ = serving.summarize(recs, wall)
+        summ["p50_p99_ms_by_N"] = serving.by_size(recs)
         summ["peak_device_bytes"] = st["peak"]
+        summ["plans"] = pl
         res[key] = summ
     # saturation: the same 64-request mix submitted at once (capacity, OLEs/s)
     import numpy as np
     rng = np.random.default_rng(11)
     mix = [int(sizes[i]) for i in rng.integers(0, len(sizes), 64)]
-    for slots, key in ((4, "saturated_4_slots"), (1, "saturated_one_request_per_launch")):
-        with pcgsq.Server(0, 0, slots) as srv:
+    for slots, per_shape, key in ((4, 2, "saturated_4_slots"),
+                                  (1, 1, "saturated_one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots, window=window if slots > 1 else 0) as srv:
             for N in sizes:
-                srv.reserve(N, 4, 16, 1)
+                srv.reserve(N, 4, 16, per_shape)
             t0 = pcgsq.Server.clock()
             ids = [srv.submit(N, 4, 16, 5000 + k) for k, N in enumerate(mix)]
             for rid in ids:
@@ -225,7 +243,14 @@ def main() -> int:
     ap.add_argument("--c", type=int, default=4)
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,served",
+                    help="comma list of expand, ntt_batch, dpf_batch, served; without expand "
+                         "only the named parts run and print (for profiling one workload)")
     args = ap.parse_args()
+    parts = set(args.parts.split(","))
+    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "served"}
+    if unknown:
+        ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the expansion "
               "shards by DPF block across GPUs but no multi-GPU run is provided", file=sys.stderr)
@@ -234,6 +259,18 @@ def main() -> int:
     import pcgsq
 
     N, c, t = args.N, args.c, args.t
+    if "expand" not in parts:
+        res = {}
+        with pcgsq.Context(0) as ctx:
+            res["hbm_copy_GBps"] = ctx.probe_copy(1 << 30) / 1e9   # PMC calibration kernel
+            if "ntt_batch" in parts:
+                res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq)
+            if "dpf_batch" in parts:
+                res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
+        if "served" in parts:
+            res["served"] = bench_served(pcgsq)
+        print(json.dumps(res), flush=True)
+        return 0
     with pcgsq.Context(0) as ctx, pcgsq.OlePlan(ctx, N, c, t, 1, pcgsq.PRG_CHACHA8) as pl:
         pl.deal(1)
         for _ in range(args.warmup):
@@ -262,9 +299,9 @@ def main() -> int:
         ctx.profile(False)
         ceiling = ctx.probe_chacha8()
         copy_bps = ctx.probe_copy(1 << 30)
-        ntt_batch = bench_ntt_batch(ctx, pcgsq)
-        dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling)
-    served = bench_served(pcgsq)
+        ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
+        dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
+    served = bench_served(pcgsq) if "served" in parts else None
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
     n_leaf, ms_leaf = zones["dpf.leaf_convert"]
@@ -272,7 +309,8 @@ def main() -> int:
     ach = leaves * CHACHA8_OPS / (avg_leaf * 1e-3)
     peak_meas = ceiling * CHACHA8_OPS
     peak = max(VALU_NOMINAL, peak_meas)
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "dpf_leaf_convert_N20_c4_t16_pmc.txt"))
+    traffic_file = os.path.join(ROOT, "profiles", "pcg_ole_expand_N20_c4_t16_hbm_traffic.md")
+    traffic = pmc_traffic(traffic_file)
     roofline = {"bound": "valu", "kernel": "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + "
                                            "out-hash)",
                 "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "Tops/s",
@@ -292,7 +330,8 @@ def main() -> int:
                 % (nf, N), "achieved": ntt_bytes_per_pass / (ntt_avg * 1e-3) / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ntt_bytes_per_pass / (ntt_avg * 1e-3) / HBM_PEAK,
-                "traffic": None, "avg_launch_ms": ntt_avg}
+                "traffic": pmc_traffic(traffic_file, "ntt_fwd_pass_hbm_bytes_per_launch"),
+                "avg_launch_ms": ntt_avg}
     out = {
         "metric": METRIC,
         "value": N * args.steps / dt,
@@ -326,6 +365,10 @@ def main() -> int:
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(N, c, t, threads)
+        # the reference-faithful mode: one thread per party, both parties
+        out["cpu_baseline_one_thread_per_party"] = cpu_baseline(N, c, t, 2, polymul_sample=4)
+        out["cpu_published_context"] = ("CPU_baseline 2PC at N=2^20, c=4, t=16 on a Xeon Gold "
+                                        "6442Y: 90.8 s wall, 11.5 k OLEs/s (reference results)")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -318,13 +318,16 @@ int pcgsq_ole_g(pcgsq_ole_plan* plan, uint64_t* g_host);
 /* ---- Serving: continuous batching of expansion requests -------------------
  * No counterpart in the reference (its benchmarks run one expansion per
  * process).  A server on `device` runs up to `slots` requests at once, each
- * on a pooled plan with its own stream; it admits waiting requests, oldest
- * first, as soon as a slot is free and an idle plan of the request's shape
- * exists or the device-memory budget (0 = 70% of free memory) holds a new one
- * (idle plans are evicted least recently used first); running requests
- * overlap on the device.  Seeds are dealt on the device and outputs stay
- * there until copied or released.  A request's result does not depend on
- * what else runs. */
+ * on a pooled plan with its own stream.  When a slot is free it admits the
+ * oldest waiting request that can start now: one whose shape has an idle
+ * plan (requests of one shape keep their order).  A shape with no plan gets
+ * one built within the device-memory budget (0 = 70% of free memory; idle
+ * plans of other shapes are evicted least recently used first); further
+ * plans of a shape, so that its requests overlap, come from
+ * pcgsq_server_reserve.  Running requests overlap on the device.  Seeds are
+ * dealt on the device and outputs stay there, in buffers pooled by size,
+ * until copied or released.  A request's result does not depend on what else
+ * runs. */
 typedef struct pcgsq_server pcgsq_server;
 enum { PCGSQ_REQ_QUEUED = 0, PCGSQ_REQ_RUNNING = 1, PCGSQ_REQ_DONE = 2, PCGSQ_REQ_FAILED = 3 };
 int pcgsq_server_create(int device, size_t hbm_budget_bytes, int slots, pcgsq_server** out);
@@ -340,7 +343,7 @@ int pcgsq_server_outputs(pcgsq_server* s, uint64_t id, uint64_t* x0, uint64_t* z
                          uint64_t* x1, uint64_t* z1);
 /* Device pointer of a finished request's outputs [4][N]. */
 int pcgsq_server_outputs_dev(pcgsq_server* s, uint64_t id, const uint64_t** out);
-/* Drop a finished request and its device outputs. */
+/* Drop a finished request; its output buffer returns to the server's pool. */
 int pcgsq_server_release(pcgsq_server* s, uint64_t id);
 /* Submit, admission and completion times (seconds of pcgsq_server_clock). */
 int pcgsq_server_timing(pcgsq_server* s, uint64_t id, double* t_submit, double* t_start,
@@ -354,6 +357,14 @@ int pcgsq_server_reserve(pcgsq_server* s, const pcgsq_params* p, uint64_t pub_se
 /* Device bytes the server charges a request of these parameters (its plan
  * under the server's per-plan budget, plus the outputs). */
 int pcgsq_server_charge(pcgsq_server* s, const pcgsq_params* p, size_t* bytes);
+/* Work window: the most ring dimension (sum of N) that running requests may
+ * hold at once (0, the default: no limit).  A request that would exceed it
+ * waits unless nothing runs, and so does every later request at least as
+ * large, so that large requests start in arrival order while smaller ones
+ * fill the slots around them. */
+int pcgsq_server_set_window(pcgsq_server* s, uint64_t max_running_N);
+/* Plans in the pool, plans built and plans evicted since creation. */
+int pcgsq_server_plans(pcgsq_server* s, int* plans, long* built, long* evicted);
 /* The server's monotonic clock (seconds). */
 double pcgsq_server_clock(void);
 

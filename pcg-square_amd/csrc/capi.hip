@@ -194,7 +194,7 @@ int pcgsq_ctx_create(int device, pcgsq_ctx** out) {
 int pcgsq_ctx_destroy(pcgsq_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    {
+    if (ctx->owns_stream) {
       CtxGuard g(ctx);
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamDestroy(ctx->stream);

@@ -12,6 +12,7 @@
 struct pcgsq_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;   // false: the stream is lent (a server's slot stream)
   std::recursive_mutex mu;
   pcgsq::DpfScratch dpf;
   pcgsq::NttEngine ntt;

@@ -228,15 +228,7 @@ template <int W, int PRG>
 void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
   constexpr int M = 1 << W, NK = M - 1;
   const int n = r.n, levels = n / W;
-  const size_t maxS = (size_t)1 << (W * (levels - 1));  // frontier at the last parent level
-  // The widest stored level is the last parent level (maxS nodes per tree),
-  // each node with its NK hashes.
-  const size_t need = (size_t)r.ntrees * maxS * (1 + NK) * sizeof(Blk);
-  s.fh[0].ensure(need, "hipMalloc dpf frontier");
-  s.fh[1].ensure(need, "hipMalloc dpf frontier");
-  const size_t cw_bytes = (size_t)levels * r.ntrees * NK * sizeof(Blk);
-  s.cw.ensure(cw_bytes, "hipMalloc dpf cw");
-  if (r.host_ex) s.peer.ensure(cw_bytes, "hipMalloc dpf peer");
+  dpf_reserve(s, n, W, (size_t)r.ntrees, r.host_ex != nullptr);
 
   auto frontier = [&](int which, size_t S, Blk*& F, Blk*& H) {
     F = s.fh[which].as<Blk>();
@@ -313,7 +305,8 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     }
   }
   if (r.cw_out)
-    hip_check(hipMemcpyAsync(r.cw_out, s.cw.as<Blk>(), cw_bytes, hipMemcpyDeviceToDevice, st),
+    hip_check(hipMemcpyAsync(r.cw_out, s.cw.as<Blk>(), (size_t)levels * r.ntrees * NK * sizeof(Blk),
+                             hipMemcpyDeviceToDevice, st),
               "copy dpf cw");
 }
 
@@ -324,6 +317,19 @@ int dpf_leaf_nparts(int n, int w, int iters_max) {
   const size_t S = (size_t)1 << (w * (levels - 1));
   if (S < (size_t)kTpb) return 1;
   return (int)(S / ((size_t)level_iters(S, iters_max) * kTpb));
+}
+
+void dpf_reserve(DpfScratch& s, int n, int w, size_t ntrees, bool host_exchange) {
+  const int nk = (1 << w) - 1, levels = n / w;
+  // The widest stored level is the last parent level (2^(w(levels-1)) nodes
+  // per tree), each node with its nk hashes.
+  const size_t maxS = (size_t)1 << (w * (levels - 1));
+  const size_t need = ntrees * maxS * (size_t)(1 + nk) * sizeof(Blk);
+  s.fh[0].ensure(need, "hipMalloc dpf frontier");
+  s.fh[1].ensure(need, "hipMalloc dpf frontier");
+  const size_t cw_bytes = (size_t)levels * ntrees * (size_t)nk * sizeof(Blk);
+  s.cw.ensure(cw_bytes, "hipMalloc dpf cw");
+  if (host_exchange) s.peer.ensure(cw_bytes, "hipMalloc dpf peer");
 }
 
 void dpf_expand(hipStream_t st, const DpfRun& r, DpfScratch& s) {

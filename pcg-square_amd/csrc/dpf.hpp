@@ -66,6 +66,10 @@ struct DpfScratch {
 // Number of partial-sum slots per tree written by kLeafConvert.
 int dpf_leaf_nparts(int n, int w, int iters_max);
 
+// Allocates everything dpf_expand needs for `ntrees` trees of domain 2^n and
+// arity 2^w, so that a later dpf_expand of that shape allocates nothing.
+void dpf_reserve(DpfScratch& scratch, int n, int w, size_t ntrees, bool host_exchange);
+
 // Runs the whole level loop on `stream`.  Synchronizes with the host only in
 // host-exchange mode (once per level, like the reference).
 void dpf_expand(hipStream_t stream, const DpfRun& run, DpfScratch& scratch);

@@ -312,6 +312,10 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   pl->cw.ensure((size_t)d.ninst * sizeof(uint64_t), "ole cw");
   pl->fw.ensure(((size_t)2 * d.nblk + 2 * d.c) * d.N * sizeof(uint64_t), "ole ntt buffers");
   pl->out.ensure((size_t)4 * d.N * sizeof(uint64_t), "ole outputs");
+  // the DPF frontier and the NTT scratch too, so that a run allocates nothing
+  // (an allocation inside a run would stall other streams' work)
+  dpf_reserve(pl->dpf, d.D_log, d.w, gtrees, false);
+  (void)pl->ctx->ntt.scratch(((size_t)2 * d.nblk + 2 * d.c) * d.N, 0);
   if (d.prg == PCGSQ_PRG_AES) {
     uint32_t t0[256];
     aes_ttable_host(t0);

@@ -1,20 +1,28 @@
 // Serving PCG-OLE expansions with continuous batching on one GPU.
 //
 // Clients submit expansion requests (N, c, t, w, PRG, seed, pub_seed) at any
-// time.  A worker thread keeps a pool of expansion plans, each bound to a
-// context of its own (a stream, NTT tables and scratch), and runs at most
-// `slots` requests at once.  The oldest waiting request is admitted as soon
-// as fewer than `slots` run and either an idle plan of its shape exists or
-// the device-memory budget holds a new one (idle plans of other shapes are
-// evicted, least recently used first): its seed is dealt on the device, the
+// time.  A worker thread keeps a pool of expansion plans, each with a context
+// of its own (NTT tables and scratch, all allocated when the plan is built),
+// and runs at most `slots` requests at once, one per slot stream: a running
+// plan's context borrows the stream of its slot.  The server owns exactly
+// `slots` streams, so that with slots <= the device's hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default) every running request has a hardware
+// queue of its own and never waits in order behind another one's kernels.  Waiting
+// requests are admitted oldest first, except that a request whose shape has
+// no idle plan is passed over (requests of one shape keep their order): a
+// slot going free is taken by the oldest request that can start now.  A
+// shape with no plan at all gets one built (idle plans of other shapes are
+// evicted, least recently used first, if the device-memory budget needs it);
+// more plans of a shape, for requests of one shape to overlap, come from
+// pcgsq_server_reserve.  On admission the seed is dealt on the device, the
 // expansion is enqueued on the plan's stream and the outputs are copied into
-// the request's own device buffer, followed by an event.  Requests on
-// different plans run concurrently on the device (the streams overlap); a
-// request arriving while others run is admitted the moment a slot and memory
-// free up, without waiting for the others to finish.  Completion is the
-// event, polled by the worker; outputs stay on the device until the client
-// copies or releases them.  Results do not depend on the plan or on what else
-// runs: every request's work is a pure function of its parameters and seeds.
+// a device buffer of the request's own, followed by an event that the worker
+// polls.  Output buffers are pooled by size, so that in the steady state
+// nothing is allocated or freed: a device allocation or free can wait for
+// every stream's work and would stall the other requests.  Requests on
+// different plans overlap on the device (the streams run concurrently).
+// Results do not depend on the plan or on what else runs: every request's
+// work is a pure function of its parameters and seeds.
 //
 // No counterpart in the reference (its benchmarks run one expansion per
 // process: CPU_baseline/test/bench_pcg_ole_2pc.cpp).
@@ -22,7 +30,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
-#include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -51,10 +59,14 @@ struct Request {
   double t_submit = 0, t_start = 0, t_done = 0;
   int state = PCGSQ_REQ_QUEUED;
   std::string error;
-  DevBuf out;            // [x0, z0, x1, z1][N]
+  void* out = nullptr;   // pooled device buffer [x0, z0, x1, z1][N]
   size_t out_bytes = 0;
-  int slot = -1;
+  int readers = 0;       // outputs() copies in progress
 };
+
+bool same_shape(const Request& a, const Request& b) {
+  return same_params(a.p, b.p) && a.pub_seed == b.pub_seed;
+}
 
 struct PlanEntry {
   pcgsq_ctx* ctx = nullptr;
@@ -63,6 +75,7 @@ struct PlanEntry {
   uint64_t pub_seed = 0;
   size_t bytes = 0;
   Request* req = nullptr;
+  int slot = -1;          // slot stream while running
   hipEvent_t ev = nullptr;
   double last_used = 0;
 };
@@ -74,6 +87,30 @@ void destroy_entry(PlanEntry& e) {
   e = PlanEntry{};
 }
 
+// A plan of shape p on a context of its own that borrows stream `st` (no
+// server state touched).
+std::unique_ptr<PlanEntry> build_entry(int device, const pcgsq_params& p, uint64_t pub_seed,
+                                       size_t plan_budget, hipStream_t st) {
+  std::unique_ptr<PlanEntry> e(new PlanEntry);
+  try {
+    int rc = pcgsq_ctx_create(device, &e->ctx);
+    if (rc != PCGSQ_OK) throw Error(rc, pcgsq_last_error());
+    hip_check(hipStreamDestroy(e->ctx->stream), "server stream");
+    e->ctx->stream = st;
+    e->ctx->owns_stream = false;
+    rc = pcgsq_ole_plan_create(e->ctx, &p, pub_seed, plan_budget, &e->plan);
+    if (rc != PCGSQ_OK) throw Error(rc, pcgsq_last_error());
+    hip_check(hipSetDevice(device), "server hipSetDevice");
+    hip_check(hipEventCreateWithFlags(&e->ev, hipEventDisableTiming), "server event");
+  } catch (...) {
+    destroy_entry(*e);
+    throw;
+  }
+  e->pp = p;
+  e->pub_seed = pub_seed;
+  return e;
+}
+
 }  // namespace
 }  // namespace pcgsq
 
@@ -83,29 +120,53 @@ struct pcgsq_server {
   int device = 0, max_running = 1;
   size_t budget = 0, used = 0, peak = 0, plan_cap = 0;
   std::vector<std::unique_ptr<PlanEntry>> pool;
-  std::deque<Request*> queue;
+  std::vector<hipStream_t> streams;        // one per slot
+  std::vector<char> slot_busy;
+  std::multimap<size_t, void*> free_out;   // idle output buffers by size (counted in used)
+  std::list<Request*> queue;               // list: iterators survive submits while unlocked
   std::map<uint64_t, std::unique_ptr<Request>> reqs;
   std::mutex mu;
   std::condition_variable cv;
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1;
-  long completed = 0;
+  uint64_t window = 0;     // most ring dimension (sum of N) running at once; 0 = no limit
+  long completed = 0, plans_built = 0, plans_evicted = 0;
 
   size_t plan_budget() const { return plan_cap; }
 
   int running() const {
     int n = 0;
-    for (const auto& e : pool) n += e->req != nullptr;
+    for (char b : slot_busy) n += b != 0;
     return n;
+  }
+
+  int free_slot() const {
+    for (int k = 0; k < (int)slot_busy.size(); ++k)
+      if (!slot_busy[(size_t)k]) return k;
+    return -1;
   }
 
   static bool fits(const PlanEntry& e, const Request& r) {
     return same_params(e.pp, r.p) && e.pub_seed == r.pub_seed;
   }
 
-  // Evict idle plans, least recently used first, until `need` more bytes fit.
+  void charge(size_t b) {
+    used += b;
+    peak = std::max(peak, used);
+  }
+
+  // Free pooled output buffers, then idle plans (least recently used first),
+  // until `need` more bytes fit the budget (lock held).  Freeing waits for the
+  // device, so it happens only under memory pressure.
   bool make_room(size_t need) {
+    while (used + need > budget && !free_out.empty()) {
+      auto it = std::prev(free_out.end());
+      (void)hipSetDevice(device);
+      (void)hipFree(it->second);
+      used -= it->first;
+      free_out.erase(it);
+    }
     while (used + need > budget) {
       int victim = -1;
       for (int i = 0; i < (int)pool.size(); ++i)
@@ -116,82 +177,181 @@ struct pcgsq_server {
       used -= pool[(size_t)victim]->bytes;
       destroy_entry(*pool[(size_t)victim]);
       pool.erase(pool.begin() + victim);
+      ++plans_evicted;
     }
     return true;
   }
 
-  // A new plan of r's shape (lock held); nullptr if memory does not allow.
-  PlanEntry* new_entry(const pcgsq_params& p, uint64_t pub_seed, size_t extra) {
+  // An output buffer of `bytes`: a pooled one, else a new one if the budget
+  // holds it (lock held); nullptr if memory does not allow.
+  void* take_out(size_t bytes) {
+    auto it = free_out.find(bytes);
+    if (it != free_out.end()) {
+      void* p = it->second;
+      free_out.erase(it);
+      return p;
+    }
+    if (!make_room(bytes)) return nullptr;
+    void* p = nullptr;
+    hip_check(hipSetDevice(device), "server hipSetDevice");
+    hip_check(hipMalloc(&p, bytes), "hipMalloc served outputs");
+    charge(bytes);
+    return p;
+  }
+
+  void put_out(Request* r) {
+    if (r->out) free_out.emplace(r->out_bytes, r->out);
+    r->out = nullptr;
+  }
+
+  // A new plan of this shape, built with the lock held (reserve); nullptr if
+  // memory does not allow.
+  PlanEntry* new_entry(const pcgsq_params& p, uint64_t pub_seed) {
     const size_t est = pcgsq_ole_plan_estimate(&p, plan_budget());
     if (est == 0) throw Error(kErrInvalid, "invalid expansion parameters");
-    if (!make_room(est + extra)) return nullptr;
-    std::unique_ptr<PlanEntry> e(new PlanEntry);
-    try {
-      int rc = pcgsq_ctx_create(device, &e->ctx);
-      if (rc != PCGSQ_OK) throw Error(rc, pcgsq_last_error());
-      rc = pcgsq_ole_plan_create(e->ctx, &p, pub_seed, plan_budget(), &e->plan);
-      if (rc != PCGSQ_OK) throw Error(rc, pcgsq_last_error());
-      hip_check(hipSetDevice(device), "server hipSetDevice");
-      hip_check(hipEventCreateWithFlags(&e->ev, hipEventDisableTiming), "server event");
-    } catch (...) {
-      destroy_entry(*e);
-      throw;
-    }
-    e->pp = p;
-    e->pub_seed = pub_seed;
+    if (!make_room(est)) return nullptr;
+    const int k = free_slot();
+    std::unique_ptr<PlanEntry> e =
+        build_entry(device, p, pub_seed, plan_budget(), streams[(size_t)(k < 0 ? 0 : k)]);
     e->bytes = est;
-    used += est;
-    peak = std::max(peak, used);
+    charge(est);
     pool.push_back(std::move(e));
+    ++plans_built;
     return pool.back().get();
   }
 
-  // Admit the oldest request if a slot and the memory budget allow (lock held).
-  bool admit_one() {
-    if (queue.empty() || running() >= max_running) return false;
-    Request* r = queue.front();
-    PlanEntry* e = nullptr;
-    for (auto& x : pool)
-      if (!x->req && fits(*x, *r)) {
-        e = x.get();
-        break;
-      }
-    try {
-      if (!e) {
-        e = new_entry(r->p, r->pub_seed, r->out_bytes);
-        if (!e) return false;  // wait for memory
-      } else if (!make_room(r->out_bytes)) {
-        return false;
-      }
-    } catch (const std::exception& ex) {
-      queue.pop_front();
-      r->state = PCGSQ_REQ_FAILED;
-      r->error = ex.what();
-      return true;
-    }
-    queue.pop_front();
+  void fail(Request* r, const std::string& why) {
+    r->state = PCGSQ_REQ_FAILED;
+    r->error = why;
+    put_out(r);
+  }
+
+  // Start r on the idle plan e in free slot k (lock held).
+  void start(Request* r, PlanEntry* e, int k) {
     r->t_start = now_s();
     r->state = PCGSQ_REQ_RUNNING;
-    used += r->out_bytes;
-    peak = std::max(peak, used);
     e->last_used = r->t_start;
+    e->req = r;
+    e->slot = k;
+    slot_busy[(size_t)k] = 1;
+    e->ctx->stream = streams[(size_t)k];
     try {
       CtxGuard g(e->ctx);
-      r->out.ensure(r->out_bytes, "hipMalloc served outputs");
       if (pcgsq_ole_deal(e->plan, r->seed) != PCGSQ_OK || pcgsq_ole_run(e->plan) != PCGSQ_OK)
         throw Error(kErrDevice, pcgsq_last_error());
-      hip_check(hipMemcpyAsync(r->out.ptr, pcgsq_ole_outputs_dev(e->plan), r->out_bytes,
+      hip_check(hipMemcpyAsync(r->out, pcgsq_ole_outputs_dev(e->plan), r->out_bytes,
                                hipMemcpyDeviceToDevice, e->ctx->stream),
                 "served outputs");
       hip_check(hipEventRecord(e->ev, e->ctx->stream), "served event");
-      e->req = r;
     } catch (const std::exception& ex) {
-      r->state = PCGSQ_REQ_FAILED;
-      r->error = ex.what();
-      used -= r->out_bytes;
-      r->out.release();
+      e->req = nullptr;
+      e->slot = -1;
+      slot_busy[(size_t)k] = 0;
+      fail(r, ex.what());
+      ++completed;
     }
-    return true;
+  }
+
+  // One admission pass over the queue (lock held on entry and exit; released
+  // while a plan is built).  Returns whether anything changed.
+  bool admit(std::unique_lock<std::mutex>& lk) {
+    bool changed = false;
+    std::vector<const Request*> passed;  // shapes passed over: later ones of a shape wait too
+    uint64_t run_n = 0;                  // ring dimension of the running requests
+    for (const auto& x : pool)
+      if (x->req) run_n += (uint64_t)x->req->p.N;
+    uint64_t held_n = ~0ULL;  // smallest N held back by the window: no larger one passes it
+    auto it = queue.begin();
+    while (it != queue.end() && running() < max_running && !stop) {
+      Request* r = *it;
+      const uint64_t n = (uint64_t)r->p.N;
+      bool wait = n >= held_n;
+      for (const Request* q : passed) wait = wait || same_shape(*q, *r);
+      if (!wait && window && run_n > 0 && run_n + n > window) {
+        held_n = n;
+        wait = true;
+      }
+      if (wait) {
+        ++it;
+        continue;
+      }
+      PlanEntry* e = nullptr;
+      bool have_shape = false;
+      for (auto& x : pool)
+        if (fits(*x, *r)) {
+          have_shape = true;
+          if (!x->req) {
+            e = x.get();
+            break;
+          }
+        }
+      const int k = free_slot();
+      if (!e && !have_shape) {
+        // no plan of this shape: build one, outside the lock, on the slot
+        // stream it will run on (idle, so its synchronization waits for nothing)
+        const size_t est = pcgsq_ole_plan_estimate(&r->p, plan_budget());
+        if (!make_room(est)) {
+          passed.push_back(r);
+          ++it;
+          continue;
+        }
+        charge(est);
+        std::unique_ptr<PlanEntry> ne;
+        std::string err;
+        const pcgsq_params pp = r->p;
+        const uint64_t ps = r->pub_seed;
+        slot_busy[(size_t)k] = 1;   // held through the build
+        lk.unlock();
+        try {
+          ne = build_entry(device, pp, ps, plan_budget(), streams[(size_t)k]);
+        } catch (const std::exception& ex) {
+          err = ex.what();
+        }
+        lk.lock();
+        slot_busy[(size_t)k] = 0;
+        changed = true;
+        if (!ne) {
+          used -= est;
+          it = queue.erase(it);
+          fail(r, err);
+          ++completed;
+          continue;
+        }
+        ne->bytes = est;
+        pool.push_back(std::move(ne));
+        ++plans_built;
+        e = pool.back().get();
+      }
+      if (!e) {  // its shape's plans are busy: wait for one
+        passed.push_back(r);
+        ++it;
+        continue;
+      }
+      e->req = r;  // keeps e from eviction while the outputs find room
+      void* ob = nullptr;
+      try {
+        ob = take_out(r->out_bytes);
+      } catch (const std::exception& ex) {
+        e->req = nullptr;
+        it = queue.erase(it);
+        fail(r, ex.what());
+        ++completed;
+        changed = true;
+        continue;
+      }
+      e->req = nullptr;
+      if (!ob) {
+        passed.push_back(r);
+        ++it;
+        continue;
+      }
+      r->out = ob;
+      it = queue.erase(it);
+      start(r, e, k);
+      if (e->req) run_n += n;
+      changed = true;
+    }
+    return changed;
   }
 
   void run() {
@@ -205,19 +365,17 @@ struct pcgsq_server {
         Request* r = e->req;
         r->t_done = now_s();
         e->last_used = r->t_done;
-        if (q == hipSuccess) {
+        if (q == hipSuccess)
           r->state = PCGSQ_REQ_DONE;
-        } else {
-          r->state = PCGSQ_REQ_FAILED;
-          r->error = hipGetErrorString(q);
-          used -= r->out_bytes;
-          r->out.release();
-        }
+        else
+          fail(r, hipGetErrorString(q));
         e->req = nullptr;
+        slot_busy[(size_t)e->slot] = 0;
+        e->slot = -1;
         ++completed;
         progressed = true;
       }
-      while (admit_one()) progressed = true;
+      progressed = admit(lk) || progressed;
       if (progressed) cv.notify_all();
       // poll completions every 50 us while work runs; a waiting request with
       // nothing running waits for memory (a release); otherwise sleep until a
@@ -248,6 +406,10 @@ int pcgsq_server_create(int device, size_t hbm_budget_bytes, int slots, pcgsq_se
     // each plan keeps its DPF frontier within a share of the budget
     s->plan_cap = std::max<size_t>((size_t)1 << 30, s->budget / (size_t)slots / 2);
     s->max_running = slots;
+    s->streams.assign((size_t)slots, nullptr);
+    s->slot_busy.assign((size_t)slots, 0);
+    for (auto& st : s->streams)
+      hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "server stream");
     s->worker = std::thread([p = s.get()] {
       (void)hipSetDevice(p->device);
       p->run();
@@ -265,12 +427,18 @@ int pcgsq_server_destroy(pcgsq_server* s) {
     }
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    for (auto& e : s->pool) {
-      if (e->ctx) (void)pcgsq_ctx_synchronize(e->ctx);
-      destroy_entry(*e);
-    }
+    (void)hipSetDevice(s->device);
+    for (hipStream_t st : s->streams)
+      if (st) (void)hipStreamSynchronize(st);
+    for (auto& e : s->pool) destroy_entry(*e);
     s->pool.clear();
+    for (hipStream_t st : s->streams)
+      if (st) (void)hipStreamDestroy(st);
+    for (auto& kv : s->reqs)
+      if (kv.second->out) (void)hipFree(kv.second->out);
+    for (auto& kv : s->free_out) (void)hipFree(kv.second);
     s->reqs.clear();
+    s->free_out.clear();
     delete s;
   });
 }
@@ -330,18 +498,31 @@ int pcgsq_server_outputs(pcgsq_server* s, uint64_t id, uint64_t* x0, uint64_t* z
                          uint64_t* z1) {
   return guarded([&] {
     require(s, "server_outputs: null server");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    require(it != s->reqs.end(), "server_outputs: unknown request");
-    Request* r = it->second.get();
-    require(r->state == PCGSQ_REQ_DONE, "server_outputs: request not done");
+    Request* r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->reqs.find(id);
+      require(it != s->reqs.end(), "server_outputs: unknown request");
+      r = it->second.get();
+      require(r->state == PCGSQ_REQ_DONE, "server_outputs: request not done");
+      ++r->readers;  // release() refuses while the copy runs
+    }
+    // copy without the server lock: the worker keeps admitting meanwhile
+    struct Done {
+      pcgsq_server* s;
+      Request* r;
+      ~Done() {
+        std::lock_guard<std::mutex> lk(s->mu);
+        --r->readers;
+      }
+    } done{s, r};
     hip_check(hipSetDevice(s->device), "server hipSetDevice");
     const size_t N = (size_t)r->p.N;
+    const uint64_t* src = static_cast<const uint64_t*>(r->out);
     uint64_t* dst[4] = {x0, z0, x1, z1};
     for (int i = 0; i < 4; ++i)
       if (dst[i])
-        hip_check(hipMemcpy(dst[i], r->out.as<uint64_t>() + i * N, N * 8, hipMemcpyDeviceToHost),
-                  "server outputs");
+        hip_check(hipMemcpy(dst[i], src + i * N, N * 8, hipMemcpyDeviceToHost), "server outputs");
   });
 }
 
@@ -352,7 +533,7 @@ int pcgsq_server_outputs_dev(pcgsq_server* s, uint64_t id, const uint64_t** out)
     auto it = s->reqs.find(id);
     require(it != s->reqs.end(), "server_outputs_dev: unknown request");
     require(it->second->state == PCGSQ_REQ_DONE, "server_outputs_dev: request not done");
-    *out = it->second->out.as<uint64_t>();
+    *out = static_cast<const uint64_t*>(it->second->out);
   });
 }
 
@@ -365,7 +546,8 @@ int pcgsq_server_release(pcgsq_server* s, uint64_t id) {
     Request* r = it->second.get();
     require(r->state == PCGSQ_REQ_DONE || r->state == PCGSQ_REQ_FAILED,
             "server_release: request still queued or running");
-    if (r->state == PCGSQ_REQ_DONE) s->used -= r->out_bytes;
+    require(r->readers == 0, "server_release: outputs are being copied");
+    s->put_out(r);  // the buffer goes back to the pool, still charged
     s->reqs.erase(it);
     s->cv.notify_all();
   });
@@ -406,7 +588,7 @@ int pcgsq_server_reserve(pcgsq_server* s, const pcgsq_params* p, uint64_t pub_se
     for (auto& e : s->pool)
       have += same_params(e->pp, *p) && e->pub_seed == pub_seed;
     for (; have < count; ++have)
-      require(s->new_entry(*p, pub_seed, 0) != nullptr,
+      require(s->new_entry(*p, pub_seed) != nullptr,
               "server_reserve: the plans do not fit the memory budget", kErrNoMemory);
   });
 }
@@ -417,6 +599,27 @@ int pcgsq_server_charge(pcgsq_server* s, const pcgsq_params* p, size_t* bytes) {
     const size_t est = pcgsq_ole_plan_estimate(p, s->plan_budget());
     require(est != 0, "server_charge: bad parameters");
     *bytes = est + (size_t)4 * (size_t)p->N * sizeof(uint64_t);
+  });
+}
+
+int pcgsq_server_set_window(pcgsq_server* s, uint64_t max_running_N) {
+  return guarded([&] {
+    require(s, "server_set_window: null server");
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->window = max_running_N;
+    }
+    s->cv.notify_all();
+  });
+}
+
+int pcgsq_server_plans(pcgsq_server* s, int* plans, long* built, long* evicted) {
+  return guarded([&] {
+    require(s, "server_plans: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (plans) *plans = (int)s->pool.size();
+    if (built) *built = s->plans_built;
+    if (evicted) *evicted = s->plans_evicted;
   });
 }
 

@@ -152,6 +152,9 @@ def lib() -> C.CDLL:
                                      C.POINTER(C.c_int)]
     L.pcgsq_server_clock.restype = C.c_double
     L.pcgsq_server_charge.argtypes = [vp, C.POINTER(Params), C.POINTER(C.c_size_t)]
+    L.pcgsq_server_plans.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long),
+                                     C.POINTER(C.c_long)]
+    L.pcgsq_server_set_window.argtypes = [vp, C.c_uint64]
     L.pcgsq_server_reserve.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_int]
     L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
     L.pcgsq_dealer_u64.restype = C.c_uint64
@@ -694,10 +697,18 @@ class Server:
     (pcgsq_server_*): `slots` concurrent execution slots, admission by
     device-memory budget, outputs kept on the device until read/released."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4):
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4, window: int = 0):
         h = vp()
         _check(lib().pcgsq_server_create(device, hbm_budget, slots, C.byref(h)))
         self.h = h
+        if window:
+            self.set_window(window)
+
+    def set_window(self, max_running_N: int) -> None:
+        """Most ring dimension (sum of N) running at once; 0 = no limit.  A
+        request that would exceed it waits (unless nothing runs), and so does
+        every later request at least as large."""
+        _check(lib().pcgsq_server_set_window(self.h, max_running_N))
 
     def close(self):
         if self.h:
@@ -756,9 +767,16 @@ class Server:
         return {"budget": b.value, "used": u.value, "peak": pk.value, "completed": done.value,
                 "queued": q.value}
 
+    def plans(self) -> dict:
+        """Plans in the pool, plans built and plans evicted so far."""
+        n, built, ev = C.c_int(), C.c_long(), C.c_long()
+        _check(lib().pcgsq_server_plans(self.h, C.byref(n), C.byref(built), C.byref(ev)))
+        return {"plans": n.value, "built": built.value, "evicted": ev.value}
+
     def reserve(self, N: int, c: int, t: int, count: int = 1, pub_seed: int = 0, w: int = 1,
                 prg: int = PRG_CHACHA8) -> None:
-        """Create plans of this shape ahead of requests."""
+        """Create plans of this shape ahead of requests (until `count` exist);
+        requests of one shape overlap only on plans reserved this way."""
         _check(lib().pcgsq_server_reserve(self.h, C.byref(Params(N, c, t, w, prg)), pub_seed,
                                           count))
 

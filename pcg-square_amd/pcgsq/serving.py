@@ -88,3 +88,12 @@ def summarize(records, wall: float) -> dict:
             "max_ms": max(lat) if lat else float("nan"),
             "by_N": {f"2^{int(np.log2(n))}": sum(1 for r in records if r["N"] == n)
                      for n in sorted({r["N"] for r in records})}}
+
+
+def by_size(records) -> dict:
+    """p50 and p99 latency (ms) per ring dimension."""
+    out = {}
+    for n in sorted({r["N"] for r in records}):
+        lat = [r["latency_ms"] for r in records if r["N"] == n]
+        out[f"2^{int(n).bit_length() - 1}"] = [percentile(lat, 50), percentile(lat, 99)]
+    return out

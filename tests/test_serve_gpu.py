@@ -127,3 +127,78 @@ def test_served_poisson_run(pq):
                                threads=16)
         for g, e in zip(outs, exp):
             assert np.array_equal(g, e)
+
+
+def test_served_busy_shape_does_not_block_other_shapes(pq):
+    """With one 2^22 plan, a second 2^22 request waits for it, and a 2^16
+    request queued behind that second one starts before it (skip-ahead
+    admission), while the two 2^22 requests keep their order."""
+    with pq.Server(0, 0, 3) as srv:
+        srv.reserve(1 << 22, 4, 16, 1)
+        srv.reserve(1 << 16, 4, 16, 1)
+        big1 = srv.submit(1 << 22, 4, 16, 301)
+        big2 = srv.submit(1 << 22, 4, 16, 302)
+        small = srv.submit(1 << 16, 4, 16, 303)
+        for rid in (big1, big2, small):
+            srv.wait(rid, 120.0)
+        _, s1, d1 = srv.timing(big1)
+        _, s2, _ = srv.timing(big2)
+        _, ss, ds = srv.timing(small)
+        assert s2 >= d1 - 1e-3, "the second 2^22 request started before the first ended"
+        assert ss < s2 and ds <= d1 + 0.05, "the 2^16 request waited behind the busy shape"
+        assert srv.plans()["built"] == 2
+        for rid in (big1, big2, small):
+            srv.release(rid)
+
+
+def test_served_steady_state_allocates_nothing(pq):
+    """After one request per shape, a second round of the same mix builds and
+    evicts no plan and charges no new device memory (outputs are pooled), and
+    its results equal the first round's for the same seeds."""
+    mix = [(1 << 16, 401), (1 << 18, 402), (1 << 20, 403), (1 << 16, 404)]
+    with pq.Server(0, 0, 4) as srv:
+        digests = []
+        for rnd in range(2):
+            ids = [srv.submit(N, 4, 16, seed) for N, seed in mix]
+            for rid in ids:
+                srv.wait(rid, 120.0)
+            digests.append([_digest(srv.outputs(rid, N)) for rid, (N, _) in zip(ids, mix)])
+            for rid in ids:
+                srv.release(rid)
+            if rnd == 0:
+                used0, plans0 = srv.stats()["used"], srv.plans()
+        assert srv.stats()["used"] == used0
+        assert srv.plans() == plans0 and plans0["evicted"] == 0
+        assert digests[0] == digests[1]
+
+
+def test_served_window_keeps_large_requests_in_order(pq):
+    """Window 2^22 + 2^19: two 2^22 requests and a 2^20 one start in arrival
+    order and never overlap one another, while 2^16 requests submitted after
+    them run alongside; every result equals the same request run alone."""
+    win = (1 << 22) + (1 << 19)
+    with pq.Server(0, 0, 4, window=win) as srv:
+        for N in (1 << 16, 1 << 20, 1 << 22):
+            srv.reserve(N, 4, 16, 2)
+        heavy = [srv.submit(1 << 22, 4, 16, 501), srv.submit(1 << 20, 4, 16, 502),
+                 srv.submit(1 << 22, 4, 16, 503)]
+        small = [srv.submit(1 << 16, 4, 16, 510 + k) for k in range(3)]
+        for rid in heavy + small:
+            srv.wait(rid, 120.0)
+        th = [srv.timing(rid) for rid in heavy]
+        for (_, s0, d0), (_, s1, _) in zip(th, th[1:]):
+            assert s1 >= d0 - 1e-3, "two large requests overlapped despite the window"
+        ts = [srv.timing(rid) for rid in small]
+        assert all(d <= th[0][2] + 1e-3 for _, _, d in ts), \
+            "small requests waited behind the large ones"
+        outs = {rid: _digest(srv.outputs(rid, N)) for rid, N in
+                zip(heavy + small, [1 << 22, 1 << 20, 1 << 22] + [1 << 16] * 3)}
+        for rid in heavy + small:
+            srv.release(rid)
+    with pq.Server(0, 0, 1) as alone:
+        for rid, (N, seed) in zip(heavy + small, [(1 << 22, 501), (1 << 20, 502), (1 << 22, 503)]
+                                  + [(1 << 16, 510 + k) for k in range(3)]):
+            r2 = alone.submit(N, 4, 16, seed)
+            alone.wait(r2, 120.0)
+            assert _digest(alone.outputs(r2, N)) == outs[rid], f"N={N} seed={seed}"
+            alone.release(r2)
