@@ -289,6 +289,12 @@ def main() -> int:
             pl.run()
             ctx.synchronize()
             lats.append((time.perf_counter() - s) * 1e3)
+        # ---- host-buffer form: each expansion also copies x0, z0, x1, z1 to host ----
+        t0 = time.perf_counter()
+        for _ in range(3):
+            pl.run()
+            pl.outputs()
+        dt_pcie = (time.perf_counter() - t0) / 3
         # ---- live per-launch timing of every kernel (HIP events on the stream) ----
         ctx.profile(True)
         nprof = 2
@@ -352,6 +358,7 @@ def main() -> int:
                        "form": "one request per launch, host-visible completion",
                        "requests": len(lats)},
         "per_party_OLEs_per_s": N * args.steps / dt / 2,
+        "pcie_inclusive_OLEs_per_s": N / dt_pcie,
         "roofline": roofline,
         "roofline_ntt": roof_ntt,
         "phases_ms": {k: v[1] / nprof for k, v in sorted(zones.items())},
