@@ -172,13 +172,13 @@ size_t lds_max() {
 
 struct PassKernel {
   const void* fn;
-  int nlog, rmax;
+  int nlog, rmax, groups;
 };
 
 template <int K, int RA, int RB, int RC>
 PassKernel pk() {
   using P = nttpass::Part<RA, RB, RC>;
-  return PassKernel{(const void*)ntt_pass_kernel<K, RA, RB, RC>, P::NLOG, P::RMAX};
+  return PassKernel{(const void*)ntt_pass_kernel<K, RA, RB, RC>, P::NLOG, P::RMAX, P::G};
 }
 
 // Stage grouping per pass length (variant 1 = more, shorter groups).  Four-step
@@ -211,11 +211,11 @@ PassKernel pass_kernel(int nlog, int variant) {
 
 // Columns per tile: requested, else about 256 threads; bounded by 512
 // threads, two tiles' LDS per CU and the column count.
-int tile_cols(int want, int nlog, int rmax, int ncols) {
+int tile_cols(int want, int nlog, int rmax, int groups, int ncols) {
   int C = want > 0 ? want : std::max(1, 256 >> (nlog - rmax));
   if (want <= 0) C = std::min(C, 16);
   while (C > 1 && (C << (nlog - rmax)) > kPassThreads) C >>= 1;
-  while (C > 1 && ntt_tile_lds_bytes(nlog, C) > lds_max() / (want > 0 ? 1 : 2)) C >>= 1;
+  while (C > 1 && ntt_tile_lds_bytes(nlog, C, groups) > lds_max() / (want > 0 ? 1 : 2)) C >>= 1;
   while (C > 1 && C > ncols) C >>= 1;
   return C;
 }
@@ -225,11 +225,11 @@ void launch_pass(hipStream_t st, NttPassDesc d, int nlog, int variant, int want_
                  int batch, Profiler* prof, const char* zone) {
   ProfZone pz(prof, st, zone);
   const PassKernel k = pass_kernel<K>(nlog, variant);
-  const int C = tile_cols(want_cols, nlog, k.rmax, ncols);
+  const int C = tile_cols(want_cols, nlog, k.rmax, k.groups, ncols);
   d.c_log = ilog2_exact((uint64_t)C);
   d.ncols = ncols;
   const uint32_t tiles = (uint32_t)((ncols + C - 1) / C);
-  const size_t lds = ntt_tile_lds_bytes(nlog, C);
+  const size_t lds = ntt_tile_lds_bytes(nlog, C, k.groups);
   require(lds <= lds_max(), "ntt: tile exceeds LDS");
   if (lds > 65536) {
     static std::mutex mu;

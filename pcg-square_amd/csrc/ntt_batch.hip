@@ -66,10 +66,12 @@ struct BatchQueues {
   uint32_t n;     // lists in use (the device's XCD count)
 };
 
-// Register grouping per pass length, chosen so a tile is 256 threads wide.
+// Register grouping per pass length, chosen so a tile is 256 threads wide:
+// the largest group and the number of groups of the shapes below.
 __host__ __device__ constexpr int batch_rmax(int nlog) {
   return (nlog == 6 || nlog == 9) ? 3 : 4;
 }
+__host__ __device__ constexpr int batch_groups(int nlog) { return nlog >= 9 ? 3 : 2; }
 
 // One-pass transforms up to 2^12; four-step passes of 2^6 (rows only) to 2^11.
 #define PCGSQ_BATCH_SHAPES(X)                                                                 \
@@ -106,6 +108,14 @@ __device__ __noinline__ void tile_fn(const NttPassDesc* d, uint32_t tile) {
   for (int k = 0; k < (int)(sizeof(NttPassDesc) / 4); ++k) dst[k] = src[k];
   nttpass::pass_tile<K, A, B, C>(desc, batch_lds, 0, __builtin_amdgcn_readfirstlane(tile));
 }
+
+// the host sizes tiles by batch_rmax / batch_groups: every shape must agree
+#define PCGSQ_SHAPE_CHECK(K, N, A, B, C)                                                        \
+  static_assert(nttpass::Part<A, B, C>::RMAX == batch_rmax(N) &&                               \
+                    nttpass::Part<A, B, C>::G == batch_groups(N) && A + B + C == N,            \
+                "batch shape disagrees with batch_rmax / batch_groups");
+PCGSQ_BATCH_SHAPES(PCGSQ_SHAPE_CHECK)
+#undef PCGSQ_SHAPE_CHECK
 
 __device__ __forceinline__ void run_tile(int kind, int nlog, const NttPassDesc* d,
                                          uint32_t tile) {
@@ -301,7 +311,7 @@ struct pcgsq_ntt_batch {
           b.wait_need = ph[(size_t)last].tiles;
         }
         last = x * 4 + p;
-        lds = std::max(lds, ntt_tile_lds_bytes(b.nlog, C));
+        lds = std::max(lds, ntt_tile_lds_bytes(b.nlog, C, batch_groups(b.nlog)));
       }
       const double passes = pl->single() ? 1.0 : 2.0;
       alg_bytes += ((ops & 1 ? 1 : 0) + (ops & 2 ? 1 : 0)) * 16.0 * (double)(1 << L) * passes;

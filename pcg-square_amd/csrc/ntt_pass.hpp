@@ -45,7 +45,8 @@ struct NttPassDesc {
 // The kinds of pass, fixed at compile time so that the per-element load and
 // store code is straight-line:
 //   kFwdCols  four-step pass 1: natural rows in, bit-reversed positions out
-//             with the twiddle psi^(c (2p + 1)); lanes along the columns.
+//             with the twiddle psi^(c (2p + 1)), left in [0, 2P) (the first
+//             passes, STORE == 1, do not reduce fully); lanes along the columns.
 //   kFwdRows  four-step pass 2: contiguous sub-transforms in (lanes along the
 //             positions), transposed store.
 //   kInvRows / kInvRowsProd  inverse pass 1 (optionally multiplying two
@@ -85,11 +86,26 @@ __device__ __forceinline__ Pair ld_pair(gcptr tab, uint32_t i) {
   return Pair{t.x, t.y};
 }
 
-// Padding words per column so that column c starts on bank slot c * 32 / C.
+// Two LDS layouts of a tile, chosen by the number of register groups:
+//  * one or two groups: columns padded so that column c starts on bank slot
+//    c * 32 / C, and the low five position bits XORed with the next five;
+//  * three groups: one pad word after every 8 positions (word pos + pos / 8)
+//    and 16 / C pad words per column for C >= 4.  A set's positions are its
+//    base with the element index in disjoint bits, so every element's word is
+//    the set's word plus a compile-time constant (an instruction offset, no
+//    address arithmetic per element), and the conflicts of all three
+//    groups' access orders stay at two-way or less.
 __host__ __device__ constexpr int lds_pad(int C) { return C >= 2 ? (C >= 32 ? 1 : 32 / C) : 0; }
 
-// LDS word of (column c, position pos): columns are contiguous, the low five
-// position bits are XORed with the next five.
+__host__ __device__ constexpr uint32_t lds_col_stride(int nlog, int C, int groups) {
+  return groups == 3 ? (1u << nlog) + (1u << nlog >> 3) + (C >= 4 ? 16u / (uint32_t)C : 0u)
+                     : (1u << nlog) + (uint32_t)lds_pad(C);
+}
+
+// Three-group layout: word of position pos within its column.
+__host__ __device__ constexpr uint32_t lds_lin(uint32_t pos) { return pos + (pos >> 3); }
+
+// One/two-group layout: LDS word of (column c, position pos).
 template <int NLOG>
 __device__ __forceinline__ uint32_t lds_at(uint32_t c, uint32_t pos, uint32_t col_stride) {
   uint32_t x = pos;
@@ -164,7 +180,7 @@ enum : int { kFromHbm = 1, kToHbm = 2, kCopy = 4 };
 
 // Group `S0, R` of a pass over a tile: for each set owned by this thread,
 // gather (from HBM or LDS), run the stages, scatter (to LDS or HBM).
-template <int K, int NLOG, int S0, int R, int RMAX, int ROLE>
+template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G>
 __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
                                           uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
                                           gcptr __restrict__ inb, gcptr __restrict__ in2b,
@@ -174,7 +190,7 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
   constexpr int SLOG = NLOG - R;              // log2 of sets per column
   constexpr int PER = 1 << (RMAX - R);        // sets per thread
   using Tr = KindTraits<K>;
-  const uint32_t col_stride = (1u << NLOG) + lds_pad((int)C);
+  const uint32_t col_stride = lds_col_stride(NLOG, (int)C, G);
   // lanes run along the columns when the HBM side of this group has unit
   // column stride, else along the positions
   constexpr bool cfast = (ROLE & kFromHbm) ? Tr::LOAD_CFAST
@@ -206,6 +222,11 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
         v[w] = inb[off];
         if constexpr (Tr::PROD) v[w] = fmul(v[w], in2b[off]);
       }
+    } else if constexpr (G == 3) {
+      // base and w << TMIN have disjoint bits: lds_lin distributes over them
+      const uint32_t sw = c * col_stride + lds_lin(base);
+#pragma unroll
+      for (int w = 0; w < M; ++w) v[w] = lds[sw + lds_lin((uint32_t)w << TMIN)];
     } else {
 #pragma unroll
       for (int w = 0; w < M; ++w)
@@ -232,8 +253,15 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
         } else if constexpr (Tr::STORE == 2) {
           x = shoup_lazy(x, d.scale_w, d.scale_wq);
         }
-        outb[(p << d.out_ps_log) + (c << d.out_cs_log)] = red_full(x);
+        // a first pass hands its [0, 2P) values to the second as they are
+        // (forward stages take [0, 4P), inverse stages [0, 2P)); only a
+        // transform's final pass reduces to [0, P)
+        outb[(p << d.out_ps_log) + (c << d.out_cs_log)] = Tr::STORE == 1 ? x : red_full(x);
       }
+    } else if constexpr (G == 3) {
+      const uint32_t sw = c * col_stride + lds_lin(base);
+#pragma unroll
+      for (int w = 0; w < M; ++w) lds[sw + lds_lin((uint32_t)w << TMIN)] = v[w];
     } else {
 #pragma unroll
       for (int w = 0; w < M; ++w)
@@ -271,44 +299,45 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every
     // element before any bit-reversed store lands
-    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
     __syncthreads();
     if (active)
-      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy>(d, lds, t, T, C, c0, inb, in2b, outb);
+      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
   } else if constexpr (P::G == 2) {
     constexpr int SA = 0, SB = RA;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
     } else {
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
     }
   } else {
     constexpr int SA = 0, SB = RA, SC = RA + RB;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
     } else {
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
     }
   }
 }
 
 }  // namespace nttpass
 
-// LDS bytes of a tile of C columns of length 2^nlog.
-inline size_t ntt_tile_lds_bytes(int nlog, int C) {
-  return (size_t)C * ((size_t(1) << nlog) + (size_t)nttpass::lds_pad(C)) * sizeof(uint64_t);
+// LDS bytes of a tile of C columns of length 2^nlog whose pass has `groups`
+// register groups.
+inline size_t ntt_tile_lds_bytes(int nlog, int C, int groups) {
+  return (size_t)C * nttpass::lds_col_stride(nlog, C, groups) * sizeof(uint64_t);
 }
 
 }  // namespace pcgsq

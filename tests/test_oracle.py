@@ -289,3 +289,19 @@ def test_ref_headers_agree_with_oracle():
     ref.ref_chacha8_blocks((C.c_uint8 * 32).from_buffer_copy(key), O.OUT_DOMAIN, 5, out, 2)
     assert bytes(out)[:64] == O.chacha8_block(key, O.OUT_DOMAIN, 5)
     assert bytes(out)[64:] == O.chacha8_block(key, O.OUT_DOMAIN, 6)
+
+
+def test_served_hash_fixture_first_entry():
+    """tests/golden/served_hashes.json was written by the oracle: its first
+    (2^16) entry recomputes to the same four hashes, and every entry's OLE
+    parameters are the served workload's (c=4, t=16, ChaCha8)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "served_hashes.json")
+    reqs = json.load(open(path))["requests"]
+    assert {r["N"] for r in reqs} == {1 << 16, 1 << 18, 1 << 20, 1 << 22}
+    assert all(r["c"] == 4 and r["t"] == 16 and r["prg"] == "chacha8" for r in reqs)
+    r = reqs[0]
+    outs = O.pcg_ole_expand(r["N"], r["c"], r["t"], r["w"], O.PRG_CHACHA8, seed=r["seed"],
+                            pub_seed=r["pub_seed"], threads=8)
+    assert [f"{O.fnv1a64(v):016x}" for v in outs] == r["fnv1a64"]

@@ -202,3 +202,20 @@ def test_served_window_keeps_large_requests_in_order(pq):
             alone.wait(r2, 120.0)
             assert _digest(alone.outputs(r2, N)) == outs[rid], f"N={N} seed={seed}"
             alone.release(r2)
+
+
+def test_served_outputs_match_precomputed_cpu_hashes(pq):
+    """The served-path gate: every request of tests/golden/served_hashes.json
+    (2^16..2^22, hashes written by the CPU oracle) submitted together to a
+    4-slot server returns x0, z0, x1, z1 with exactly those FNV-1a-64 hashes."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "served_hashes.json")
+    reqs = json.load(open(path))["requests"]
+    with pq.Server(0, 0, 4, window=(1 << 22) + (1 << 19)) as srv:
+        ids = [srv.submit(r["N"], r["c"], r["t"], r["seed"], r["pub_seed"]) for r in reqs]
+        for rid, r in zip(ids, reqs):
+            srv.wait(rid, 120.0)
+            got = [f"{h:016x}" for h in _digest(srv.outputs(rid, r["N"]))]
+            assert got == r["fnv1a64"], f"N={r['N']} seed={r['seed']}: hashes differ from the CPU oracle"
+            srv.release(rid)
