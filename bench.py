@@ -50,6 +50,19 @@ def pmc_traffic(path: str, key: str = "hbm_bytes_per_launch"):
     return float(m.group(1)) if m else None
 
 
+def gmw_and_gates(N: int, t: int, w: int = 1) -> int:
+    """AND gates of one instance's GMW preprocessing: the Kogge-Stone adder
+    over nb = log2(N/t) bits (nb + sum_d 2(nb - d), bool_circuit/adder.h:3-5)
+    and the DltSft circuit, levels * (2^w - 1) * (w - 1 + 128)."""
+    nb = (N // t).bit_length() - 1
+    n = nb + 1
+    s, d = nb, 1
+    while d < nb:
+        s += 2 * (nb - d)
+        d <<= 1
+    return s + (n // w) * ((1 << w) - 1) * (w - 1 + 128)
+
+
 def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) -> dict:
     """The CPU oracle (a C restatement of CPU_baseline's local compute) on a
     bounded sample: one DPF block of the c^2 (both parties, DPF + output
@@ -289,6 +302,18 @@ def main() -> int:
             pl.run()
             ctx.synchronize()
             lats.append((time.perf_counter() - s) * 1e3)
+        # ---- the 2PC preprocessing under GMW on the device (both parties) ----
+        for gmw in (False, True):
+            pl.deal(2, gmw=gmw)
+        ctx.synchronize()
+        deal_ms = {}
+        for gmw in (False, True):
+            t0 = time.perf_counter()
+            for _ in range(5):
+                pl.deal(2, gmw=gmw)
+            ctx.synchronize()
+            deal_ms[gmw] = (time.perf_counter() - t0) * 1e3 / 5
+        pl.deal(1)
         # ---- host-buffer form: each expansion also copies x0, z0, x1, z1 to host ----
         t0 = time.perf_counter()
         for _ in range(3):
@@ -359,6 +384,12 @@ def main() -> int:
                        "requests": len(lats)},
         "per_party_OLEs_per_s": N * args.steps / dt / 2,
         "pcie_inclusive_OLEs_per_s": N / dt_pcie,
+        "preprocessing_gmw": {
+            "what": "position adder (Kogge-Stone) + DltSft circuit under GMW for both parties, "
+                    "dealer AND triples; replaces the dealer's DltSft (same outputs)",
+            "instances": c * c * t * t, "and_gates": c * c * t * t * gmw_and_gates(N, t),
+            "deal_ms": deal_ms[True], "dealer_deal_ms": deal_ms[False],
+            "and_gates_per_s": c * c * t * t * gmw_and_gates(N, t) / (deal_ms[True] * 1e-3)},
         "roofline": roofline,
         "roofline_ntt": roof_ntt,
         "phases_ms": {k: v[1] / nprof for k, v in sorted(zones.items())},

@@ -294,6 +294,19 @@ size_t pcgsq_ole_plan_estimate(const pcgsq_params* p, size_t hbm_budget_bytes);
  * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199):
  * the dealer runs on the device, asynchronously on the context stream. */
 int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed);
+/* The same deal with the 2PC preprocessing of pcg_ole_impl.h:114-199 run on
+ * the device for both parties: alpha = off0 + off1 by the Kogge-Stone adder
+ * circuit (bool_circuit/adder.h:21-63) and DltSft by the F_DeltaShiftShare
+ * circuit (half_tree_dpf/dpf.cpp:172-240), both under GMW
+ * (bool_circuit/circuit.cpp:245-380) with masks and AND triples drawn by the
+ * dealer in place of emp::PRG and FerretCOT.  beta and the F_p Beaver
+ * triples (Gilboa OT in the reference) stay with the dealer.  The DltSft
+ * shares differ from pcgsq_ole_deal's but reconstruct to the same values, so
+ * the expansion's outputs are identical. */
+int pcgsq_ole_deal_gmw(pcgsq_ole_plan* plan, uint64_t seed);
+/* Both parties' dealt DltSft shares, [2][c*c*t*t][levels][2^w - 1] blocks
+ * (synchronizes). */
+int pcgsq_ole_dealt_dltsft(pcgsq_ole_plan* plan, pcgsq_blk* dltsft);
 /* Run the whole expansion for both parties on the device-resident inputs
  * (DPF batches, output layer with Beaver opens, step 4), asynchronously on
  * the context stream. */

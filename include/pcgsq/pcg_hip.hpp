@@ -275,11 +275,14 @@ class OleExpander {
     return t;
   }
 
-  Output expand(uint64_t seed) {
+  // gmw: run the 2PC preprocessing circuits (position adder, DltSft) under
+  // GMW on the device instead of taking DltSft from the dealer (pcg_ole_impl.h:
+  // 114-199); the outputs are the same.
+  Output expand(uint64_t seed, bool gmw = false) {
     Output o;
     o.x0.resize(N_); o.z0.resize(N_); o.x1.resize(N_); o.z1.resize(N_);
     std::lock_guard<std::mutex> lk(detail::mu());
-    detail::check(pcgsq_ole_deal(plan_, seed));
+    detail::check(gmw ? pcgsq_ole_deal_gmw(plan_, seed) : pcgsq_ole_deal(plan_, seed));
     detail::check(pcgsq_ole_run(plan_));
     detail::check(pcgsq_ole_outputs(plan_, o.x0.data(), o.z0.data(), o.x1.data(), o.z1.data()));
     return o;

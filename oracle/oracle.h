@@ -194,6 +194,20 @@ size_t orc_dealer_sizes(const orc_params* p, size_t* n_pos, size_t* n_inst,
                         size_t* n_dlt);
 int orc_dealer(const orc_params* p, uint64_t seed, orc_party_inputs* in0,
                orc_party_inputs* in1, uint64_t* alpha /* [c*c][B] or NULL */);
+/* The dealer's inputs with DltSft produced instead by the GMW preprocessing
+ * of pcg_ole_impl.h:114-199 (Kogge-Stone position adder, then the
+ * F_DeltaShiftShare circuit of dpf.cpp:172-240), evaluated gate by gate for
+ * both parties with dealer masks and AND triples (orc_gmw.c gives the spec).
+ * The DltSft shares differ from orc_dealer's; their XOR, and everything
+ * else, is the same.  alpha_sh0/1[c*c][B] (or NULL): the adder's XOR shares
+ * of alpha. */
+int orc_gmw_preprocess(const orc_params* p, uint64_t seed, orc_party_inputs* in0,
+                       orc_party_inputs* in1, uint64_t* alpha_sh0, uint64_t* alpha_sh1);
+/* AND ops of the nb-bit Kogge-Stone adder (1 + 2*ceil(log2 nb)). */
+int orc_gmw_adder_ops(int nb);
+/* AND gates of one instance's two circuits (adder of nb bits, DltSft of
+ * depth n at arity 2^w). */
+int orc_gmw_and_gates(int nb, int w, int n);
 /* public_a[c-1][N] = dealer_u64(pub_seed, PUB, i, j) % P */
 void orc_public_a(uint64_t pub_seed, int c, int N, uint64_t* public_a);
 /* Fixed public W (stands in for emp::PRG(&F_RAND_SEED), dpf.cpp:25-26,291). */

@@ -29,6 +29,11 @@ class Blk(C.Structure):
     _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
 
 
+class PartyInputs(C.Structure):
+    """orc_party_inputs: one party's dealt inputs (pointers)."""
+    _fields_ = [(n, C.c_void_p) for n in ("pos", "pay", "root", "dltsft", "beta", "r", "s", "rs")]
+
+
 class Params(C.Structure):
     _fields_ = [("N", C.c_int), ("c", C.c_int), ("t", C.c_int), ("w", C.c_int),
                 ("prg", C.c_int)]
@@ -115,6 +120,17 @@ def lib() -> C.CDLL:
         L.orc_mt64_fill.argtypes = [C.c_void_p, u64p, C.c_size_t, C.c_uint64]
         L.orc_dpf_real_bench_checksum.restype = C.c_uint64
         L.orc_dpf_real_bench_checksum.argtypes = [C.c_int] * 5
+        L.orc_dealer_sizes.restype = C.c_size_t
+        L.orc_dealer_sizes.argtypes = [C.POINTER(Params), C.POINTER(C.c_size_t),
+                                       C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.orc_dealer.argtypes = [C.POINTER(Params), C.c_uint64, C.POINTER(PartyInputs),
+                                 C.POINTER(PartyInputs), u64p]
+        L.orc_dealer.restype = C.c_int
+        L.orc_gmw_preprocess.argtypes = [C.POINTER(Params), C.c_uint64, C.POINTER(PartyInputs),
+                                         C.POINTER(PartyInputs), u64p, u64p]
+        L.orc_gmw_preprocess.restype = C.c_int
+        L.orc_gmw_adder_ops.argtypes = [C.c_int]
+        L.orc_gmw_and_gates.argtypes = [C.c_int, C.c_int, C.c_int]
     return _lib
 
 
@@ -343,6 +359,44 @@ def public_a(pub_seed: int, c: int, N: int) -> np.ndarray:
     if c > 1:
         lib().orc_public_a(pub_seed, c, N, _p(out))
     return out[: c - 1]
+
+
+def dealt_inputs(N, c, t, w=1, seed=1, gmw=False):
+    """Both parties' dealt inputs: dict of pos, pay [2][c*t], root [2][ninst][2],
+    dltsft [2][ninst][levels][nk][2], beta, r, s, rs [2][ninst], and alpha
+    ([ninst], dealer) or alpha_sh ([2][ninst], the GMW adder's XOR shares).
+    gmw=True: DltSft from the GMW preprocessing circuits instead of the dealer."""
+    p = Params(N, c, t, w, PRG_CHACHA8)
+    npos, ninst, ndlt = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    lib().orc_dealer_sizes(C.byref(p), C.byref(npos), C.byref(ninst), C.byref(ndlt))
+    npos, ninst, ndlt = npos.value, ninst.value, ndlt.value
+    out = {"pos": np.empty((2, npos), np.uint64), "pay": np.empty((2, npos), np.uint64),
+           "root": np.empty((2, ninst, 2), np.uint64), "dltsft": np.empty((2, ndlt, 2), np.uint64)}
+    for k in ("beta", "r", "s", "rs"):
+        out[k] = np.empty((2, ninst), np.uint64)
+    pi = [PartyInputs(*[out[k][s].ctypes.data for k in ("pos", "pay", "root", "dltsft", "beta",
+                                                         "r", "s", "rs")]) for s in range(2)]
+    if gmw:
+        out["alpha_sh"] = np.empty((2, ninst), np.uint64)
+        rc = lib().orc_gmw_preprocess(C.byref(p), seed, C.byref(pi[0]), C.byref(pi[1]),
+                                      _p(out["alpha_sh"][0]), _p(out["alpha_sh"][1]))
+    else:
+        out["alpha"] = np.empty(ninst, np.uint64)
+        rc = lib().orc_dealer(C.byref(p), seed, C.byref(pi[0]), C.byref(pi[1]), _p(out["alpha"]))
+    if rc:
+        raise ValueError("dealer: invalid parameters")
+    n = int(2 * N // t).bit_length() - 1
+    levels, nk = n // w, (1 << w) - 1
+    out["dltsft"] = out["dltsft"].reshape(2, ninst, levels, nk, 2)
+    return out
+
+
+def gmw_and_gates(nb: int, w: int, n: int) -> int:
+    return lib().orc_gmw_and_gates(nb, w, n)
+
+
+def gmw_adder_ops(nb: int) -> int:
+    return lib().orc_gmw_adder_ops(nb)
 
 
 def pcg_ole_expand(N, c, t, w=1, prg=PRG_CHACHA8, seed=1, pub_seed=0, threads=8):

@@ -29,6 +29,10 @@
 //   r_s = U(seed,6,g,s) % P, s_s = U(seed,7,g,s) % P,
 //   rs0 = U(seed,8,g,0) % P, rs1 = (r0+r1)(s0+s1) - rs0;
 //   public_a[i][j] = U(pub_seed,9,i,j) % P.
+// GMW preprocessing (pcgsq_ole_deal_gmw): the same draws, except that alpha
+// and DltSft come from the parties' Kogge-Stone adder and F_DeltaShiftShare
+// circuits under GMW with masks and AND triples from U(seed,14,g,j) (spec at
+// gmw_and below; oracle/orc_gmw.c restates it gate by gate).
 #include <string.h>
 
 #include <algorithm>
@@ -53,6 +57,101 @@ PCGSQ_HD uint64_t dealer_hd(uint64_t seed, uint32_t domain, uint64_t a, uint64_t
 }
 
 constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
+
+// ---- GMW preprocessing (pcg_ole_impl.h:114-199), both parties per thread ----
+// XOR-shared words: bit i of (s0, s1) are the two parties' shares of wire i
+// of a bit-sliced gate layer.  Randomness is dealer domain 14, word
+// M(g, j) = U(seed, 14, g, j) for global instance g: input masks j = 0..7,
+// AND op o takes a0, a1, b0, b1, c0 = M(8 + 5o + 0..4), c1 = (a0^a1)&(b0^b1)^c0
+// (a gate in bit i of a layer uses bit i of each word).
+struct Sh {
+  uint64_t s0, s1;
+};
+
+// GMW AND (bool_circuit/circuit.cpp:245-256): open d = x ^ a, e = y ^ b;
+// party 0 gets d&e ^ d&b0 ^ e&a0 ^ c0, party 1 d&b1 ^ e&a1 ^ c1.
+__device__ __forceinline__ Sh gmw_and(Sh x, Sh y, uint64_t seed, uint64_t g, uint32_t op) {
+  const uint64_t j = 8 + 5 * (uint64_t)op;
+  const uint64_t a0 = dealer_hd(seed, 14, g, j), a1 = dealer_hd(seed, 14, g, j + 1);
+  const uint64_t b0 = dealer_hd(seed, 14, g, j + 2), b1 = dealer_hd(seed, 14, g, j + 3);
+  const uint64_t c0 = dealer_hd(seed, 14, g, j + 4);
+  const uint64_t c1 = ((a0 ^ a1) & (b0 ^ b1)) ^ c0;
+  const uint64_t d = x.s0 ^ a0 ^ x.s1 ^ a1, e = y.s0 ^ b0 ^ y.s1 ^ b1;
+  return Sh{(d & e) ^ (d & b0) ^ (e & a0) ^ c0, (d & b1) ^ (e & a1) ^ c1};
+}
+
+// The Kogge-Stone adder (bool_circuit/adder.h:21-63) on nb-bit offsets owned
+// by party 0 (off0) and party 1 (off1), one bit-sliced layer per AND depth:
+// op 0 the generate ANDs, prefix step s (distance d) ops 2s-1 = P & G<<d and
+// 2s = P & P<<d over bits >= d.  Returns XOR shares of the nb+1 sum bits.
+__device__ Sh gmw_adder(uint64_t off0, uint64_t off1, int nb, uint64_t seed, uint64_t g) {
+  const uint64_t mask = (1ULL << nb) - 1;
+  const uint64_t m0 = dealer_hd(seed, 14, g, 0) & mask, m1 = dealer_hd(seed, 14, g, 1) & mask;
+  const Sh A{off0 ^ m0, m0}, B{m1, off1 ^ m1};  // inputs shared by their owners
+  Sh G = gmw_and(A, B, seed, g, 0);
+  G.s0 &= mask;
+  G.s1 &= mask;
+  Sh P{A.s0 ^ B.s0, A.s1 ^ B.s1};
+  const Sh P0 = P;
+  uint32_t s = 1;
+  for (int d = 1; d < nb; d <<= 1, ++s) {
+    const uint64_t hi = mask & ~((1ULL << d) - 1);
+    const Sh X = gmw_and(P, Sh{G.s0 << d, G.s1 << d}, seed, g, 2 * s - 1);
+    const Sh Y = gmw_and(P, Sh{P.s0 << d, P.s1 << d}, seed, g, 2 * s);
+    G.s0 ^= X.s0 & hi;
+    G.s1 ^= X.s1 & hi;
+    P.s0 = (P.s0 & ~hi) | (Y.s0 & hi);
+    P.s1 = (P.s1 & ~hi) | (Y.s1 & hi);
+  }
+  return Sh{P0.s0 ^ (G.s0 << 1), P0.s1 ^ (G.s1 << 1)};
+}
+
+// The F_DeltaShiftShare circuit (half_tree_dpf/dpf.cpp:172-240) on the alpha
+// shares and the Delta shares (each party's input shared by its masks),
+// writing both parties' DltSft shares [levels][nk].  Level i, child k uses
+// ops base = aops + (i*nk + k)*(w+1): its AND-tree gates base + j (bit 0),
+// its 128 Delta ANDs base + w-1 (bits 0..63) and base + w (bits 64..127).
+__device__ void gmw_dltsft(Sh alpha, Blk d0, Blk d1, int n, int w, int aops, uint64_t seed,
+                           uint64_t g, Blk* __restrict__ out0, Blk* __restrict__ out1) {
+  const int nk = (1 << w) - 1, levels = n / w;
+  const uint64_t m2 = dealer_hd(seed, 14, g, 2), m3 = dealer_hd(seed, 14, g, 3);
+  const Sh A{(alpha.s0 ^ m2) ^ m3, m2 ^ (alpha.s1 ^ m3)};
+  const uint64_t m4 = dealer_hd(seed, 14, g, 4), m5 = dealer_hd(seed, 14, g, 5);
+  const uint64_t m6 = dealer_hd(seed, 14, g, 6), m7 = dealer_hd(seed, 14, g, 7);
+  const Sh Xlo{d0.lo ^ m4 ^ m6, m4 ^ d1.lo ^ m6}, Xhi{d0.hi ^ m5 ^ m7, m5 ^ d1.hi ^ m7};
+  for (int i = 0; i < levels; ++i)
+    for (int k = 0; k < nk; ++k) {
+      const uint32_t base = (uint32_t)(aops + (i * nk + k) * (w + 1));
+      Sh f[4];
+      for (int c = 0; c < w; ++c) {
+        const int pos = n - w * (i + 1) + c;
+        f[c] = Sh{(A.s0 >> pos) & 1, (A.s1 >> pos) & 1};
+        if (!((k >> c) & 1)) f[c].s0 ^= 1;  // NOT: party 0 flips its share
+      }
+      int nf = w, tree = 0;
+      while (nf > 1) {  // pairwise, odd one carried (dpf.cpp:219-226)
+        int nn = 0;
+        for (int j = 0; j + 1 < nf; j += 2) {
+          const Sh z = gmw_and(f[j], f[j + 1], seed, g, base + (uint32_t)tree++);
+          f[nn++] = Sh{z.s0 & 1, z.s1 & 1};
+        }
+        if (nf & 1) f[nn++] = f[nf - 1];
+        nf = nn;
+      }
+      const Sh I{0ULL - f[0].s0, 0ULL - f[0].s1};  // the indicator in every bit
+      const Sh lo = gmw_and(I, Xlo, seed, g, base + (uint32_t)w - 1);
+      const Sh hi = gmw_and(I, Xhi, seed, g, base + (uint32_t)w);
+      out0[i * nk + k] = Blk{lo.s0, hi.s0};
+      out1[i * nk + k] = Blk{lo.s1, hi.s1};
+    }
+}
+
+// AND ops of the nb-bit adder: the generate layer and two per prefix step.
+int gmw_adder_ops(int nb) {
+  int s = 0;
+  for (int d = 1; d < nb; d <<= 1) ++s;
+  return 1 + 2 * s;
+}
 
 // The dealer of the spec above, on the device: the seed is all a request
 // carries to the GPU (t and bin are powers of two since t | N).  Noise: one
@@ -112,7 +211,7 @@ __global__ void public_a_kernel(uint64_t* __restrict__ a, uint64_t pub_seed, int
 __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __restrict__ roots,
                                  Blk* __restrict__ dlt, uint64_t* __restrict__ bt,
                                  uint64_t seed, int c, int t_log, int levels, int nk, int w,
-                                 int D_log, uint64_t bin, int ninst) {
+                                 int D_log, uint64_t bin, int ninst, int gmw, int gmw_aops) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = 1 << t_log, B = t << t_log;
   if (idx >= B) return;
@@ -133,6 +232,12 @@ __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __rest
   const size_t ni = (size_t)ninst;
   roots[g] = Blk{d0.lo ^ kWlo, d0.hi ^ kWhi};
   roots[ni + g] = Blk{d1.lo ^ kWlo, d1.hi ^ kWhi};
+  if (gmw) {
+    // alpha and DltSft by the parties' GMW circuits (pcg_ole_impl.h:114-199)
+    const Sh alpha = gmw_adder(p0 & (bin - 1), p1 & (bin - 1), D_log - 1, seed, G);
+    gmw_dltsft(alpha, d0, d1, D_log, w, gmw_aops, seed, G, dlt + (size_t)g * levels * nk,
+               dlt + (ni + g) * levels * nk);
+  } else
   for (int i = 0; i < levels; ++i) {
     const int digit = (int)((alpha >> (D_log - w * (i + 1))) & (uint64_t)((1 << w) - 1));
     for (int k = 0; k < nk; ++k) {
@@ -336,7 +441,9 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   hip_check(hipStreamSynchronize(pl->ctx->stream), "ole ahat");
 }
 
-void plan_deal(pcgsq_ole_plan* pl, uint64_t seed) {
+// gmw: alpha and DltSft by the GMW circuits instead of the dealer's formula
+// (same Delta shares and positions; DltSft shares differ, their XOR agrees).
+void plan_deal(pcgsq_ole_plan* pl, uint64_t seed, bool gmw = false) {
   const OleDims& d = pl->d;
   hipStream_t st = pl->ctx->stream;
   const int t_log = ilog((uint64_t)d.t);
@@ -346,7 +453,7 @@ void plan_deal(pcgsq_ole_plan* pl, uint64_t seed) {
   hipLaunchKernelGGL(deal_inst_kernel, dim3((unsigned)((d.B + 255) / 256), d.c, d.c), dim3(256),
                      0, st, pl->noise.as<uint64_t>(), pl->roots.as<Blk>(), pl->dlt.as<Blk>(),
                      pl->bt.as<uint64_t>(), seed, d.c, t_log, d.levels, d.nk, d.w, d.D_log,
-                     (uint64_t)d.bin, d.ninst);
+                     (uint64_t)d.bin, d.ninst, gmw ? 1 : 0, gmw_adder_ops(d.D_log - 1));
   hip_check(hipGetLastError(), "launch dealer");
   pl->dealt = true;
 }
@@ -477,6 +584,29 @@ int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed) {
     require(plan, "ole_deal: null plan");
     CtxGuard g(plan->ctx);
     plan_deal(plan, seed);
+  });
+}
+
+int pcgsq_ole_deal_gmw(pcgsq_ole_plan* plan, uint64_t seed) {
+  return guarded([&] {
+    require(plan, "ole_deal_gmw: null plan");
+    require(plan->d.D_log >= 1 && plan->d.D_log <= 63, "ole_deal_gmw: domain out of range");
+    CtxGuard g(plan->ctx);
+    plan_deal(plan, seed, true);
+  });
+}
+
+int pcgsq_ole_dealt_dltsft(pcgsq_ole_plan* plan, pcgsq_blk* dltsft) {
+  return guarded([&] {
+    require(plan && dltsft, "ole_dealt_dltsft: null argument");
+    require(plan->dealt, "ole_dealt_dltsft: nothing dealt");
+    CtxGuard g(plan->ctx);
+    const OleDims& d = plan->d;
+    const size_t n = (size_t)2 * d.ninst * d.levels * d.nk;
+    hip_check(hipMemcpyAsync(dltsft, plan->dlt.ptr, n * sizeof(Blk), hipMemcpyDeviceToHost,
+                             plan->ctx->stream),
+              "ole dltsft");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole dltsft");
   });
 }
 

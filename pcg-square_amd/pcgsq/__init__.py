@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
     L.pcgsq_ole_plan_bytes.argtypes = [vp]
     L.pcgsq_ole_plan_bytes.restype = C.c_size_t
     L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
+    L.pcgsq_ole_deal_gmw.argtypes = [vp, C.c_uint64]
+    L.pcgsq_ole_dealt_dltsft.argtypes = [vp, C.c_void_p]
     L.pcgsq_ole_run.argtypes = [vp]
     L.pcgsq_ole_run_blocks.argtypes = [vp]
     L.pcgsq_ole_outputs.argtypes = [vp, u64p, u64p, u64p, u64p]
@@ -463,8 +465,19 @@ class OlePlan:
     def device_bytes(self) -> int:
         return lib().pcgsq_ole_plan_bytes(self.h)
 
-    def deal(self, seed: int) -> None:
-        _check(lib().pcgsq_ole_deal(self.h, seed))
+    def deal(self, seed: int, gmw: bool = False) -> None:
+        """Deal the inputs of `seed`; gmw=True runs the parties' preprocessing
+        circuits (position adder, DltSft) under GMW on the device instead of
+        taking DltSft from the dealer (same outputs)."""
+        _check((lib().pcgsq_ole_deal_gmw if gmw else lib().pcgsq_ole_deal)(self.h, seed))
+
+    def dealt_dltsft(self) -> np.ndarray:
+        """Both parties' dealt DltSft shares, [2][c*c*t*t][levels][2^w - 1][2]."""
+        n = (2 * self.N // self.t).bit_length() - 1
+        levels, nk = n // self.w, (1 << self.w) - 1
+        out = np.empty((2, self.c * self.c * self.t * self.t, levels, nk, 2), dtype=np.uint64)
+        _check(lib().pcgsq_ole_dealt_dltsft(self.h, out.ctypes.data))
+        return out
 
     def run(self) -> None:
         _check(lib().pcgsq_ole_run(self.h))

@@ -12,6 +12,7 @@
 //      alias case out == lhs (:121-135), against a schoolbook product
 //   4. OleExpander: x0 * x1 == z0 + z1 at N = 2^12
 //   5. OleExpander::triples: every slot c0 + c1 == (a0 + a1)(b0 + b1)
+//   6. OleExpander::expand(seed, gmw = true) == expand(seed)
 // Exit 0 = PASS, 1 = FAIL, 77 = no device (the reference's skip code).
 #include <stdio.h>
 #include <string.h>
@@ -168,6 +169,11 @@ int main() {
     }
     CHECK(tok, "triples: c0 + c1 == (a0 + a1)(b0 + b1)");
     printf("triples N=%d: %s\n", N, tok ? "hold" : "FAIL");
+    // 6. the preprocessing under GMW on the device gives the same expansion
+    auto og = ole.expand(3, true);
+    const bool same = og.x0 == o.x0 && og.z0 == o.z0 && og.x1 == o.x1 && og.z1 == o.z1;
+    CHECK(same, "GMW-preprocessed expansion == dealer expansion");
+    printf("GMW preprocessing N=%d: %s\n", N, same ? "same outputs" : "DIFFERS");
   }
   printf("%s\n", fails ? "RESULT: FAIL" : "RESULT: PASS");
   return fails ? 1 : 0;

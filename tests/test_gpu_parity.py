@@ -297,3 +297,57 @@ def test_triples_full_size_are_triples(ctx, pq):
     prod = [int(x) for x in ((al * bl).astype(object) + ((al * bh + ah * bl).astype(object) << 31)
                              + ((ah * bh).astype(object) << 62))]
     assert all(p % P == int(v) for p, v in zip(prod, cc))
+
+
+# ---- GMW preprocessing on the device (pcg_ole_impl.h:114-199) -----------------
+
+@pytest.mark.parametrize("N,c,t,w", [(1 << 11, 2, 16, 1), (1 << 11, 2, 16, 2), (1 << 11, 2, 16, 4),
+                                     (1 << 12, 2, 16, 3), (1 << 13, 3, 8, 1)])
+def test_gmw_preprocessing_matches_oracle(ctx, pq, N, c, t, w):
+    """Both parties' DltSft shares from the device's bit-sliced GMW
+    evaluation equal the oracle's gate-by-gate evaluation, bit for bit, and
+    reconstruct to the dealer's DltSft."""
+    exp = O.dealt_inputs(N, c, t, w, seed=41, gmw=True)["dltsft"]
+    dealer = O.dealt_inputs(N, c, t, w, seed=41)["dltsft"]
+    with pq.OlePlan(ctx, N, c, t, w) as pl:
+        pl.deal(41, gmw=True)
+        got = pl.dealt_dltsft()
+        pl.deal(41)
+        got_dealer = pl.dealt_dltsft()
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got_dealer, dealer)
+    assert np.array_equal(got[0] ^ got[1], got_dealer[0] ^ got_dealer[1])
+
+
+@pytest.mark.parametrize("N,c,t,w,prg", [(1 << 12, 2, 16, 1, 1), (1 << 12, 2, 16, 3, 1),
+                                         (1 << 12, 2, 16, 1, 0)])
+def test_gmw_expansion_matches_oracle(ctx, pq, N, c, t, w, prg):
+    """An expansion whose preprocessing ran under GMW on the device gives the
+    oracle's x0, z0, x1, z1 (the dealer path's outputs) bit for bit."""
+    exp = O.pcg_ole_expand(N, c, t, w, prg, seed=43, pub_seed=0, threads=8)
+    with pq.OlePlan(ctx, N, c, t, w, prg) as pl:
+        pl.deal(43, gmw=True)
+        pl.run()
+        got = pl.outputs()
+    for g, e, name in zip(got, exp, ("x0", "z0", "x1", "z1")):
+        assert np.array_equal(g, e), name
+
+
+def test_gmw_expansion_full_size_equals_dealer_path(ctx, pq):
+    """N = 2^20, c = 4, t = 16: the GMW-preprocessed expansion's outputs
+    equal the dealer path's, bit for bit (4096 instances per party, 2290 AND
+    gates each)."""
+    N = 1 << 20
+    with pq.OlePlan(ctx, N, 4, 16) as pl:
+        pl.deal(45)
+        pl.run()
+        ref = [O.fnv1a64(v) for v in pl.outputs()]
+        pl.deal(45, gmw=True)
+        d_g = pl.dealt_dltsft()
+        pl.run()
+        got = [O.fnv1a64(v) for v in pl.outputs()]
+        pl.deal(45)
+        d_d = pl.dealt_dltsft()
+    assert got == ref
+    assert np.array_equal(d_g[0] ^ d_g[1], d_d[0] ^ d_d[1])
+    assert not np.array_equal(d_g[0], d_d[0])

@@ -305,3 +305,36 @@ def test_served_hash_fixture_first_entry():
     outs = O.pcg_ole_expand(r["N"], r["c"], r["t"], r["w"], O.PRG_CHACHA8, seed=r["seed"],
                             pub_seed=r["pub_seed"], threads=8)
     assert [f"{O.fnv1a64(v):016x}" for v in outs] == r["fnv1a64"]
+
+
+@pytest.mark.parametrize("N,c,t,w", [(1 << 11, 2, 16, 1), (1 << 11, 2, 16, 2), (1 << 11, 2, 16, 4),
+                                     (1 << 12, 2, 16, 3), (1 << 13, 3, 8, 1)])
+def test_gmw_preprocessing_reconstructs(N, c, t, w):
+    """The GMW preprocessing (pcg_ole_impl.h:114-199) evaluated gate by gate:
+    for every instance the adder's XOR shares reconstruct alpha = off0 + off1
+    (a closed form, here the dealer's alpha) and the DltSft circuit's shares
+    reconstruct [digit_i(alpha) = k] * Delta (the dealer's XOR); the shares
+    themselves are fresh, and every other dealt value is the dealer's."""
+    d = O.dealt_inputs(N, c, t, w, seed=17)
+    g = O.dealt_inputs(N, c, t, w, seed=17, gmw=True)
+    assert np.array_equal(g["alpha_sh"][0] ^ g["alpha_sh"][1], d["alpha"])
+    assert np.array_equal(g["dltsft"][0] ^ g["dltsft"][1], d["dltsft"][0] ^ d["dltsft"][1])
+    assert not np.array_equal(g["dltsft"][0], d["dltsft"][0])
+    assert not np.array_equal(g["alpha_sh"][0], d["alpha"])
+    for k in ("pos", "pay", "root", "beta", "r", "s", "rs"):
+        assert np.array_equal(g[k], d[k]), k
+
+
+def test_gmw_and_gate_counts():
+    """AND gates per instance: the Kogge-Stone adder's nb + sum_d 2(nb - d)
+    (bool_circuit/adder.h:3-5) plus levels * (2^w - 1) * (w - 1 + 128) for
+    the DltSft circuit (dpf.cpp:64-70)."""
+    def ks(nb):
+        s, d = nb, 1
+        while d < nb:
+            s += 2 * (nb - d)
+            d <<= 1
+        return s
+    for nb, w, n in ((8, 1, 9), (16, 1, 17), (7, 2, 8), (11, 4, 12), (29, 1, 30)):
+        assert O.gmw_and_gates(nb, w, n) == ks(nb) + (n // w) * ((1 << w) - 1) * (w - 1 + 128)
+        assert O.gmw_adder_ops(nb) == 1 + 2 * max(0, (nb - 1).bit_length())
