@@ -185,6 +185,26 @@ def bench_dpf_batch(ctx, pcgsq, ceiling_blocks: float, reps: int = 5) -> dict:
     return out
 
 
+def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
+    """Higher DPF arity (w = 1, 2, 4: up to 15 children per ChaCha8 block) at
+    N = 2^20, c = 4, t = 32 (DPF domain 2^16, divisible by every w; the
+    headline's 2^17 admits only w = 1)."""
+    N, c, t = 1 << 20, 4, 32
+    out = {"workload": "pcg-ole expansion N=2^20, c=4, t=32, ChaCha8, both parties"}
+    for w in (1, 2, 4):
+        with pcgsq.OlePlan(ctx, N, c, t, w) as pl:
+            pl.deal(1)
+            pl.run()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                pl.run()
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[f"w{w}"] = {"ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
+    return out
+
+
 def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     """Served PCG-OLE: Poisson arrivals (rate = 75% of the measured
     one-at-a-time capacity), N uniform over {2^16, 2^18, 2^20, 2^22}, c=4,
@@ -256,12 +276,12 @@ def main() -> int:
     ap.add_argument("--c", type=int, default=4)
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,served",
-                    help="comma list of expand, ntt_batch, dpf_batch, served; without expand "
-                         "only the named parts run and print (for profiling one workload)")
+    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,arity,served",
+                    help="comma list of expand, ntt_batch, dpf_batch, arity, served; without "
+                         "expand only the named parts run and print (for profiling one workload)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
-    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "served"}
+    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "served"}
     if unknown:
         ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
@@ -280,6 +300,8 @@ def main() -> int:
                 res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq)
             if "dpf_batch" in parts:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
+            if "arity" in parts:
+                res["arity"] = bench_arity(ctx, pcgsq)
         if "served" in parts:
             res["served"] = bench_served(pcgsq)
         print(json.dumps(res), flush=True)
@@ -332,6 +354,7 @@ def main() -> int:
         copy_bps = ctx.probe_copy(1 << 30)
         ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
+        arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
     served = bench_served(pcgsq) if "served" in parts else None
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
@@ -398,6 +421,7 @@ def main() -> int:
         "hbm_copy_GBps": copy_bps / 1e9,
         "batched_ntt": ntt_batch,
         "batched_dpf": dpf_batch,
+        "arity": arity,
         "served": served,
     }
     if not args.no_cpu:
