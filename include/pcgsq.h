@@ -75,6 +75,14 @@ int pcgsq_ctx_profile_read(pcgsq_ctx* ctx, char* buf, size_t len);
 /* Register-only ChaCha8 block throughput of the device (blocks/s): the
  * compute ceiling the DPF kernels are measured against. */
 int pcgsq_probe_chacha8(pcgsq_ctx* ctx, int iters, double* blocks_per_s);
+/* Device F_p arithmetic on host arrays of n values, for checking it against
+ * exact arithmetic (no counterpart in the reference; a self-test of the
+ * device field code): op 0 high 64 bits of a*b (the butterflies' form),
+ * 1 Shoup a*b mod P with c = floor(b*2^64/P), result in [0, 2P) (the
+ * butterflies' form), 2 the same in the general form, 3 a*b mod P for
+ * a, b < 2^62.  c may be NULL for ops 0 and 3.  Synchronizes. */
+int pcgsq_field_op(pcgsq_ctx* ctx, int op, size_t n, const uint64_t* a, const uint64_t* b,
+                   const uint64_t* c, uint64_t* out);
 /* Plain device copy of `bytes` (read + write counted): achievable HBM
  * bytes/s, and the known-size kernel that HBM counters are scaled by. */
 int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s);

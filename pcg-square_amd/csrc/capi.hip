@@ -74,6 +74,22 @@ __global__ void chacha8_probe_kernel(uint32_t* out, int iters) {
   out[t] = k0 ^ k1 ^ k2 ^ k3;
 }
 
+// Device F_p arithmetic applied elementwise, for checking it against exact
+// host arithmetic: op 0 mulhi64_dev(a, b), 1 shoup_lazy_dev(a, b, c),
+// 2 shoup_lazy(a, b, c), 3 fmul(a, b).
+__global__ void field_op_kernel(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                                const uint64_t* __restrict__ c, uint64_t* __restrict__ o,
+                                size_t n, int op) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  switch (op) {
+    case 0: o[i] = mulhi64_dev(a[i], b[i]); break;
+    case 1: o[i] = shoup_lazy_dev(a[i], b[i], c[i]); break;
+    case 2: o[i] = shoup_lazy(a[i], b[i], c[i]); break;
+    default: o[i] = fmul(a[i], b[i]); break;
+  }
+}
+
 // Plain 16-byte-per-thread copy: the known-size reference for HBM counters.
 __global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -274,6 +290,31 @@ int pcgsq_probe_chacha8(pcgsq_ctx* ctx, int iters, double* blocks_per_s) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *blocks_per_s = (double)grid * tpb * iters / (best * 1e-3);
+  });
+}
+
+int pcgsq_field_op(pcgsq_ctx* ctx, int op, size_t n, const uint64_t* a, const uint64_t* b,
+                   const uint64_t* c, uint64_t* out) {
+  return guarded([&] {
+    require(ctx && a && b && out && n > 0, "field_op: bad argument");
+    require(op >= 0 && op <= 3, "field_op: op must be 0..3");
+    require(c || (op != 1 && op != 2), "field_op: Shoup needs the quotients");
+    CtxGuard g(ctx);
+    DevBuf da, db, dc, dout;
+    da.ensure(n * 8, "field_op");
+    db.ensure(n * 8, "field_op");
+    dc.ensure(n * 8, "field_op");
+    dout.ensure(n * 8, "field_op");
+    hip_check(hipMemcpyAsync(da.ptr, a, n * 8, hipMemcpyHostToDevice, ctx->stream), "field_op");
+    hip_check(hipMemcpyAsync(db.ptr, b, n * 8, hipMemcpyHostToDevice, ctx->stream), "field_op");
+    hip_check(hipMemcpyAsync(dc.ptr, c ? c : b, n * 8, hipMemcpyHostToDevice, ctx->stream),
+              "field_op");
+    hipLaunchKernelGGL(field_op_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, da.as<uint64_t>(), db.as<uint64_t>(), dc.as<uint64_t>(),
+                       dout.as<uint64_t>(), n, op);
+    hip_check(hipGetLastError(), "launch field_op");
+    hip_check(hipMemcpyAsync(out, dout.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream), "field_op");
+    hip_check(hipStreamSynchronize(ctx->stream), "field_op");
   });
 }
 

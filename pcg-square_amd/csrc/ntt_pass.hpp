@@ -135,7 +135,7 @@ __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_
         const int w2 = w | (1 << bit);
         const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
         const uint64_t U = red2p(v[w]);
-        const uint64_t T = shoup_lazy(v[w2], t.w, t.wq);
+        const uint64_t T = shoup_lazy_dev(v[w2], t.w, t.wq);
         v[w] = U + T;
         v[w2] = U - T + kTwoP;
       }
@@ -152,7 +152,7 @@ __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_
         const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
         const uint64_t U = v[w], V = v[w2];  // both < 2P
         v[w] = red2p(U + V);
-        v[w2] = shoup_lazy(U - V + kTwoP, t.w, t.wq);
+        v[w2] = shoup_lazy_dev(U - V + kTwoP, t.w, t.wq);
       }
     }
   }
@@ -248,10 +248,10 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
           const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
           const Pair a = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
           const Pair b = ld_pair((gcptr)d.tw_hi, ex >> d.tw_lo_bits);
-          x = shoup_lazy(x, a.w, a.wq);
-          x = shoup_lazy(x, b.w, b.wq);
+          x = shoup_lazy_dev(x, a.w, a.wq);
+          x = shoup_lazy_dev(x, b.w, b.wq);
         } else if constexpr (Tr::STORE == 2) {
-          x = shoup_lazy(x, d.scale_w, d.scale_wq);
+          x = shoup_lazy_dev(x, d.scale_w, d.scale_wq);
         }
         // a first pass hands its [0, 2P) values to the second as they are
         // (forward stages take [0, 4P), inverse stages [0, 2P)); only a

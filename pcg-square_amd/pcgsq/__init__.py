@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
     L.pcgsq_ctx_profile_read.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.pcgsq_probe_chacha8.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.pcgsq_probe_copy.argtypes = [vp, C.c_size_t, C.POINTER(C.c_double)]
+    L.pcgsq_field_op.argtypes = [vp, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]
     L.pcgsq_dev_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.pcgsq_dev_free.argtypes = [vp, vp]
     L.pcgsq_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
@@ -249,6 +250,16 @@ class Context:
         v = C.c_double()
         _check(lib().pcgsq_probe_copy(self.h, nbytes, C.byref(v)))
         return v.value
+
+    def field_op(self, op: int, a, b, c=None) -> np.ndarray:
+        """Device F_p arithmetic elementwise (op 0 mulhi64, 1 the butterflies'
+        Shoup multiply, 2 the general Shoup multiply, 3 a*b mod P)."""
+        a, b = _u64(a), _u64(b)
+        cc = None if c is None else _u64(c)
+        out = np.empty_like(a)
+        _check(lib().pcgsq_field_op(self.h, op, len(a), _p(a), _p(b),
+                                    None if cc is None else _p(cc), _p(out)))
+        return out
 
     def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
         """Copy a host array into a new device buffer owned by this context."""

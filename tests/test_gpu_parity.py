@@ -351,3 +351,29 @@ def test_gmw_expansion_full_size_equals_dealer_path(ctx, pq):
     assert got == ref
     assert np.array_equal(d_g[0] ^ d_g[1], d_d[0] ^ d_d[1])
     assert not np.array_equal(d_g[0], d_d[0])
+
+
+def test_device_field_arithmetic_exact(ctx, pq):
+    """The butterflies' 64x64 high product (carry from v_mad_u64_u32's
+    carry-out) and Shoup multiply (x*w + q*(2^64 - P) in 32-bit pieces)
+    against exact integers: random operands over the whole 64-bit range plus
+    every pair of edge values (0, 1, 2^32 - 1, 2^32, 2^63, 2^64 - 1, P - 1,
+    2P - 1, 4P - 1); the Shoup form also equals the general one bit for bit."""
+    rng = np.random.default_rng(77)
+    edges = [0, 1, (1 << 32) - 1, 1 << 32, 1 << 63, (1 << 64) - 1, P - 1, 2 * P - 1, 4 * P - 1]
+    a = np.concatenate([rng.integers(0, 2**64, 20000, dtype=np.uint64),
+                        np.array([x for x in edges for _ in edges], dtype=np.uint64)])
+    b = np.concatenate([rng.integers(0, 2**64, 20000, dtype=np.uint64),
+                        np.array([y for _ in edges for y in edges], dtype=np.uint64)])
+    hi = ctx.field_op(0, a, b)
+    for x, y, h in zip(a, b, hi):
+        assert int(h) == (int(x) * int(y)) >> 64, (int(x), int(y))
+    # Shoup: w < P, wq = floor(w 2^64 / P), any 64-bit x; result in [0, 2P)
+    w = b % np.uint64(P)
+    w_edges = [0, 1, 2, P - 2, P - 1, 1 << 61, P // 2]
+    w[:len(w_edges)] = w_edges
+    wq = np.array([(int(v) << 64) // P for v in w], dtype=np.uint64)
+    r1 = ctx.field_op(1, a, w, wq)
+    assert np.array_equal(r1, ctx.field_op(2, a, w, wq))
+    for x, ww, r in zip(a, w, r1):
+        assert int(r) < 2 * P and int(r) % P == (int(x) * int(ww)) % P, (int(x), int(ww))
