@@ -97,21 +97,35 @@ __global__ __launch_bounds__(kTpb) void gather_g_kernel(GatherArgs a) {
       const size_t d = rr + (size_t)ds * bin;
       const int kk_lo = sd - t + 1 > 0 ? sd - t + 1 : 0;
       const int kk_hi = sd < t - 1 ? sd : t - 1;
-      for (int kk = kk_lo; kk <= kk_hi; ++kk) {
-        const int inst = kk * t + (sd - kk);
-        if (inst >= B) break;
-        uint64_t C, tau;
-        if constexpr (FROM_LEAVES) {
+      if constexpr (FROM_LEAVES) {
+        for (int kk = kk_lo; kk <= kk_hi; ++kk) {
+          const int inst = kk * t + (sd - kk);
+          if (inst >= B) break;
           const Blk leaf = lv[(size_t)inst * D + d];
-          C = freduce64(out_hash_lo64(leaf));
-          tau = leaf.lo & 1ULL;
-        } else {
-          const uint64_t v = ct[(size_t)inst * D + d];
-          C = v & ((1ULL << 63) - 1);
-          tau = v >> 63;
+          const uint64_t C = freduce64(out_hash_lo64(leaf));
+          part = fadd(part, (leaf.lo & 1ULL) ? fadd(C, CW[inst]) : C);
         }
-        const uint64_t y = tau ? fadd(C, CW[inst]) : C;
-        part = fadd(part, y);
+      } else {
+        // kRows rows' loads in flight per thread (the instance index grows
+        // with kk, so rows past B end the range)
+        constexpr int kRows = 4;
+        for (int kk = kk_lo; kk <= kk_hi; kk += kRows) {
+          uint64_t v[kRows], cw[kRows];
+          bool ok[kRows];
+#pragma unroll
+          for (int j = 0; j < kRows; ++j) {
+            const int k2 = kk + j;
+            const int inst = k2 * t + (sd - k2);
+            ok[j] = k2 <= kk_hi && inst < B;
+            v[j] = ok[j] ? ct[(size_t)inst * D + d] : 0;
+            cw[j] = ok[j] ? CW[inst] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kRows; ++j) {
+            const uint64_t C = v[j] & ((1ULL << 63) - 1);
+            part = fadd(part, (v[j] >> 63) ? fadd(C, cw[j]) : C);
+          }
+        }
       }
     }
     acc = h ? fsub(acc, part) : fadd(acc, part);
