@@ -92,17 +92,29 @@ def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) 
                        f"{full:.1f} s")}
 
 
-def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
+NTT_BATCH_PROFILE = os.path.join(ROOT, "profiles", "ntt_batch_256_uneven_2e12_2e20.md")
+
+
+def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
     """Batched NTT of uneven lengths: 256 transforms, lengths drawn uniformly
     from {2^12, 2^14, 2^16, 2^18, 2^20}, uniform coefficients in [0, P),
-    forward then inverse of every transform in one persistent launch (graph
-    replay), autotuned; against one launch per transform (the plain form)."""
+    forward then inverse of every transform in one replay of the batch's
+    captured launches.  The autotuner picks the splits of the persistent
+    launch, its workgroups per CU and then the mode (persistent, or grouped
+    by length over offset tables); against one launch per transform (the
+    plain form) and against the same transforms regrouped by length into
+    contiguous batches.  `mode` ("persistent" / "grouped") skips the tuner and
+    the comparisons: only that mode runs (the profiled command)."""
     import numpy as np
     rng = np.random.default_rng(2024)
     logN = [int(12 + 2 * x) for x in rng.integers(0, 5, 256)]
     with pcgsq.NttBatch(ctx, logN) as b:
         b.load(rng.integers(0, pcgsq.P, b.total, dtype=np.uint64))
-        shape = b.autotune()
+        if mode is None:
+            shape = b.autotune()
+        else:
+            b.set_mode(pcgsq.NTT_BATCH_GROUPED if mode == "grouped" else pcgsq.NTT_BATCH_PERSISTENT)
+            shape = {"mode": mode, "wgs_per_cu": 0, "splits": {}}
         b.run(graph=True)
         b.finish()
         ctx.profile(True)
@@ -114,6 +126,9 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
         ms = z["ntt.batch"][1] / z["ntt.batch"][0]
         info = b.info()
         pa, _, _, _ = b.dev()
+        if mode is not None:
+            return {"mode": mode, "ms": ms, "alg_bytes": info["alg_bytes"],
+                    "GBps": info["alg_bytes"] / (ms * 1e-3) / 1e9, "replays": reps + 1}
 
         def plain():
             for i, L in enumerate(logN):
@@ -128,21 +143,52 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5) -> dict:
             plain()
         ctx.synchronize()
         ms_plain = (time.perf_counter() - t0) * 1e3 / reps
+
+    # The same transforms regrouped: one contiguous batch per length, each
+    # run by the per-length pass launches (summed device time of those
+    # launches; the regrouping copy is not counted).
+    counts = {L: logN.count(L) for L in sorted(set(logN))}
+    groups = {L: ctx.to_device(np.zeros(k << L, dtype=np.uint64)) for L, k in counts.items()}
+    try:
+        def grouped():
+            for L, k in counts.items():
+                ctx.ntt_forward_dev(groups[L].ptr, k, 1 << L)
+                ctx.ntt_inverse_dev(groups[L].ptr, k, 1 << L)
+
+        grouped()
+        ctx.synchronize()
+        ctx.profile(True)
+        for _ in range(reps):
+            grouped()
+        ctx.synchronize()
+        z = ctx.profile_read()
+        ctx.profile(False)
+        ms_grouped = sum(v[1] for k, v in z.items() if k.startswith("ntt.")) / reps
+    finally:
+        for g in groups.values():
+            g.close()
     gbps = info["alg_bytes"] / (ms * 1e-3) / 1e9
+    grouped_mode = shape["mode"] == "grouped"
     return {"workload": "256 forward+inverse NTTs, lengths uniform over 2^12..2^20 (x4 steps), "
-                        "one persistent launch from per-XCD work lists, graph replay",
+                        "one replay of the batch's captured launches, autotuned mode",
             "lengths": {f"2^{L}": logN.count(L) for L in sorted(set(logN))},
             "ms": ms, "alg_bytes": info["alg_bytes"], "GBps": gbps,
             "hbm_frac": gbps * 1e9 / HBM_PEAK, "work_items": info["items"],
-            "grid": info["grid"], "tuned": {"wgs_per_cu": shape["wgs_per_cu"],
+            "grid": info["grid"], "tuned": {"mode": shape["mode"],
+                                            "wgs_per_cu": shape["wgs_per_cu"],
                                             "splits": {f"2^{k}": v for k, v in
                                                        shape["splits"].items()}},
             "one_launch_per_transform_ms": ms_plain,
             "speedup_vs_one_launch_per_transform": ms_plain / ms,
-            "roofline": {"bound": "hbm", "kernel": "ntt_batch_kernel", "achieved": gbps,
-                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": gbps * 1e9 / HBM_PEAK,
-                         "traffic": pmc_traffic(os.path.join(
-                             ROOT, "profiles", "ntt_batch_256_uneven_2e12_2e20.md")),
+            "contiguous_by_length_ms": ms_grouped,
+            "roofline": {"bound": "hbm",
+                         "kernel": ("ntt_pass_kernel, per-length launches over offset tables"
+                                    if grouped_mode else "ntt_batch_kernel (persistent)"),
+                         "achieved": gbps, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": gbps * 1e9 / HBM_PEAK,
+                         "traffic": pmc_traffic(NTT_BATCH_PROFILE,
+                                                "grouped_hbm_bytes_per_run" if grouped_mode
+                                                else "persistent_hbm_bytes_per_run"),
                          "avg_launch_ms": ms}}
 
 
@@ -279,6 +325,9 @@ def main() -> int:
     ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,arity,served",
                     help="comma list of expand, ntt_batch, dpf_batch, arity, served; without "
                          "expand only the named parts run and print (for profiling one workload)")
+    ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
+                    help="batched NTT: run only this mode, untuned, with no comparisons "
+                         "(for profiling it)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
     unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "served"}
@@ -297,7 +346,7 @@ def main() -> int:
         with pcgsq.Context(0) as ctx:
             res["hbm_copy_GBps"] = ctx.probe_copy(1 << 30) / 1e9   # PMC calibration kernel
             if "ntt_batch" in parts:
-                res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq)
+                res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq, mode=args.ntt_batch_mode)
             if "dpf_batch" in parts:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:

@@ -197,6 +197,8 @@ int pcgsq_ntt_tune(pcgsq_ctx* ctx, int N, int n1_log, int variant, int cols1, in
 typedef struct pcgsq_ntt_batch pcgsq_ntt_batch;
 #define PCGSQ_NTT_BATCH_FORWARD 1
 #define PCGSQ_NTT_BATCH_INVERSE 2
+#define PCGSQ_NTT_BATCH_PERSISTENT 0 /* one persistent launch (default) */
+#define PCGSQ_NTT_BATCH_GROUPED 1    /* per-length pass launches over offset tables */
 int pcgsq_ntt_batch_create(pcgsq_ctx* ctx, const int* logN, int count, int ops,
                            pcgsq_ntt_batch** out);
 int pcgsq_ntt_batch_destroy(pcgsq_ntt_batch* b);
@@ -207,6 +209,11 @@ int pcgsq_ntt_batch_destroy(pcgsq_ntt_batch* b);
  * (0 = default); logN = 0 changes only the last two. */
 int pcgsq_ntt_batch_tune(pcgsq_ntt_batch* b, int logN, int n1_log, int wgs_per_cu,
                          uint32_t spin_limit);
+/* How a run executes (results are the same): PCGSQ_NTT_BATCH_PERSISTENT, one
+ * persistent launch draining per-XCD work lists; or PCGSQ_NTT_BATCH_GROUPED,
+ * for each length the per-length pass launches (the context's tuned shape for
+ * that length) over all transforms of that length through an offset table. */
+int pcgsq_ntt_batch_set_mode(pcgsq_ntt_batch* b, int mode);
 /* Device buffers (inputs a, forward f, inverse r) and their length in words. */
 int pcgsq_ntt_batch_dev(pcgsq_ntt_batch* b, uint64_t** a, uint64_t** f, uint64_t** r,
                         size_t* total_words);
@@ -219,8 +226,10 @@ int pcgsq_ntt_batch_run(pcgsq_ntt_batch* b, int use_graph);
 int pcgsq_ntt_batch_finish(pcgsq_ntt_batch* b);
 /* Copy f and/or r to the host (either may be NULL; synchronizes). */
 int pcgsq_ntt_batch_read(pcgsq_ntt_batch* b, uint64_t* f_host, uint64_t* r_host);
-/* Work items, grid, LDS bytes per workgroup and the algorithmic HBM bytes of
- * one run (16 * N * passes per transform and direction). */
+/* Work items (grouped mode: launches), grid and LDS bytes per workgroup of
+ * the persistent launch (0 in grouped mode), and the algorithmic HBM bytes of
+ * one run: 16 * N * p(N) per transform and direction, p = 1 up to 2^13 (the
+ * transform fits one workgroup's LDS), else 2. */
 int pcgsq_ntt_batch_info(pcgsq_ntt_batch* b, int* nitems, int* grid, size_t* lds_bytes,
                          double* alg_bytes);
 

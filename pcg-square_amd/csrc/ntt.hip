@@ -29,7 +29,8 @@ __global__ __launch_bounds__(kPassThreads) void ntt_pass_kernel(NttPassDesc d, u
   extern __shared__ uint64_t lds[];
   uint32_t bid = blockIdx.x;
   if ((tiles & 7) == 0) bid = (bid & 7) * (tiles >> 3) + (bid >> 3);
-  nttpass::pass_tile<K, RA, RB, RC>(d, lds, blockIdx.y, bid);
+  const size_t base = d.boff ? (size_t)d.boff[blockIdx.y] : (size_t)blockIdx.y << d.N_log;
+  nttpass::pass_tile<K, RA, RB, RC>(d, lds, base, bid);
 }
 
 // ---- naive per-stage kernels ------------------------------------------------
@@ -250,8 +251,10 @@ void launch_pass(hipStream_t st, NttPassDesc d, int nlog, int variant, int want_
 }
 
 void launch_spec(hipStream_t st, PassSpec ps, int batch, Profiler* prof, const char* zone) {
-  const int grid_y = ps.kind == kFwdWhole || ps.kind == kInvWhole || ps.kind == kInvWholeProd
-                         ? 1 : batch;
+  // whole transforms back to back are the columns of one launch; with an
+  // offset table each is a launch row of its own
+  const bool whole = ps.kind == kFwdWhole || ps.kind == kInvWhole || ps.kind == kInvWholeProd;
+  const int grid_y = whole && !ps.d.boff ? 1 : batch;
   switch (ps.kind) {
     case kFwdCols: return launch_pass<kFwdCols>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
     case kFwdRows: return launch_pass<kFwdRows>(st, ps.d, ps.nlog, ps.variant, ps.cols, ps.ncols, grid_y, prof, zone);
@@ -309,6 +312,11 @@ void naive_inverse_brv(hipStream_t st, NttPlan& pl, uint64_t* d, int batch) {
 }
 
 }  // namespace
+
+void ntt_launch_pass(hipStream_t st, const PassSpec& ps, int batch, Profiler* prof,
+                     const char* zone) {
+  launch_spec(st, ps, batch, prof, zone);
+}
 
 void launch_pointwise_mul(hipStream_t st, uint64_t* out, const uint64_t* a, const uint64_t* b,
                           size_t n) {

@@ -25,6 +25,12 @@
 // Per transform: ops & 1 = forward (a -> f), ops & 2 = inverse (of f when
 // both are asked, else of a) -> r.  Results equal the per-length launches
 // bit for bit (same passes, same tables).
+//
+// The batch can instead run grouped by length (PCGSQ_NTT_BATCH_GROUPED): per
+// length, the per-length pass kernels in launches of their own over all
+// transforms of that length, found through an offset table (no regrouping
+// copy), shaped as the context's per-length launches are (the autotuner's
+// split, grouping and tile widths).  The autotuner picks the faster mode.
 #include <algorithm>
 #include <map>
 #include <memory>
@@ -67,24 +73,27 @@ struct BatchQueues {
 };
 
 // Register grouping per pass length, chosen so a tile is 256 threads wide:
-// the largest group and the number of groups of the shapes below.
+// the largest group and the number of groups of the shapes below.  The kernel
+// takes the VGPRs of its largest shape; 2^8 runs as 3+3+2 stages (two 4-stage
+// groups need 132 VGPRs, which leaves room for only 3 workgroups per CU where
+// the LDS holds 4; every shape now stays under 128).
 __host__ __device__ constexpr int batch_rmax(int nlog) {
-  return (nlog == 6 || nlog == 9) ? 3 : 4;
+  return (nlog == 6 || nlog == 8 || nlog == 9) ? 3 : 4;
 }
-__host__ __device__ constexpr int batch_groups(int nlog) { return nlog >= 9 ? 3 : 2; }
+__host__ __device__ constexpr int batch_groups(int nlog) { return nlog >= 8 ? 3 : 2; }
 
 // One-pass transforms up to 2^12; four-step passes of 2^6 (rows only) to 2^11.
 #define PCGSQ_BATCH_SHAPES(X)                                                                 \
   X(kFwdWhole, 10, 4, 3, 3) X(kFwdWhole, 11, 4, 4, 3) X(kFwdWhole, 12, 4, 4, 4)              \
   X(kInvWhole, 10, 4, 3, 3) X(kInvWhole, 11, 4, 4, 3) X(kInvWhole, 12, 4, 4, 4)              \
   X(kFwdRows, 6, 3, 3, 0) X(kInvRows, 6, 3, 3, 0)                                            \
-  X(kFwdCols, 7, 4, 3, 0) X(kFwdCols, 8, 4, 4, 0) X(kFwdCols, 9, 3, 3, 3)                    \
+  X(kFwdCols, 7, 4, 3, 0) X(kFwdCols, 8, 3, 3, 2) X(kFwdCols, 9, 3, 3, 3)                    \
   X(kFwdCols, 10, 4, 3, 3) X(kFwdCols, 11, 4, 4, 3)                                          \
-  X(kFwdRows, 7, 4, 3, 0) X(kFwdRows, 8, 4, 4, 0) X(kFwdRows, 9, 3, 3, 3)                    \
+  X(kFwdRows, 7, 4, 3, 0) X(kFwdRows, 8, 3, 3, 2) X(kFwdRows, 9, 3, 3, 3)                    \
   X(kFwdRows, 10, 4, 3, 3) X(kFwdRows, 11, 4, 4, 3)                                          \
-  X(kInvRows, 7, 4, 3, 0) X(kInvRows, 8, 4, 4, 0) X(kInvRows, 9, 3, 3, 3)                    \
+  X(kInvRows, 7, 4, 3, 0) X(kInvRows, 8, 3, 3, 2) X(kInvRows, 9, 3, 3, 3)                    \
   X(kInvRows, 10, 4, 3, 3) X(kInvRows, 11, 4, 4, 3)                                          \
-  X(kInvCols, 7, 4, 3, 0) X(kInvCols, 8, 4, 4, 0) X(kInvCols, 9, 3, 3, 3)                    \
+  X(kInvCols, 7, 4, 3, 0) X(kInvCols, 8, 3, 3, 2) X(kInvCols, 9, 3, 3, 3)                    \
   X(kInvCols, 10, 4, 3, 3) X(kInvCols, 11, 4, 4, 3)
 
 // Each shape is a separate (not inlined) function, so the persistent kernel
@@ -201,6 +210,20 @@ bool batch_split_ok(int L, int s) {
   return s >= 7 && s <= 11 && L - s >= 6 && L - s <= 11;
 }
 
+// Algorithmic HBM bytes of one transform of 2^L in one direction: 16 bytes
+// per element and HBM round trip, one round trip up to 2^13 (the transform
+// fits one workgroup's LDS), two above.
+double ntt_alg_bytes(int L) { return 16.0 * (double)(1u << L) * (L <= 13 ? 1.0 : 2.0); }
+
+// Transforms of one length in the grouped mode: the per-length pass
+// launches with an offset table (transform b of the launch at word offs[b]).
+struct BatchGroup {
+  int L = 0, k = 0;
+  DevBuf offs;
+  std::unique_ptr<NttPlan> plan;
+  PassSpec sp[4];
+};
+
 }  // namespace
 }  // namespace pcgsq
 
@@ -209,6 +232,8 @@ using namespace pcgsq;
 struct pcgsq_ntt_batch {
   pcgsq_ctx* ctx = nullptr;
   int count = 0, ops = 3;
+  int mode = PCGSQ_NTT_BATCH_PERSISTENT;
+  std::vector<std::unique_ptr<BatchGroup>> groups;  // grouped mode, longest first
   std::vector<int> logN;
   std::vector<size_t> off;
   size_t total = 0;
@@ -231,9 +256,64 @@ struct pcgsq_ntt_batch {
     graph = nullptr;
   }
 
+  // Grouped mode: one offset table and plan per length; each plan takes the
+  // shape the context's per-length launches use (the NTT autotuner's).
+  void build_grouped() {
+    groups.clear();
+    std::map<int, std::vector<uint64_t>, std::greater<int>> by_len;
+    for (int x = 0; x < count; ++x) by_len[logN[(size_t)x]].push_back(off[(size_t)x]);
+    uint64_t* A = a.as<uint64_t>();
+    uint64_t* F = f.as<uint64_t>();
+    uint64_t* R = r.as<uint64_t>();
+    uint64_t* TF = tmp_f.as<uint64_t>();
+    uint64_t* TI = tmp_i.as<uint64_t>();
+    nitems = 0;
+    for (auto& kv : by_len) {
+      std::unique_ptr<BatchGroup> g(new BatchGroup);
+      g->L = kv.first;
+      g->k = (int)kv.second.size();
+      require(g->k <= 65535, "ntt batch: at most 65535 transforms of one length");
+      const NttPlan& tuned = ctx->ntt.plan(1 << g->L);
+      g->plan.reset(new NttPlan(1 << g->L, tuned.n1_log));
+      g->plan->variant = tuned.variant;
+      g->plan->cols1 = tuned.cols1;
+      g->plan->cols2 = tuned.cols2;
+      upload(g->offs, kv.second, "ntt batch offsets");
+      const NttPlan& pl = *g->plan;
+      const bool one = pl.single();
+      if (ops & 1) {
+        g->sp[0] = plan_pass(pl, kPhFwd1, A, nullptr, one ? F : TF, g->k);
+        g->sp[1] = plan_pass(pl, kPhFwd2, TF, nullptr, F, g->k);
+      }
+      if (ops & 2) {
+        const uint64_t* inv_in = (ops & 1) ? F : A;
+        g->sp[2] = plan_pass(pl, kPhInv1, inv_in, nullptr, one ? R : TI, g->k);
+        g->sp[3] = plan_pass(pl, kPhInv2, TI, nullptr, R, g->k);
+      }
+      for (PassSpec& sp : g->sp) {
+        if (sp.kind < 0) continue;
+        sp.d.boff = g->offs.as<uint64_t>();
+        if (one) sp.ncols = 1;  // one transform per launch row
+        ++nitems;
+      }
+      groups.push_back(std::move(g));
+    }
+    grid = 0;
+    lds = 0;
+    nctr = 0;
+  }
+
   // Tiles, work lists and launch shape from the current splits.
   void build() {
     drop_graph();
+    alg_bytes = 0;
+    for (int x = 0; x < count; ++x)
+      alg_bytes += ((ops & 1 ? 1 : 0) + (ops & 2 ? 1 : 0)) * ntt_alg_bytes(logN[(size_t)x]);
+    if (mode == PCGSQ_NTT_BATCH_GROUPED) {
+      build_grouped();
+      return;
+    }
+    groups.clear();
     int xccs = 1, cus = 0;
     hip_check(hipDeviceGetAttribute(&xccs, hipDeviceAttributeNumberOfXccs, ctx->device),
               "ntt batch XCD count");
@@ -265,7 +345,6 @@ struct pcgsq_ntt_batch {
 
     std::vector<BatchPhase> ph((size_t)count * 4);
     lds = 0;
-    alg_bytes = 0;
     uint64_t* A = a.as<uint64_t>();
     uint64_t* F = f.as<uint64_t>();
     uint64_t* R = r.as<uint64_t>();
@@ -313,8 +392,6 @@ struct pcgsq_ntt_batch {
         last = x * 4 + p;
         lds = std::max(lds, ntt_tile_lds_bytes(b.nlog, C, batch_groups(b.nlog)));
       }
-      const double passes = pl->single() ? 1.0 : 2.0;
-      alg_bytes += ((ops & 1 ? 1 : 0) + (ops & 2 ? 1 : 0)) * 16.0 * (double)(1 << L) * passes;
     }
     // one pass-major work list per XCD, longest transforms first within a pass
     std::vector<BatchItem> it;
@@ -355,6 +432,12 @@ struct pcgsq_ntt_batch {
   }
 
   void enqueue(hipStream_t st) {
+    if (mode == PCGSQ_NTT_BATCH_GROUPED) {
+      for (auto& g : groups)
+        for (const PassSpec& sp : g->sp)
+          if (sp.kind >= 0) ntt_launch_pass(st, sp, g->k, nullptr, "ntt.batch.pass");
+      return;
+    }
     hipLaunchKernelGGL(batch_reset_kernel, dim3((nctr + 255) / 256), dim3(256), 0, st,
                        ctr.as<uint32_t>(), nctr);
     hipLaunchKernelGGL(ntt_batch_kernel, dim3((unsigned)grid), dim3(kBatchThreads), lds, st,
@@ -365,6 +448,7 @@ struct pcgsq_ntt_batch {
 
   // After a run: every list drained and no wait ran out of spins.
   void check() {
+    if (mode == PCGSQ_NTT_BATCH_GROUPED) return;  // plain launches: nothing to drain
     std::vector<uint32_t> h((size_t)kCtrDone);
     hip_check(hipMemcpy(h.data(), ctr.ptr, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
               "ntt batch status");
@@ -436,6 +520,18 @@ int pcgsq_ntt_batch_tune(pcgsq_ntt_batch* b, int logN, int n1_log, int wgs_per_c
     b->wgs_per_cu = wgs_per_cu;
     b->spin_limit = spin_limit ? spin_limit : (1u << 22);
     hip_check(hipStreamSynchronize(b->ctx->stream), "ntt batch sync");
+    b->build();
+  });
+}
+
+int pcgsq_ntt_batch_set_mode(pcgsq_ntt_batch* b, int mode) {
+  return guarded([&] {
+    require(b, "ntt_batch_set_mode: null batch");
+    require(mode == PCGSQ_NTT_BATCH_PERSISTENT || mode == PCGSQ_NTT_BATCH_GROUPED,
+            "ntt_batch_set_mode: mode must be PCGSQ_NTT_BATCH_PERSISTENT or _GROUPED");
+    CtxGuard g(b->ctx);
+    hip_check(hipStreamSynchronize(b->ctx->stream), "ntt batch sync");
+    b->mode = mode;
     b->build();
   });
 }

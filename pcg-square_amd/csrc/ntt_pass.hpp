@@ -31,6 +31,8 @@ struct NttPassDesc {
   const uint64_t* in;
   const uint64_t* in2;       // *Prod kinds: the load multiplies in[] by in2[] (mod P)
   uint64_t* out;
+  const uint64_t* boff;      // word offset of each transform of the launch (in, in2 and out
+                             // alike); nullptr: back to back, transform b at b * 2^N_log
   const uint64_t* tab;       // stage table [n][2]
   const uint64_t* tw_lo;     // four-step twiddle psi^(+-e), two-level tables [.][2]
   const uint64_t* tw_hi;
@@ -280,11 +282,12 @@ struct Part {
   static __host__ __device__ constexpr int threads(int C) { return C << (NLOG - RMAX); }
 };
 
-// One tile (C columns starting at column tile `ct` of transform `b`).
+// One tile (C columns starting at column tile `ct` of the transform that
+// starts at word `base` of in, in2 and out).
 // Threads t >= Part::threads(C) idle but take part in the barriers.
 template <int K, int RA, int RB, int RC>
 __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __restrict__ lds,
-                                          size_t b, uint32_t ct) {
+                                          size_t base, uint32_t ct) {
   using P = Part<RA, RB, RC>;
   constexpr int NLOG = P::NLOG, RMAX = P::RMAX;
   const uint32_t C = 1u << d.c_log;
@@ -292,10 +295,10 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
   const uint32_t t = threadIdx.x;
   const bool active = t < T;
   const uint32_t c0 = ct << d.c_log;
-  gcptr inb = (gcptr)(d.in + (b << d.N_log) + ((size_t)c0 << d.in_cs_log));
-  gcptr in2b = KindTraits<K>::PROD ? (gcptr)(d.in2 + (b << d.N_log) + ((size_t)c0 << d.in_cs_log))
+  gcptr inb = (gcptr)(d.in + base + ((size_t)c0 << d.in_cs_log));
+  gcptr in2b = KindTraits<K>::PROD ? (gcptr)(d.in2 + base + ((size_t)c0 << d.in_cs_log))
                                    : nullptr;
-  gptr outb = (gptr)(d.out + (b << d.N_log) + ((size_t)c0 << d.out_cs_log));
+  gptr outb = (gptr)(d.out + base + ((size_t)c0 << d.out_cs_log));
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every
     // element before any bit-reversed store lands

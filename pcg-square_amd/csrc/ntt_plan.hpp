@@ -136,6 +136,14 @@ struct PassSpec {
   NttPassDesc d{};
 };
 
+struct Profiler;
+
+// Launch one pass over `batch` transforms (ntt.hip): the per-length pass
+// kernels, tiles sized by the spec's cols.  With ps.d.boff set, transform b of
+// the launch starts at word boff[b] of in, in2 and out.
+void ntt_launch_pass(hipStream_t st, const PassSpec& ps, int batch, Profiler* prof,
+                     const char* zone);
+
 // Descriptor of pass `phase` of plan `pl` reading `in` (and `in2`, the second
 // factor of a product, for phase 2) and writing `out`.  `batch` transforms
 // lie back to back (stride N).

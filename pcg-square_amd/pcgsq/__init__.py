@@ -116,6 +116,7 @@ def lib() -> C.CDLL:
     L.pcgsq_ntt_batch_create.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(vp)]
     L.pcgsq_ntt_batch_destroy.argtypes = [vp]
     L.pcgsq_ntt_batch_tune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint32]
+    L.pcgsq_ntt_batch_set_mode.argtypes = [vp, C.c_int]
     L.pcgsq_ntt_batch_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                       C.POINTER(C.c_size_t)]
     L.pcgsq_ntt_batch_load.argtypes = [vp, u64p]
@@ -520,12 +521,14 @@ class OlePlan:
 
 
 NTT_BATCH_FORWARD, NTT_BATCH_INVERSE = 1, 2
+NTT_BATCH_PERSISTENT, NTT_BATCH_GROUPED = 0, 1
 
 
 class NttBatch:
-    """Transforms of uneven lengths (2^10..2^22) in one persistent launch
-    (pcgsq_ntt_batch_*).  Transform i occupies words offsets[i] ..
-    offsets[i] + 2^logN[i] of the concatenated buffers."""
+    """Transforms of uneven lengths (2^10..2^22) in one persistent launch, or
+    grouped by length into per-length launches (pcgsq_ntt_batch_*).
+    Transform i occupies words offsets[i] .. offsets[i] + 2^logN[i] of the
+    concatenated buffers."""
 
     def __init__(self, ctx: Context, logN, ops: int = NTT_BATCH_FORWARD | NTT_BATCH_INVERSE):
         self.ctx = ctx
@@ -558,6 +561,11 @@ class NttBatch:
     def tune(self, logN: int = 0, n1_log: int = 0, wgs_per_cu: int = 0, spin_limit: int = 0):
         _check(lib().pcgsq_ntt_batch_tune(self.h, logN, n1_log, wgs_per_cu, spin_limit))
 
+    def set_mode(self, mode: int) -> None:
+        """NTT_BATCH_PERSISTENT (one persistent launch) or NTT_BATCH_GROUPED
+        (per-length pass launches over offset tables); results are equal."""
+        _check(lib().pcgsq_ntt_batch_set_mode(self.h, mode))
+
     def load(self, concat: np.ndarray) -> None:
         a = _u64(concat)
         assert a.size == self.total
@@ -589,8 +597,9 @@ class NttBatch:
 
     def autotune(self, reps: int = 3, log=None) -> dict:
         """Coordinate search over the split of every length in the batch and
-        the resident workgroups per CU, timing whole replays of the captured
-        launch on the loaded inputs; keeps the fastest shape.  Shapes never
+        the resident workgroups per CU of the persistent launch, then the
+        grouped-by-length mode, timing whole replays of the captured launch
+        on the loaded inputs; keeps the fastest.  Shapes and modes never
         change results."""
         def timed():
             self.run(graph=True)
@@ -603,6 +612,7 @@ class NttBatch:
             self.ctx.profile(False)
             return z["ntt.batch"][1] / z["ntt.batch"][0]
 
+        self.set_mode(NTT_BATCH_PERSISTENT)
         best = {"wgs_per_cu": 0, "splits": {}, "ms": timed()}
         for L in sorted(set(self.logN), reverse=True):
             cands = [s for s in range(7, 12) if 6 <= L - s <= 11] + ([L] if L <= 12 else [])
@@ -625,6 +635,15 @@ class NttBatch:
             if ms < best["ms"]:
                 best["ms"], best["wgs_per_cu"] = ms, w
         self.tune(0, 0, best["wgs_per_cu"])
+        best["mode"] = "persistent"
+        self.set_mode(NTT_BATCH_GROUPED)
+        ms = timed()
+        if log:
+            log(f"batch grouped by length: {ms:.3f} ms")
+        if ms < best["ms"]:
+            best["ms"], best["mode"] = ms, "grouped"
+        else:
+            self.set_mode(NTT_BATCH_PERSISTENT)
         return best
 
     def split(self, arr: np.ndarray):

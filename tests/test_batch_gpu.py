@@ -1,5 +1,5 @@
-"""Batched NTT of uneven lengths in one persistent launch (pcgsq_ntt_batch_*),
-through the C ABI.  Needs an MI355X.  Every result is compared bit for bit
+"""Batched NTT of uneven lengths in one persistent launch or grouped by
+length (pcgsq_ntt_batch_*), through the C ABI.  Needs an MI355X.  Every result is compared bit for bit
 with the per-length launches (the single-launch four-step / one-pass path)
 and, for the shorter lengths, with the CPU oracle."""
 import numpy as np
@@ -119,6 +119,58 @@ def test_ntt_batch_config3_workload(ctx, pq):
         b.run(graph=True)
         b.finish()
         _check_batch(ctx, b, a)
+
+
+def test_ntt_batch_grouped_mode(ctx, pq):
+    """Grouped by length (per-length pass launches over offset tables) gives
+    the same transforms as the persistent launch, for both directions, one
+    direction only, graph replays and switches between the modes."""
+    rng = np.random.default_rng(7)
+    logN = [int(x) for x in rng.choice([10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22], 40)]
+    with pq.NttBatch(ctx, logN) as b:
+        a = rng.integers(0, P, b.total, dtype=np.uint64)
+        b.load(a)
+        b.set_mode(pq.NTT_BATCH_GROUPED)
+        assert b.info()["grid"] == 0
+        for graph in (False, True, True):
+            b.run(graph=graph)
+            b.finish()
+            _check_batch(ctx, b, a)
+        b.set_mode(pq.NTT_BATCH_PERSISTENT)
+        b.run(graph=True)
+        b.finish()
+        _check_batch(ctx, b, a, oracle_max_log=0)
+    for ops, which in ((pq.NTT_BATCH_FORWARD, 0), (pq.NTT_BATCH_INVERSE, 1)):
+        with pq.NttBatch(ctx, logN[:12], ops=ops) as b:
+            a = rng.integers(0, P, b.total, dtype=np.uint64)
+            b.load(a)
+            b.set_mode(pq.NTT_BATCH_GROUPED)
+            b.run()
+            got = b.split(b.read()[which])
+            for i, x in enumerate(b.split(a)):
+                exp = (ctx.ntt_forward if which == 0 else ctx.ntt_inverse)(x[None, :])[0]
+                assert np.array_equal(got[i], exp), f"transform {i} (2^{b.logN[i]})"
+
+
+def test_ntt_batch_grouped_follows_context_shapes(pq):
+    """The grouped mode runs each length in the context's tuned shape (split,
+    grouping, tile widths): unusual shapes give the same transforms."""
+    rng = np.random.default_rng(8)
+    with pq.Context(0, apply_tuning=False) as c:
+        c.ntt_tune(1 << 16, 7, 1, 2, 8)
+        c.ntt_tune(1 << 13, 13, 0, 4, 0)
+        c.ntt_tune(1 << 20, 11, 2, 0, 4)
+        logN = [16, 13, 20, 16, 13, 10]
+        with pq.NttBatch(c, logN) as b:
+            a = rng.integers(0, P, b.total, dtype=np.uint64)
+            b.load(a)
+            b.set_mode(pq.NTT_BATCH_GROUPED)
+            b.run(graph=True)
+            b.finish()
+            f, r = b.read()
+            assert np.array_equal(r, a)
+            for i, x in enumerate(b.split(a)):
+                assert np.array_equal(b.split(f)[i], O.ntt_forward(x)), f"transform {i} vs oracle"
 
 
 def test_ntt_batch_spin_limit_reports_error(ctx, pq):
