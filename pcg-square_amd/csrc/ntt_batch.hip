@@ -218,9 +218,9 @@ double ntt_alg_bytes(int L) { return 16.0 * (double)(1u << L) * (L <= 13 ? 1.0 :
 // Transforms of one length in the grouped mode: the per-length pass
 // launches with an offset table (transform b of the launch at word offs[b]).
 struct BatchGroup {
-  int L = 0, k = 0;
+  int L = 0, k = 0;  // k <= 65535 transforms (one launch row each): a length with more
+                     // transforms gets several groups
   DevBuf offs;
-  std::unique_ptr<NttPlan> plan;
   PassSpec sp[4];
 };
 
@@ -234,6 +234,7 @@ struct pcgsq_ntt_batch {
   int count = 0, ops = 3;
   int mode = PCGSQ_NTT_BATCH_PERSISTENT;
   std::vector<std::unique_ptr<BatchGroup>> groups;  // grouped mode, longest first
+  std::map<int, std::unique_ptr<NttPlan>> gplans;   // grouped mode: L -> plan
   std::vector<int> logN;
   std::vector<size_t> off;
   size_t total = 0;
@@ -259,7 +260,9 @@ struct pcgsq_ntt_batch {
   // Grouped mode: one offset table and plan per length; each plan takes the
   // shape the context's per-length launches use (the NTT autotuner's).
   void build_grouped() {
+    constexpr size_t kMaxRows = 65535;  // grid.y of one launch
     groups.clear();
+    gplans.clear();
     std::map<int, std::vector<uint64_t>, std::greater<int>> by_len;
     for (int x = 0; x < count; ++x) by_len[logN[(size_t)x]].push_back(off[(size_t)x]);
     uint64_t* A = a.as<uint64_t>();
@@ -269,34 +272,40 @@ struct pcgsq_ntt_batch {
     uint64_t* TI = tmp_i.as<uint64_t>();
     nitems = 0;
     for (auto& kv : by_len) {
-      std::unique_ptr<BatchGroup> g(new BatchGroup);
-      g->L = kv.first;
-      g->k = (int)kv.second.size();
-      require(g->k <= 65535, "ntt batch: at most 65535 transforms of one length");
-      const NttPlan& tuned = ctx->ntt.plan(1 << g->L);
-      g->plan.reset(new NttPlan(1 << g->L, tuned.n1_log));
-      g->plan->variant = tuned.variant;
-      g->plan->cols1 = tuned.cols1;
-      g->plan->cols2 = tuned.cols2;
-      upload(g->offs, kv.second, "ntt batch offsets");
-      const NttPlan& pl = *g->plan;
+      const int L = kv.first;
+      const NttPlan& tuned = ctx->ntt.plan(1 << L);
+      std::unique_ptr<NttPlan>& pp = gplans[L];
+      pp.reset(new NttPlan(1 << L, tuned.n1_log));
+      pp->variant = tuned.variant;
+      pp->cols1 = tuned.cols1;
+      pp->cols2 = tuned.cols2;
+      const NttPlan& pl = *pp;
       const bool one = pl.single();
-      if (ops & 1) {
-        g->sp[0] = plan_pass(pl, kPhFwd1, A, nullptr, one ? F : TF, g->k);
-        g->sp[1] = plan_pass(pl, kPhFwd2, TF, nullptr, F, g->k);
+      for (size_t b0 = 0; b0 < kv.second.size(); b0 += kMaxRows) {
+        std::unique_ptr<BatchGroup> g(new BatchGroup);
+        g->L = L;
+        g->k = (int)std::min(kMaxRows, kv.second.size() - b0);
+        upload(g->offs,
+               std::vector<uint64_t>(kv.second.begin() + (ptrdiff_t)b0,
+                                     kv.second.begin() + (ptrdiff_t)(b0 + (size_t)g->k)),
+               "ntt batch offsets");
+        if (ops & 1) {
+          g->sp[0] = plan_pass(pl, kPhFwd1, A, nullptr, one ? F : TF, g->k);
+          g->sp[1] = plan_pass(pl, kPhFwd2, TF, nullptr, F, g->k);
+        }
+        if (ops & 2) {
+          const uint64_t* inv_in = (ops & 1) ? F : A;
+          g->sp[2] = plan_pass(pl, kPhInv1, inv_in, nullptr, one ? R : TI, g->k);
+          g->sp[3] = plan_pass(pl, kPhInv2, TI, nullptr, R, g->k);
+        }
+        for (PassSpec& sp : g->sp) {
+          if (sp.kind < 0) continue;
+          sp.d.boff = g->offs.as<uint64_t>();
+          if (one) sp.ncols = 1;  // one transform per launch row
+          ++nitems;
+        }
+        groups.push_back(std::move(g));
       }
-      if (ops & 2) {
-        const uint64_t* inv_in = (ops & 1) ? F : A;
-        g->sp[2] = plan_pass(pl, kPhInv1, inv_in, nullptr, one ? R : TI, g->k);
-        g->sp[3] = plan_pass(pl, kPhInv2, TI, nullptr, R, g->k);
-      }
-      for (PassSpec& sp : g->sp) {
-        if (sp.kind < 0) continue;
-        sp.d.boff = g->offs.as<uint64_t>();
-        if (one) sp.ncols = 1;  // one transform per launch row
-        ++nitems;
-      }
-      groups.push_back(std::move(g));
     }
     grid = 0;
     lds = 0;
@@ -314,6 +323,7 @@ struct pcgsq_ntt_batch {
       return;
     }
     groups.clear();
+    gplans.clear();
     int xccs = 1, cus = 0;
     hip_check(hipDeviceGetAttribute(&xccs, hipDeviceAttributeNumberOfXccs, ctx->device),
               "ntt batch XCD count");

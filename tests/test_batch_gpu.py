@@ -152,6 +152,31 @@ def test_ntt_batch_grouped_mode(ctx, pq):
                 assert np.array_equal(got[i], exp), f"transform {i} (2^{b.logN[i]})"
 
 
+def test_ntt_batch_grouped_more_rows_than_one_launch(ctx, pq):
+    """65536 transforms of one length (one more than a launch has rows) plus
+    one other: the grouped mode splits the length over two launches and
+    equals the persistent launch."""
+    rng = np.random.default_rng(9)
+    logN = [10] * 65536
+    logN[12345] = 12
+    with pq.NttBatch(ctx, logN) as b:
+        a = rng.integers(0, P, b.total, dtype=np.uint64)
+        b.load(a)
+        b.set_mode(pq.NTT_BATCH_GROUPED)
+        b.run(graph=True)
+        b.finish()
+        f, r = b.read()
+        assert np.array_equal(r, a)
+        b.set_mode(pq.NTT_BATCH_PERSISTENT)
+        b.run()
+        b.finish()
+        f2, _ = b.read()
+        assert np.array_equal(f, f2)
+        fs, as_ = b.split(f), b.split(a)
+        for i in (0, 12345, 65535):
+            assert np.array_equal(fs[i], O.ntt_forward(as_[i])), f"transform {i} vs oracle"
+
+
 def test_ntt_batch_grouped_follows_context_shapes(pq):
     """The grouped mode runs each length in the context's tuned shape (split,
     grouping, tile widths): unusual shapes give the same transforms."""
