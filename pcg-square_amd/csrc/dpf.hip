@@ -9,6 +9,12 @@
 // across the workgroup through LDS, and across workgroups with one 64-bit
 // atomic XOR per (workgroup, word): XOR is associative and commutative, so
 // the result is bit-exact whatever the order.
+//
+// The last tree level stores only its hashes, not its nodes: the leaf level
+// rebuilds each parent node from the grandparent's stored node and hashes and
+// the previous level's CW (XORs only, no PRG call).  That removes a third of
+// the widest level's HBM traffic (16 of 48 bytes per node), and with it the
+// power that held that level's clock down.
 #include <stdio.h>
 #include <string.h>
 
@@ -35,9 +41,11 @@ int level_iters(size_t S, int iters_max) {
 }
 
 struct LevelArgs {
-  const Blk* Fp;     // parents [ntrees][S]
+  const Blk* Fp;     // parents [ntrees][S] (unused when the parents are rebuilt)
   const Blk* Hp;     // parents' hashes [ntrees][S][nk]
-  Blk* Fc;           // children [ntrees][S*m]        (hash / store modes)
+  const Blk* Fg;     // grandparents [ntrees][S/m] and their hashes [ntrees][S/m][nk]
+  const Blk* Hg;     //   (leaf modes that rebuild the parents)
+  Blk* Fc;           // children [ntrees][S*m]        (hash mode: nullptr = not stored; store mode)
   Blk* Hc;           // children's hashes             (hash mode)
   uint64_t* ct;      // C|tau<<63 per leaf            (convert mode)
   uint64_t* part;    // partial sums [ntrees][nparts][2] (convert mode)
@@ -52,13 +60,14 @@ struct LevelArgs {
 
 enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
 
+// CW of tree T that builds the nodes of level `lev` + 1 from level `lev`.
 template <int W>
-__device__ __forceinline__ void load_cw(const LevelArgs& a, int T, Blk* CW) {
+__device__ __forceinline__ void load_cw(const LevelArgs& a, int T, int lev, Blk* CW) {
   constexpr int NK = (1 << W) - 1;
-  const Blk* mine = a.cw + ((size_t)(a.level - 1) * a.ntrees + T) * NK;
+  const Blk* mine = a.cw + ((size_t)lev * a.ntrees + T) * NK;
   int pt = T + a.peer_shift;  // T < npeer and peer_shift < npeer
   if (pt >= a.npeer) pt -= a.npeer;
-  const Blk* peer = a.peer + ((size_t)(a.level - 1) * a.npeer + pt) * NK;
+  const Blk* peer = a.peer + ((size_t)lev * a.npeer + pt) * NK;
   Blk fold{0, 0};
 #pragma unroll
   for (int k = 0; k < NK; ++k) {
@@ -68,7 +77,9 @@ __device__ __forceinline__ void load_cw(const LevelArgs& a, int T, Blk* CW) {
   CW[NK] = fold;  // CW_{m-1} = XOR_k CW_k
 }
 
-template <int W, int PRG, int MODE>
+// REBUILD (leaf modes): the parents' nodes were not stored; rebuild them
+// from the grandparents as the level before did.
+template <int W, int PRG, int MODE, bool REBUILD>
 __global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
   constexpr int M = 1 << W, NK = M - 1;
   __shared__ typename std::conditional<PRG == kPrgAes, AesLds, int>::type aes_lds_storage;
@@ -87,47 +98,92 @@ __global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
   for (int i = 0; i < 2 * NK; ++i) acc[i] = 0;
   uint64_t accC = 0, accT = 0;
 
-  for (int it = 0; it < a.iters; ++it) {
-    const size_t f = base + (size_t)it * kTpb + threadIdx.x;
-    if (f >= total) break;
-    const int T = (int)(f >> a.S_log);
-    Blk CW[NK + 1];
-    load_cw<W>(a, T, CW);
-    const Blk parent = a.Fp[f];
-    Blk h[NK];
-    Blk hx{0, 0};
-#pragma unroll
-    for (int k = 0; k < NK; ++k) {
-      h[k] = a.Hp[f * NK + k];
-      hx = bxor(hx, h[k]);
+  // When the workgroup's parents all lie in one tree (S a multiple of the
+  // chunk: every level that matters for time), the CWs are loaded once,
+  // with scalar loads, instead of per parent.
+  auto run = [&](auto one_tree) {
+    constexpr bool ONE_TREE = decltype(one_tree)::value;
+    Blk CWt[NK + 1], CWgt[NK + 1];
+    if constexpr (ONE_TREE) {
+      const int T0 = (int)(base >> a.S_log);
+      load_cw<W>(a, T0, a.level - 1, CWt);
+      if constexpr (REBUILD) load_cw<W>(a, T0, a.level - 2, CWgt);
     }
-    const bool t = (parent.lo & 1ULL) != 0;
+    for (int it = 0; it < a.iters; ++it) {
+      const size_t f = base + (size_t)it * kTpb + threadIdx.x;
+      if (f >= total) break;
+      const int T = (int)(f >> a.S_log);
+      Blk CW[NK + 1];
+      if constexpr (ONE_TREE) {
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-      Blk node = (c < NK) ? h[c < NK ? c : 0] : bxor(hx, parent);
-      if (t) node = bxor(node, CW[c]);
-      const size_t idx = f * M + c;
-      if constexpr (MODE == kModeHash) {
-        Blk hh[NK];
-        tree_hash<PRG, NK>(node, hh, lds);
-        a.Fc[idx] = node;
+        for (int k = 0; k <= NK; ++k) CW[k] = CWt[k];
+      } else {
+        load_cw<W>(a, T, a.level - 1, CW);
+      }
+      Blk parent;
+      if constexpr (REBUILD) {
+        // parent f is child c of grandparent f >> W (S is a multiple of m)
+        const size_t g = f >> W;
+        const int c = (int)(f & (M - 1));
+        const Blk gp = a.Fg[g];
+        Blk hg = {0, 0}, hgx = {0, 0};
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-          a.Hc[idx * NK + k] = hh[k];
-          acc[2 * k] ^= hh[k].lo;
-          acc[2 * k + 1] ^= hh[k].hi;
+          const Blk v = a.Hg[g * NK + k];
+          hgx = bxor(hgx, v);
+          if (k == c) hg = v;
         }
-      } else if constexpr (MODE == kModeStore) {
-        a.Fc[idx] = node;
+        Blk CWg[NK + 1];
+        if constexpr (ONE_TREE) {
+#pragma unroll
+          for (int k = 0; k <= NK; ++k) CWg[k] = CWgt[k];
+        } else {
+          load_cw<W>(a, T, a.level - 2, CWg);
+        }
+        parent = (c < NK) ? hg : bxor(hgx, gp);
+        if ((gp.lo & 1ULL) != 0) parent = bxor(parent, CWg[c]);
       } else {
-        const uint64_t C = freduce64(out_hash_lo64(node));
-        const uint64_t tau = node.lo & 1ULL;
-        a.ct[idx] = C | (tau << 63);
-        accC = fadd(accC, C);
-        accT += tau;
+        parent = a.Fp[f];
+      }
+      Blk h[NK];
+      Blk hx{0, 0};
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        h[k] = a.Hp[f * NK + k];
+        hx = bxor(hx, h[k]);
+      }
+      const bool t = (parent.lo & 1ULL) != 0;
+#pragma unroll
+      for (int c = 0; c < M; ++c) {
+        Blk node = (c < NK) ? h[c < NK ? c : 0] : bxor(hx, parent);
+        if (t) node = bxor(node, CW[c]);
+        const size_t idx = f * M + c;
+        if constexpr (MODE == kModeHash) {
+          Blk hh[NK];
+          tree_hash<PRG, NK>(node, hh, lds);
+          if (a.Fc) a.Fc[idx] = node;
+#pragma unroll
+          for (int k = 0; k < NK; ++k) {
+            a.Hc[idx * NK + k] = hh[k];
+            acc[2 * k] ^= hh[k].lo;
+            acc[2 * k + 1] ^= hh[k].hi;
+          }
+        } else if constexpr (MODE == kModeStore) {
+          a.Fc[idx] = node;
+        } else {
+          const uint64_t C = freduce64(out_hash_lo64(node));
+          const uint64_t tau = node.lo & 1ULL;
+          a.ct[idx] = C | (tau << 63);
+          accC = fadd(accC, C);
+          accT += tau;
+        }
       }
     }
-  }
+  };
+  if (a.S_log >= a.chunk_log)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
 
   int first = 0;
   if constexpr (MODE == kModeHash) {
@@ -209,7 +265,7 @@ void launch_root(hipStream_t st, const DpfRun& r, Blk* F0, Blk* H0, Blk* cw) {
   hip_check(hipGetLastError(), "launch dpf_root_kernel");
 }
 
-template <int W, int PRG, int MODE>
+template <int W, int PRG, int MODE, bool REBUILD = false>
 void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
   char zone[32];
   snprintf(zone, sizeof zone, MODE == kModeHash ? "dpf.level.%02d"
@@ -220,7 +276,8 @@ void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
   const size_t tiles = (total + kTpb - 1) / kTpb;
   const size_t grid = (tiles + a.iters - 1) / a.iters;
   require(grid < (1ULL << 31), "dpf: launch too large");
-  hipLaunchKernelGGL((dpf_level_kernel<W, PRG, MODE>), dim3((unsigned)grid), dim3(kTpb), 0, st, a);
+  hipLaunchKernelGGL((dpf_level_kernel<W, PRG, MODE, REBUILD>), dim3((unsigned)grid), dim3(kTpb), 0,
+                     st, a);
   hip_check(hipGetLastError(), "launch dpf_level_kernel");
 }
 
@@ -287,21 +344,37 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     a.level = i;
     a.iters = level_iters(S, r.iters_max);
     a.chunk_log = ilog2_exact((uint64_t)a.iters) + ilog2_exact((uint64_t)kTpb);
+    // the last tree level keeps only its hashes; the leaf level rebuilds
+    // those parents from the grandparents (still in the other buffer)
+    const bool rebuild = i == levels && levels >= 2;
+    if (rebuild) {
+      Blk *Fg, *Hg;
+      frontier(cur ^ 1, S >> W, Fg, Hg);
+      a.Fp = nullptr;
+      a.Fg = Fg;
+      a.Hg = Hg;
+    }
     if (i < levels) {
       Blk *Fn, *Hn;
       frontier(cur ^ 1, S * M, Fn, Hn);
-      a.Fc = Fn;
+      a.Fc = i == levels - 1 ? nullptr : Fn;
       a.Hc = Hn;
       launch_level<W, PRG, kModeHash>(st, a, r.prof);
       cur ^= 1;
     } else if (r.mode == kLeafStore) {
       a.Fc = r.leaves;
-      launch_level<W, PRG, kModeStore>(st, a, r.prof);
+      if (rebuild)
+        launch_level<W, PRG, kModeStore, true>(st, a, r.prof);
+      else
+        launch_level<W, PRG, kModeStore>(st, a, r.prof);
     } else {
       a.ct = r.ct;
       a.part = r.partials;
       a.nparts = dpf_leaf_nparts(n, W, r.iters_max);
-      launch_level<W, PRG, kModeConvert>(st, a, r.prof);
+      if (rebuild)
+        launch_level<W, PRG, kModeConvert, true>(st, a, r.prof);
+      else
+        launch_level<W, PRG, kModeConvert>(st, a, r.prof);
     }
   }
   if (r.cw_out)
