@@ -411,15 +411,19 @@ def main() -> int:
     served = bench_served(pcgsq) if "served" in parts else None
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
-    n_leaf, ms_leaf = zones["dpf.leaf_convert"]
+    # the leaf level: with the output layer's gather fused in at this shape
+    leaf_zone = "dpf.leaf_gather" if "dpf.leaf_gather" in zones else "dpf.leaf_convert"
+    n_leaf, ms_leaf = zones[leaf_zone]
     avg_leaf = ms_leaf / n_leaf
     ach = leaves * CHACHA8_OPS / (avg_leaf * 1e-3)
     peak_meas = ceiling * CHACHA8_OPS
     peak = max(VALU_NOMINAL, peak_meas)
     traffic_file = os.path.join(ROOT, "profiles", "pcg_ole_expand_N20_c4_t16_hbm_traffic.md")
     traffic = pmc_traffic(traffic_file)
-    roofline = {"bound": "valu", "kernel": "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + "
-                                           "out-hash)",
+    roofline = {"bound": "valu",
+                "kernel": ("leaf_gather_kernel<w=1> (leaf level + ChaCha8 out-hash + output-layer "
+                           "gather)" if leaf_zone == "dpf.leaf_gather" else
+                           "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + out-hash)"),
                 "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "Tops/s",
                 "frac": ach / peak, "traffic": traffic,
                 "peak_nominal": VALU_NOMINAL / 1e12, "peak_measured": peak_meas / 1e12,

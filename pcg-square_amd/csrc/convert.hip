@@ -134,6 +134,62 @@ __global__ __launch_bounds__(kTpb) void gather_g_kernel(GatherArgs a) {
   a.g[(size_t)(party - a.first_party) * a.g_party_stride + (size_t)bb * a.N + q] = acc;
 }
 
+// Two output coefficients per thread; the landing slots and their CWs are
+// uniform over the workgroup (its 512 coefficients share one bin), so the CW
+// loads are scalar.  The +CW and -CW terms are summed apart without
+// reduction, as 64-bit words with a carry count, and reduced once at the end.
+__global__ __launch_bounds__(kTpb) void gather_fix_kernel(GatherFixArgs a) {
+  const int bb = blockIdx.y, party = blockIdx.z;
+  const uint32_t qt = blockIdx.x * (2 * kTpb);
+  const uint32_t q0 = qt + 2 * threadIdx.x;
+  const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
+  const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
+  Blk* gq = reinterpret_cast<Blk*>(a.g + (size_t)party * a.g_party_stride + ((size_t)bb << a.N_log) +
+                                   q0);
+  using cptr = const __attribute__((address_space(4))) uint64_t*;
+  const cptr CW = (cptr)(uintptr_t)(a.CW + (size_t)bb * a.B);
+  struct Wide {  // hi * 2^64 + lo
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+    __device__ void add(uint64_t x) {
+      lo += x;
+      hi += lo < x;
+    }
+    // mod P, with 2^64 = 4 * (3 * 2^25 - 1) mod P (hi < 2^8)
+    __device__ uint64_t mod() const {
+      return freduce64(freduce64(lo) + (uint64_t)hi * (4 * ((3ULL << 25) - 1)));
+    }
+  };
+  Wide pos0, pos1, neg0, neg1;
+  Blk m{0, 0};
+  int widx = -1;
+  for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h, int, int inst) {
+    if ((slot >> 6) != widx) {
+      widx = slot >> 6;
+      m = *reinterpret_cast<const Blk*>(a.mask + (size_t)widx * word_stride + set_off + q0);
+    }
+    const uint64_t cw = CW[inst];
+    const int b = slot & 63;
+    const uint64_t x0 = ((m.lo >> b) & 1ULL) ? cw : 0;
+    const uint64_t x1 = ((m.hi >> b) & 1ULL) ? cw : 0;
+    if (h) {
+      neg0.add(x0);
+      neg1.add(x1);
+    } else {
+      pos0.add(x0);
+      pos1.add(x1);
+    }
+  });
+  const Blk g = *gq;
+  uint64_t r0 = fsub(fadd(g.lo, pos0.mod()), neg0.mod());
+  uint64_t r1 = fsub(fadd(g.hi, pos1.mod()), neg1.mod());
+  if (party == 1) {
+    r0 = fneg(r0);
+    r1 = fneg(r1);
+  }
+  *gq = Blk{r0, r1};
+}
+
 __global__ __launch_bounds__(kTpb) void leaf_sums_kernel(const Blk* __restrict__ leaves, int B,
                                                          int D_log, int iters, int chunk_log,
                                                          int nparts,
@@ -208,6 +264,17 @@ void launch_gather_g(hipStream_t st, const GatherArgs& a) {
   else
     hipLaunchKernelGGL(gather_g_kernel<false>, g3, dim3(kTpb), 0, st, a);
   hip_check(hipGetLastError(), "launch gather_g_kernel");
+}
+
+void launch_gather_fix(hipStream_t st, const GatherFixArgs& a) {
+  require(a.sets >= 1 && a.sets <= 65535 && a.B > 0 && a.t > 0 && a.mask_words >= 1 &&
+              a.bin_log >= 9 && a.N_log >= a.bin_log && 4 * a.t <= 64 * a.mask_words &&
+              4 * a.t <= 256,
+          "gather fix: bad shape");
+  ProfZone pz(a.prof, st, "out.gather_fix");
+  const dim3 g3((unsigned)(((size_t)1 << a.N_log) / (2 * kTpb)), (unsigned)a.sets, 2);
+  hipLaunchKernelGGL(gather_fix_kernel, g3, dim3(kTpb), 0, st, a);
+  hip_check(hipGetLastError(), "launch gather_fix_kernel");
 }
 
 void launch_leaf_sums(hipStream_t st, const Blk* leaves, int B, int D_log, uint64_t* partials,

@@ -18,6 +18,59 @@
 
 namespace pcgsq {
 
+// The leaves that land on output coefficient q (0 <= q < N) of one
+// (party, block) set, in the order the output layer sums them: for h = 0, 1
+// the raw position pos = q + h*N (h = 1 folds with a minus sign), for ds =
+// 0, 1 the diagonal sd = (pos >> bin_log) - ds and, along it, the instances
+// inst = kk*t + (sd - kk) < B, each at leaf d = (pos & (bin - 1)) + ds*bin
+// (pcg_ole_impl.h:432-446).  Calls f(slot, h, ds, inst) with slot = 0, 1, ...
+// and returns the number of calls (at most 4t).  Every q of an aligned run of
+// bin coefficients sees the same sequence, so a workgroup whose coefficients
+// share one bin enumerates it with uniform control flow.
+template <class F>
+__device__ __forceinline__ int for_each_landing(uint32_t q, int N_log, int bin_log, int t, int B,
+                                                F&& f) {
+  int slot = 0;
+  for (int h = 0; h < 2; ++h) {
+    const int sb = (int)(((uint64_t)q + ((uint64_t)h << N_log)) >> bin_log);
+    for (int ds = 0; ds < 2; ++ds) {
+      const int sd = sb - ds;
+      if (sd < 0 || sd > 2 * t - 2) continue;
+      const int kk_lo = sd - t + 1 > 0 ? sd - t + 1 : 0;
+      const int kk_hi = sd < t - 1 ? sd : t - 1;
+      for (int kk = kk_lo; kk <= kk_hi; ++kk) {
+        const int inst = kk * t + (sd - kk);
+        if (inst >= B) break;
+        f(slot++, h, ds, inst);
+      }
+    }
+  }
+  return slot;
+}
+
+// The leaf level with the gather fused in gives each workgroup at least
+// kGatherTile output coefficients of one bin (two per thread; more when the
+// bins are larger), and handles at most kGatherMaxSlots landing slots
+// (t <= 64).
+constexpr int kGatherTileLog = 9;
+constexpr int kGatherTile = 1 << kGatherTileLog;
+constexpr int kGatherMaxSlots = 256;
+
+// Second half of the fused output layer: g = (-1)^party (A + sum over the
+// landing slots whose tau bit is set of +-CW[inst]), in place over A.
+// A/g: g + p*g_party_stride + bb*N; masks [mask_words][2][sets][N]; CW
+// [sets][B] for the set's instances.  Needs bin >= 256.
+struct GatherFixArgs {
+  uint64_t* g = nullptr;
+  size_t g_party_stride = 0;
+  const uint64_t* mask = nullptr;
+  int mask_words = 0;
+  const uint64_t* CW = nullptr;
+  int sets = 0, B = 0, t = 0, N_log = 0, bin_log = 0;
+  Profiler* prof = nullptr;
+};
+void launch_gather_fix(hipStream_t st, const GatherFixArgs& a);
+
 // partials [ntrees][nparts][2] -> sumC[ntrees] (mod P), sumT[ntrees].
 void launch_fold_sums(hipStream_t st, const uint64_t* partials, int ntrees, int nparts,
                       uint64_t* sumC, uint64_t* sumT);

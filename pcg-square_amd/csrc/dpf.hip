@@ -22,6 +22,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "convert.hpp"
 #include "dpf.hpp"
 #include "prg.hpp"
 
@@ -30,6 +31,10 @@ namespace pcgsq {
 namespace {
 
 constexpr int kTpb = 256;
+constexpr int kTpbLog = 8;
+static_assert(kTpb == 1 << kTpbLog, "threads per workgroup");
+
+constexpr int ilog2_constexpr(int x) { return x <= 1 ? 0 : 1 + ilog2_constexpr(x >> 1); }
 
 // Tiles of kTpb parents per workgroup: a power of two (so a parent's partial
 // slot is a shift, not a division), at most iters_max, at most S / kTpb.
@@ -56,6 +61,11 @@ struct LevelArgs {
   int peer_shift, npeer;
   int ntrees, S_log, level, iters, nparts;
   int chunk_log;     // log2(iters * threads): parents per workgroup (a power of two)
+  // leaf level with the output layer's gather fused in (see leaf_gather_kernel)
+  uint64_t* gA;
+  size_t gA_party_stride;
+  uint64_t* gmask;
+  int mask_words, t, B, N_log, bin_log, sets;
 };
 
 enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
@@ -217,7 +227,174 @@ __global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
   }
 }
 
-// Level 0: the frontier is the root.  Also moves DltSft from the reference
+// CW of tree T (the same on every lane) that builds the nodes of level
+// `lev` + 1, read through the scalar cache: earlier launches wrote the
+// shares, and nothing in the reading kernel writes them.
+template <int W>
+__device__ __forceinline__ void load_cw_uniform(const LevelArgs& a, int T, int lev, Blk* CW) {
+  constexpr int NK = (1 << W) - 1;
+  using cptr = const __attribute__((address_space(4))) uint64_t*;
+  T = __builtin_amdgcn_readfirstlane(T);
+  int pt = T + a.peer_shift;  // T < npeer and peer_shift < npeer
+  if (pt >= a.npeer) pt -= a.npeer;
+  const cptr mine = (cptr)(uintptr_t)(a.cw + ((size_t)lev * a.ntrees + T) * NK);
+  const cptr peer = (cptr)(uintptr_t)(a.peer + ((size_t)lev * a.npeer + pt) * NK);
+  Blk fold{0, 0};
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    CW[k] = Blk{mine[2 * k] ^ peer[2 * k], mine[2 * k + 1] ^ peer[2 * k + 1]};
+    fold = bxor(fold, CW[k]);
+  }
+  CW[NK] = fold;
+}
+
+// Leaf level with the output layer's gather fused in.  A workgroup owns
+// 256 * LPT consecutive output coefficients q of one (party, block) set, LPT
+// per thread, all in one bin, and walks the leaves that land on them
+// (for_each_landing: the same slots, in the same order, for every q of the
+// tile).  For each landing slot it rebuilds the thread's parents from the
+// grandparents (the last tree level stored only hashes), forms the thread's
+// LPT leaves (consecutive, so the child indices are compile-time when
+// LPT >= m), out-hashes them and adds +-C to the coefficients' sums.  tau
+// goes into one bit per slot, because tau*CW needs the CW, and the CW needs
+// every leaf's sums (the Beaver step).  A slot's leaves are 256 * LPT
+// consecutive leaves of one tree, so their sums reduce over the workgroup
+// into one partial per (tree, run of leaves).  This replaces a 2^30-word
+// C|tau array written by the leaf level and read back by a separate gather.
+template <int W, int LPT>
+__global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
+  constexpr int M = 1 << W, NK = M - 1;
+  constexpr int TILE = kTpb * LPT;
+  constexpr int CPP = LPT >= M ? M : LPT;  // the thread's leaves per parent
+  constexpr int PPT = LPT / CPP;           // the thread's parents per slot
+  __shared__ uint64_t red[kGatherMaxSlots][4][2];
+  __shared__ uint64_t red_at[kGatherMaxSlots];
+  const int bb = blockIdx.y, party = blockIdx.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t qt = blockIdx.x * TILE;
+  const uint32_t q0 = qt + LPT * threadIdx.x;
+  const size_t bin_mask = ((size_t)1 << a.bin_log) - 1;
+  const size_t rr0 = q0 & bin_mask;  // a multiple of LPT
+  const size_t S = (size_t)1 << a.S_log;
+  const int nparts_log = a.S_log + W - (kTpbLog + ilog2_constexpr(LPT));
+  const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
+  const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
+  uint64_t A[LPT], mw[LPT];
+#pragma unroll
+  for (int e = 0; e < LPT; ++e) A[e] = mw[e] = 0;
+  int widx = 0;
+  auto store_words = [&](int wi) {
+    Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
+#pragma unroll
+    for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{mw[e], mw[e + 1]};
+  };
+  const int nslots = for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h,
+                                                                             int ds, int inst) {
+    if ((slot >> 6) != widx) {
+      store_words(widx);
+      widx = slot >> 6;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) mw[e] = 0;
+    }
+    const int T = (party * a.sets + bb) * a.B + inst;
+    Blk CW[NK + 1], CWg[NK + 1];
+    load_cw_uniform<W>(a, T, a.level - 1, CW);
+    load_cw_uniform<W>(a, T, a.level - 2, CWg);
+    const size_t d0 = rr0 + ((size_t)ds << a.bin_log);  // a multiple of LPT
+    uint64_t sC = 0;
+    uint32_t sT = 0;
+#pragma unroll
+    for (int pp = 0; pp < PPT; ++pp) {
+      const size_t dp = d0 + (size_t)pp * CPP;
+      const size_t pi = (size_t)T * S + (dp >> W);  // parent (global index)
+      const size_t gi = pi >> W;                    // its grandparent (S is a multiple of m)
+      const int cp = (int)((dp >> W) & (M - 1));
+      const Blk gp = a.Fg[gi];
+      Blk hg[NK];
+      Blk parent = gp, cwg = CWg[NK];  // child m-1: (xor of the hashes) ^ grandparent
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        hg[k] = a.Hg[gi * NK + k];
+        parent = bxor(parent, hg[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+        if (k == cp) {
+          parent = hg[k];
+          cwg = CWg[k];
+        }
+      if ((gp.lo & 1ULL) != 0) parent = bxor(parent, cwg);
+      Blk hp[NK];
+      Blk hx{0, 0};
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        hp[k] = a.Hp[pi * NK + k];
+        hx = bxor(hx, hp[k]);
+      }
+      const bool tp = (parent.lo & 1ULL) != 0;
+#pragma unroll
+      for (int e = 0; e < CPP; ++e) {
+        Blk leaf, cwc;
+        if constexpr (LPT >= M) {  // the leaves are children 0..m-1
+          leaf = e < NK ? hp[e < NK ? e : 0] : bxor(hx, parent);
+          cwc = CW[e];
+        } else {  // LPT consecutive children from a run boundary inside the parent
+          const int c = (int)(dp & (M - 1)) + e;
+          leaf = bxor(hx, parent);
+          cwc = CW[NK];
+#pragma unroll
+          for (int k = 0; k < NK; ++k)
+            if (k == c) {
+              leaf = hp[k];
+              cwc = CW[k];
+            }
+        }
+        if (tp) leaf = bxor(leaf, cwc);
+        const uint64_t C = freduce64(out_hash_lo64(leaf));
+        const uint32_t tau = (uint32_t)(leaf.lo & 1ULL);
+        const int q = pp * CPP + e;
+        A[q] = fadd(A[q], h ? kP - C : C);  // kP - 0 = kP adds nothing mod P
+        mw[q] |= (uint64_t)tau << (slot & 63);
+        sC = fadd(sC, C);
+        sT += tau;
+      }
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      sC = fadd(sC, shfl_xor_u64(sC, m));
+      sT += (uint32_t)__shfl_xor((int)sT, m, 64);
+    }
+    if (lane == 0) {
+      red[slot][wave][0] = sC;
+      red[slot][wave][1] = sT;
+    }
+    if (threadIdx.x == 0)
+      red_at[slot] = ((uint64_t)T << nparts_log) +
+                     (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + ilog2_constexpr(LPT)));
+  });
+  store_words(widx);
+#pragma unroll
+  for (int e = 0; e < LPT; ++e) mw[e] = 0;
+  for (int wi = widx + 1; wi < a.mask_words; ++wi) store_words(wi);
+  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
+                                     ((size_t)bb << a.N_log) + q0);
+#pragma unroll
+  for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+  __syncthreads();
+  for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
+    uint64_t c = red[sl][0][0], tt = red[sl][0][1];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      c = fadd(c, red[sl][w][0]);
+      tt += red[sl][w][1];
+    }
+    uint64_t* dst = a.part + red_at[sl] * 2;
+    dst[0] = c;
+    dst[1] = tt;
+  }
+}
+
+// Level 0: the frontier is the root.// Level 0: the frontier is the root.  Also moves DltSft from the reference
 // layout [T][levels][nk] into the level-major share array [levels][T][nk]
 // and seeds level 0 with H(root) (dpf.cpp:502, :522-532).
 template <int W, int PRG>
@@ -279,6 +456,23 @@ void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
   hipLaunchKernelGGL((dpf_level_kernel<W, PRG, MODE, REBUILD>), dim3((unsigned)grid), dim3(kTpb), 0,
                      st, a);
   hip_check(hipGetLastError(), "launch dpf_level_kernel");
+}
+
+template <int W, int LPT>
+void launch_leaf_gather_t(hipStream_t st, const LevelArgs& a) {
+  const dim3 grid((unsigned)(((size_t)1 << a.N_log) / ((size_t)kTpb * LPT)), (unsigned)a.sets, 2);
+  hipLaunchKernelGGL((leaf_gather_kernel<W, LPT>), grid, dim3(kTpb), 0, st, a);
+  hip_check(hipGetLastError(), "launch leaf_gather_kernel");
+}
+
+template <int W>
+void launch_leaf_gather(hipStream_t st, const LevelArgs& a, Profiler* prof) {
+  ProfZone pz(prof, st, "dpf.leaf_gather");
+  switch (dpf_gather_lpt(a.bin_log + 1)) {
+    case 8: return launch_leaf_gather_t<W, 8>(st, a);
+    case 4: return launch_leaf_gather_t<W, 4>(st, a);
+    default: return launch_leaf_gather_t<W, 2>(st, a);
+  }
 }
 
 template <int W, int PRG>
@@ -367,6 +561,18 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
         launch_level<W, PRG, kModeStore, true>(st, a, r.prof);
       else
         launch_level<W, PRG, kModeStore>(st, a, r.prof);
+    } else if (r.mode == kLeafGather) {
+      a.part = r.partials;
+      a.gA = r.g_acc;
+      a.gA_party_stride = r.g_party_stride;
+      a.gmask = r.g_mask;
+      a.mask_words = r.mask_words;
+      a.t = r.t;
+      a.B = r.B;
+      a.N_log = r.N_log;
+      a.bin_log = n - 1;
+      a.sets = r.sets;
+      launch_leaf_gather<W>(st, a, r.prof);
     } else {
       a.ct = r.ct;
       a.part = r.partials;
@@ -384,6 +590,15 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
 }
 
 }  // namespace
+
+int dpf_gather_lpt(int n) {
+  const size_t bin = (size_t)1 << (n - 1);
+  return bin >= (size_t)kTpb * 8 ? 8 : bin >= (size_t)kTpb * 4 ? 4 : 2;
+}
+
+int dpf_gather_nparts(int n) {
+  return (int)(((size_t)1 << n) / ((size_t)kTpb * dpf_gather_lpt(n)));
+}
 
 int dpf_leaf_nparts(int n, int w, int iters_max) {
   const int levels = n / w;
@@ -416,6 +631,13 @@ void dpf_expand(hipStream_t st, const DpfRun& r, DpfScratch& s) {
   require(!r.roots1 || (r.dltsft1 && r.ntrees % 2 == 0), "dpf: second half needs roots1 and dltsft1");
   require(r.mode != kLeafStore || r.leaves, "dpf: leaves output missing");
   require(r.mode != kLeafConvert || (r.ct && r.partials), "dpf: convert outputs missing");
+  require(r.mode != kLeafGather ||
+              (r.g_acc && r.g_mask && r.partials && r.n / r.w >= 2 &&
+               ((size_t)1 << (r.n - 1)) >= (size_t)kGatherTile && r.t >= 1 && r.t <= 64 &&
+               r.B >= 1 && r.B <= r.t * r.t && 4 * r.t <= 64 * r.mask_words &&
+               r.sets >= 1 && r.sets <= 65535 && r.ntrees == 2 * r.sets * r.B &&
+               r.N_log >= r.n - 1 && r.N_log <= 30 && !r.host_ex),
+          "dpf: fused gather needs bin >= 512, t <= 64, two levels and [party][sets][B] trees");
 #define PCGSQ_DISPATCH(W)                                                        \
   if (r.w == W) {                                                                \
     if (r.prg == kPrgChaCha8) expand_t<W, kPrgChaCha8>(st, r, s);                \

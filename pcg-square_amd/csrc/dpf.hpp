@@ -22,6 +22,8 @@ namespace pcgsq {
 enum LeafMode : int {
   kLeafStore = 0,    // write the 2^n leaves of every tree
   kLeafConvert = 1,  // fuse the ChaCha8 out-hash: write C|tau<<63 and partial sums
+  kLeafGather = 2,   // fuse the out-hash and the output layer's gather: per output
+                     // coefficient, sum +-C and one tau bit per landing leaf
 };
 
 // Host exchange hook: fill peer[0..count) from mine[0..count) for `level`.
@@ -49,7 +51,17 @@ struct DpfRun {
   LeafMode mode = kLeafConvert;
   Blk* leaves = nullptr;          // kLeafStore: [ntrees][2^n]
   uint64_t* ct = nullptr;         // kLeafConvert: [ntrees][2^n]
-  uint64_t* partials = nullptr;   // kLeafConvert: [ntrees][nparts][2]
+  uint64_t* partials = nullptr;   // kLeafConvert: [ntrees][nparts][2]; kLeafGather: nparts =
+                                  // 2^n / kGatherTile
+  // kLeafGather: the trees are [party][sets][B] (sets blocks of B instances
+  // per party, t^2 >= B); block bb of party p sums into g_acc + p *
+  // g_party_stride + bb * N (N = 2^N_log) and writes its tau bits to
+  // g_mask [mask_words][2][sets][N].  Needs 2^(n-1) >= kGatherTile, t <= 64
+  // and at least two levels.
+  uint64_t* g_acc = nullptr;
+  size_t g_party_stride = 0;
+  uint64_t* g_mask = nullptr;
+  int mask_words = 0, t = 0, B = 0, N_log = 0, sets = 0;
   Blk* cw_out = nullptr;          // optional [levels][ntrees][nk] copy of the shares
   int iters_max = 8;              // tiles of 256 parents per workgroup
   Profiler* prof = nullptr;       // optional per-launch timing
@@ -65,6 +77,11 @@ struct DpfScratch {
 
 // Number of partial-sum slots per tree written by kLeafConvert.
 int dpf_leaf_nparts(int n, int w, int iters_max);
+// ... and by kLeafGather, whose workgroups own 256 * dpf_gather_lpt(n)
+// output coefficients (2, 4 or 8 per thread, as many as one bin of 2^(n-1)
+// coefficients holds).
+int dpf_gather_lpt(int n);
+int dpf_gather_nparts(int n);
 
 // Allocates everything dpf_expand needs for `ntrees` trees of domain 2^n and
 // arity 2^w, so that a later dpf_expand of that shape allocates nothing.

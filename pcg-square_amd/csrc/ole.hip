@@ -355,18 +355,32 @@ struct pcgsq_ole_plan {
   // public NTT(a_i)
   DevBuf ahat;
   // working set
-  DevBuf ct, part, cw, fw, out;
+  DevBuf ct, part, cw, fw, out;   // ct: C|tau per leaf, or the fused gather's tau masks
   DevBuf tri;   // triple generation: first OLE's outputs [4][N] + shares [6][N]
 };
 
 namespace {
 
+// The output layer's gather runs fused into the DPF leaf level when every
+// workgroup tile of output coefficients lies in one bin and the landing
+// slots fit its LDS (dpf.hip, leaf_gather_kernel); otherwise the leaf level
+// writes C|tau per leaf and a separate gather reads them.
+bool fused_gather(const OleDims& d) {
+  return d.bin >= kGatherTile && d.t <= 64 && d.levels >= 2;
+}
+int gather_mask_words(const OleDims& d) { return (4 * d.t + 63) / 64; }
+int leaf_nparts(const OleDims& d) {
+  return fused_gather(d) ? dpf_gather_nparts(d.D_log) : dpf_leaf_nparts(d.D_log, d.w, 8);
+}
+
 size_t plan_group_bytes(const OleDims& d, int gb) {
   const size_t trees = (size_t)2 * gb * d.B;
   const size_t maxS = (size_t)1 << (d.w * (d.levels - 1));
   const size_t fh = trees * maxS * (1 + d.nk) * sizeof(Blk) * 2;
-  const size_t ct = trees * (size_t)d.D * sizeof(uint64_t);
-  return fh + ct;
+  const size_t leaf = fused_gather(d)
+                          ? (size_t)gather_mask_words(d) * 2 * gb * d.N * sizeof(uint64_t)
+                          : trees * (size_t)d.D * sizeof(uint64_t);
+  return fh + leaf;
 }
 
 // DPF blocks per group: as many as the budget holds, dividing c^2 (equal
@@ -385,7 +399,7 @@ size_t plan_estimate(const OleDims& d, size_t budget) {
   const int gb = plan_group(d, budget);
   const size_t gtrees = (size_t)2 * gb * d.B;
   const size_t trees_all = (size_t)2 * d.ninst;
-  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  const int nparts = leaf_nparts(d);
   const size_t nfw = (size_t)2 * d.nblk + 2 * d.c;
   size_t b = plan_group_bytes(d, gb);
   b += (size_t)d.levels * gtrees * d.nk * sizeof(Blk);            // dpf cw shares
@@ -411,8 +425,11 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   pl->noise.ensure((size_t)2 * 2 * d.c * d.t * sizeof(uint64_t), "ole noise");
   pl->ahat.ensure((size_t)d.c * d.N * sizeof(uint64_t), "ole ahat");
   const size_t gtrees = (size_t)2 * gb * d.B;
-  pl->ct.ensure(gtrees * d.D * sizeof(uint64_t), "ole ct");
-  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  if (fused_gather(d))
+    pl->ct.ensure((size_t)gather_mask_words(d) * 2 * gb * d.N * sizeof(uint64_t), "ole tau masks");
+  else
+    pl->ct.ensure(gtrees * d.D * sizeof(uint64_t), "ole ct");
+  const int nparts = leaf_nparts(d);
   pl->part.ensure(gtrees * nparts * 2 * sizeof(uint64_t), "ole partials");
   pl->cw.ensure((size_t)d.ninst * sizeof(uint64_t), "ole cw");
   pl->fw.ensure(((size_t)2 * d.nblk + 2 * d.c) * d.N * sizeof(uint64_t), "ole ntt buffers");
@@ -465,7 +482,8 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
   hipStream_t st = pl->ctx->stream;
   const int gb = pl->group_blocks;
   const size_t ni = (size_t)d.ninst;
-  const int nparts = dpf_leaf_nparts(d.D_log, d.w, 8);
+  const int nparts = leaf_nparts(d);
+  const bool fused = fused_gather(d);
   uint64_t* G = pl->fw.as<uint64_t>();                                  // [2][nblk][N]
   uint64_t* E = G + (size_t)2 * d.nblk * d.N;                          // [2][c][N]
   const uint64_t* BT = pl->bt.as<uint64_t>();
@@ -482,15 +500,44 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
     r.dltsft1 = pl->dlt.as<Blk>() + (ni + i0) * d.levels * d.nk;
     r.aes_t0 = pl->aes_t0.as<uint32_t>();
     r.aes_rks = pl->aes_rks.as<uint8_t>();
-    r.mode = kLeafConvert;
-    r.ct = pl->ct.as<uint64_t>();
     r.partials = pl->part.as<uint64_t>();
     r.prof = &pl->ctx->prof;
+    if (fused) {
+      // sums of +-C straight into g; tau bits into the mask words
+      r.mode = kLeafGather;
+      r.g_acc = G + (size_t)b0 * d.N;
+      r.g_party_stride = (size_t)d.nblk * d.N;
+      r.g_mask = pl->ct.as<uint64_t>();
+      r.mask_words = gather_mask_words(d);
+      r.t = d.t;
+      r.B = d.B;
+      r.N_log = d.L;
+      r.sets = gb;
+    } else {
+      r.mode = kLeafConvert;
+      r.ct = pl->ct.as<uint64_t>();
+    }
     dpf_expand(st, r, pl->dpf);
     {
       ProfZone pzb(&pl->ctx->prof, st, "out.beaver");
       launch_beaver_cw(st, pl->part.as<uint64_t>(), gb * d.B, nparts, BT + i0, BT + 2 * ni + i0,
                        BT + 4 * ni + i0, BT + 6 * ni + i0, ni, pl->cw.as<uint64_t>() + i0);
+    }
+    if (fused) {
+      GatherFixArgs fa;
+      fa.g = G + (size_t)b0 * d.N;
+      fa.g_party_stride = (size_t)d.nblk * d.N;
+      fa.mask = pl->ct.as<uint64_t>();
+      fa.mask_words = gather_mask_words(d);
+      fa.CW = pl->cw.as<uint64_t>() + i0;
+      fa.sets = gb;
+      fa.B = d.B;
+      fa.t = d.t;
+      fa.N_log = d.L;
+      fa.bin_log = d.D_log - 1;
+      fa.prof = &pl->ctx->prof;
+      launch_gather_fix(st, fa);
+      continue;
     }
     GatherArgs ga;
     ga.ct = pl->ct.as<uint64_t>();

@@ -227,10 +227,17 @@ def test_aes_ecb_fips_and_random(ctx, pq, golden):
 
 
 @pytest.mark.parametrize("N,c,t,w,prg", [
+    # bins of N/t < 512: the leaf level writes C|tau and a gather sums them
     (1 << 12, 2, 16, 1, O.PRG_CHACHA8),     # the reference's small CPU config
     (1 << 10, 2, 4, 1, O.PRG_AES),
     (1 << 11, 3, 8, 1, O.PRG_CHACHA8),
-    (1 << 12, 2, 8, 2, O.PRG_CHACHA8)])
+    # bins of >= 512: the gather runs fused into the leaf level
+    (1 << 12, 2, 8, 2, O.PRG_CHACHA8),
+    (1 << 13, 2, 16, 1, O.PRG_AES),
+    (1 << 14, 2, 8, 3, O.PRG_CHACHA8),
+    (1 << 14, 1, 8, 4, O.PRG_AES),
+    (1 << 15, 2, 32, 1, O.PRG_CHACHA8),     # 128 landing slots: two tau words
+    (1 << 15, 1, 64, 1, O.PRG_CHACHA8)])    # 256 landing slots: four tau words
 def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
     seed = 11
     with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=3) as pl:
