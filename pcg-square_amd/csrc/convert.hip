@@ -134,11 +134,33 @@ __global__ __launch_bounds__(kTpb) void gather_g_kernel(GatherArgs a) {
   a.g[(size_t)(party - a.first_party) * a.g_party_stride + (size_t)bb * a.N + q] = acc;
 }
 
-// Two output coefficients per thread; the landing slots and their CWs are
-// uniform over the workgroup (its 512 coefficients share one bin), so the CW
-// loads are scalar.  The +CW and -CW terms are summed apart without
-// reduction, as 64-bit words with a carry count, and reduced once at the end.
+// hi * 2^64 + lo: sums of field elements without reduction (hi < 2^24).
+struct Wide {
+  uint64_t lo = 0;
+  uint32_t hi = 0;
+  __device__ __forceinline__ void add(uint64_t x) {
+    lo += x;
+    hi += lo < x;
+  }
+  // mod P, with 2^64 = 4 * (3 * 2^25 - 1) mod P
+  __device__ __forceinline__ uint64_t mod() const {
+    return freduce64(freduce64(lo) + (uint64_t)hi * (4 * ((3ULL << 25) - 1)));
+  }
+};
+
+// Two output coefficients per thread; the landing slots are the same for the
+// workgroup's 512 coefficients (one bin).  Thread i finds slot i in closed
+// form and loads its signed CW; then, per 64-slot tau word, the workgroup
+// tabulates for each nibble of the word the sum of the signed CWs of every
+// subset of its 4 slots (16 tables of 16 entries in LDS), so a coefficient
+// adds 16 table entries per word instead of testing 64 bits.  The
+// coefficients' own loads are issued first and land meanwhile.
+constexpr int kFixMaxWords = kGatherMaxSlots / 64;
+
 __global__ __launch_bounds__(kTpb) void gather_fix_kernel(GatherFixArgs a) {
+  static_assert(kGatherMaxSlots <= kTpb, "one thread per landing slot");
+  __shared__ uint64_t sc[kGatherMaxSlots];   // signed CW per slot, 0 past the last
+  __shared__ uint64_t tab[kFixMaxWords][16][16];
   const int bb = blockIdx.y, party = blockIdx.z;
   const uint32_t qt = blockIdx.x * (2 * kTpb);
   const uint32_t q0 = qt + 2 * threadIdx.x;
@@ -146,43 +168,46 @@ __global__ __launch_bounds__(kTpb) void gather_fix_kernel(GatherFixArgs a) {
   const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
   Blk* gq = reinterpret_cast<Blk*>(a.g + (size_t)party * a.g_party_stride + ((size_t)bb << a.N_log) +
                                    q0);
-  using cptr = const __attribute__((address_space(4))) uint64_t*;
-  const cptr CW = (cptr)(uintptr_t)(a.CW + (size_t)bb * a.B);
-  struct Wide {  // hi * 2^64 + lo
-    uint64_t lo = 0;
-    uint32_t hi = 0;
-    __device__ void add(uint64_t x) {
-      lo += x;
-      hi += lo < x;
-    }
-    // mod P, with 2^64 = 4 * (3 * 2^25 - 1) mod P (hi < 2^8)
-    __device__ uint64_t mod() const {
-      return freduce64(freduce64(lo) + (uint64_t)hi * (4 * ((3ULL << 25) - 1)));
-    }
-  };
-  Wide pos0, pos1, neg0, neg1;
-  Blk m{0, 0};
-  int widx = -1;
-  for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h, int, int inst) {
-    if ((slot >> 6) != widx) {
-      widx = slot >> 6;
-      m = *reinterpret_cast<const Blk*>(a.mask + (size_t)widx * word_stride + set_off + q0);
-    }
-    const uint64_t cw = CW[inst];
-    const int b = slot & 63;
-    const uint64_t x0 = ((m.lo >> b) & 1ULL) ? cw : 0;
-    const uint64_t x1 = ((m.hi >> b) & 1ULL) ? cw : 0;
-    if (h) {
-      neg0.add(x0);
-      neg1.add(x1);
-    } else {
-      pos0.add(x0);
-      pos1.add(x1);
-    }
-  });
   const Blk g = *gq;
-  uint64_t r0 = fsub(fadd(g.lo, pos0.mod()), neg0.mod());
-  uint64_t r1 = fsub(fadd(g.hi, pos1.mod()), neg1.mod());
+  Blk m[kFixMaxWords];
+#pragma unroll
+  for (int w = 0; w < kFixMaxWords; ++w)
+    m[w] = w < a.mask_words
+               ? *reinterpret_cast<const Blk*>(a.mask + (size_t)w * word_stride + set_off + q0)
+               : Blk{0, 0};
+  {
+    int h = 0, inst = 0;
+    uint64_t v = 0;
+    if (threadIdx.x < kGatherMaxSlots &&
+        landing_slot(qt, a.N_log, a.bin_log, a.t, a.B, (int)threadIdx.x, h, inst)) {
+      const uint64_t cw = a.CW[(size_t)bb * a.B + inst];
+      v = h ? kP - cw : cw;  // kP - 0 = kP adds nothing mod P
+    }
+    if (threadIdx.x < kGatherMaxSlots) sc[threadIdx.x] = v;
+  }
+  __syncthreads();
+  {
+    const int j = threadIdx.x >> 4, v = threadIdx.x & 15;
+    for (int w = 0; w < a.mask_words; ++w) {
+      uint64_t x = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((v >> b) & 1) x = fadd(x, sc[64 * w + 4 * j + b]);  // sc <= P: fadd stays below P
+      tab[w][j][v] = x;
+    }
+  }
+  __syncthreads();
+  Wide s0, s1;
+#pragma unroll
+  for (int w = 0; w < kFixMaxWords; ++w) {
+    if (w >= a.mask_words) break;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      s0.add(tab[w][j][(m[w].lo >> (4 * j)) & 15]);
+      s1.add(tab[w][j][(m[w].hi >> (4 * j)) & 15]);
+    }
+  }
+  uint64_t r0 = fadd(g.lo, s0.mod()), r1 = fadd(g.hi, s1.mod());
   if (party == 1) {
     r0 = fneg(r0);
     r1 = fneg(r1);
@@ -269,7 +294,7 @@ void launch_gather_g(hipStream_t st, const GatherArgs& a) {
 void launch_gather_fix(hipStream_t st, const GatherFixArgs& a) {
   require(a.sets >= 1 && a.sets <= 65535 && a.B > 0 && a.t > 0 && a.mask_words >= 1 &&
               a.bin_log >= 9 && a.N_log >= a.bin_log && 4 * a.t <= 64 * a.mask_words &&
-              4 * a.t <= 256,
+              a.mask_words <= kFixMaxWords,
           "gather fix: bad shape");
   ProfZone pz(a.prof, st, "out.gather_fix");
   const dim3 g3((unsigned)(((size_t)1 << a.N_log) / (2 * kTpb)), (unsigned)a.sets, 2);

@@ -48,6 +48,36 @@ __device__ __forceinline__ int for_each_landing(uint32_t q, int N_log, int bin_l
   return slot;
 }
 
+// Slot `slot` of that sequence in closed form: sets h and inst and returns
+// true, or returns false past the last slot.
+__device__ __forceinline__ bool landing_slot(uint32_t q, int N_log, int bin_log, int t, int B,
+                                             int slot, int& h, int& inst) {
+  for (int hh = 0; hh < 2; ++hh) {
+    const int sb = (int)(((uint64_t)q + ((uint64_t)hh << N_log)) >> bin_log);
+    for (int ds = 0; ds < 2; ++ds) {
+      const int sd = sb - ds;
+      if (sd < 0 || sd > 2 * t - 2) continue;
+      const int kk_lo = sd - t + 1 > 0 ? sd - t + 1 : 0;
+      int kk_hi = sd < t - 1 ? sd : t - 1;
+      // inst = sd + kk*(t-1) grows with kk; the sequence stops at inst >= B
+      if (t > 1) {
+        const int kmax = B - 1 - sd >= 0 ? (B - 1 - sd) / (t - 1) : -1;
+        kk_hi = kk_hi < kmax ? kk_hi : kmax;
+      } else if (sd >= B) {
+        kk_hi = kk_lo - 1;
+      }
+      const int n = kk_hi - kk_lo + 1 > 0 ? kk_hi - kk_lo + 1 : 0;
+      if (slot < n) {
+        h = hh;
+        inst = (kk_lo + slot) * t + (sd - kk_lo - slot);
+        return true;
+      }
+      slot -= n;
+    }
+  }
+  return false;
+}
+
 // The leaf level with the gather fused in gives each workgroup at least
 // kGatherTile output coefficients of one bin (two per thread; more when the
 // bins are larger), and handles at most kGatherMaxSlots landing slots

@@ -279,9 +279,21 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   const int nparts_log = a.S_log + W - (kTpbLog + ilog2_constexpr(LPT));
   const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
   const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
+  // A: raw sums of C, reduced every third slot (three values below P added to
+  // one below P stay below 2^64).  The slots with h = 0 (+C) all come before
+  // those with h = 1 (-C): at the switch the +C sums go to gA, and at the end
+  // gA = (+C sums) - (-C sums).
+  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
+                                     ((size_t)bb << a.N_log) + q0);
   uint64_t A[LPT], mw[LPT];
 #pragma unroll
   for (int e = 0; e < LPT; ++e) A[e] = mw[e] = 0;
+  int nadd = 0, phase = 0;
+  auto reduce_A = [&]() {
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
+    nadd = 0;
+  };
   int widx = 0;
   auto store_words = [&](int wi) {
     Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
@@ -296,6 +308,17 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
 #pragma unroll
       for (int e = 0; e < LPT; ++e) mw[e] = 0;
     }
+    if (h != phase) {
+      reduce_A();
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) A[e] = 0;
+      phase = h;
+    } else if (nadd == 3) {
+      reduce_A();
+    }
+    ++nadd;
     const int T = (party * a.sets + bb) * a.B + inst;
     Blk CW[NK + 1], CWg[NK + 1];
     load_cw_uniform<W>(a, T, a.level - 1, CW);
@@ -353,7 +376,7 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
         const uint64_t C = freduce64(out_hash_lo64(leaf));
         const uint32_t tau = (uint32_t)(leaf.lo & 1ULL);
         const int q = pp * CPP + e;
-        A[q] = fadd(A[q], h ? kP - C : C);  // kP - 0 = kP adds nothing mod P
+        A[q] += C;
         mw[q] |= (uint64_t)tau << (slot & 63);
         sC = fadd(sC, C);
         sT += tau;
@@ -376,10 +399,17 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
 #pragma unroll
   for (int e = 0; e < LPT; ++e) mw[e] = 0;
   for (int wi = widx + 1; wi < a.mask_words; ++wi) store_words(wi);
-  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
-                                     ((size_t)bb << a.N_log) + q0);
+  reduce_A();
+  if (phase == 1) {  // gA holds the +C sums
 #pragma unroll
-  for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+    for (int e = 0; e < LPT; e += 2) {
+      const Blk p = gdst[e / 2];
+      gdst[e / 2] = Blk{fsub(p.lo, A[e]), fsub(p.hi, A[e + 1])};
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+  }
   __syncthreads();
   for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
     uint64_t c = red[sl][0][0], tt = red[sl][0][1];
