@@ -361,16 +361,19 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
         if constexpr (LPT >= M) {  // the leaves are children 0..m-1
           leaf = e < NK ? hp[e < NK ? e : 0] : bxor(hx, parent);
           cwc = CW[e];
-        } else {  // LPT consecutive children from a run boundary inside the parent
-          const int c = (int)(dp & (M - 1)) + e;
+        } else {  // children r*LPT + e of the parent, r = (dp mod m) / LPT
+          const int r = (int)(dp & (M - 1)) / LPT;
           leaf = bxor(hx, parent);
           cwc = CW[NK];
 #pragma unroll
-          for (int k = 0; k < NK; ++k)
-            if (k == c) {
-              leaf = hp[k];
-              cwc = CW[k];
+          for (int rr = 0; rr < M / LPT; ++rr) {
+            constexpr int kLast = NK;
+            const int c = rr * LPT + e;  // compile-time
+            if (rr == r && c != kLast) {
+              leaf = hp[c < NK ? c : 0];
+              cwc = CW[c];
             }
+          }
         }
         if (tp) leaf = bxor(leaf, cwc);
         const uint64_t C = freduce64(out_hash_lo64(leaf));
