@@ -254,6 +254,28 @@ def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
         assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("N,t", [(1 << 13, 16), (1 << 12, 16)])
+def test_ole_one_block_per_group(ctx, pq, N, t):
+    """A budget too small for two DPF blocks runs the c^2 blocks one group at
+    a time (fused gather at 2^13, the unfused path at 2^12): the same g and
+    outputs as the oracle."""
+    c, seed = 2, 5
+    assert pq.ole_plan_estimate(N, c, t, 1, O.PRG_CHACHA8, 1) < pq.ole_plan_estimate(N, c, t)
+    with pq.OlePlan(ctx, N, c, t, 1, O.PRG_CHACHA8, pub_seed=3, hbm_budget=1) as pl:
+        pl.deal(seed)
+        pl.run_blocks()
+        g = pl.g()
+        eg0, eg1 = O.pcg_ole_blocks(N, c, t, 1, O.PRG_CHACHA8, seed=seed, blk_begin=0,
+                                    blk_end=c * c)
+        assert np.array_equal(g[0], eg0)
+        assert np.array_equal(g[1], eg1)
+        pl.run()
+        got = pl.outputs()
+    exp = O.pcg_ole_expand(N, c, t, 1, O.PRG_CHACHA8, seed=seed, pub_seed=3)
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+
+
 def test_ole_relation_full_size(ctx, pq):
     """N=2^20, c=4, t=16: x0*x1 == z0+z1 (CPU oracle poly-mul), the outputs
     are deterministic across runs, and differ across seeds."""
