@@ -7,7 +7,10 @@
 //   y_s = (-1)^s (C + tau CW);  g[(kk+ll) bin + d] += y, folded X^N = -1.
 // The fold into g is a gather: every output coefficient sums the (at most
 // 4t) leaves that land on it, so no atomics are needed and the result does
-// not depend on scheduling.
+// not depend on scheduling.  The PCG-OLE expansion does the gather inside the
+// DPF leaf level (dpf.hip) and adds the tau*CW terms afterwards
+// (launch_gather_fix); launch_gather_g serves the drop-in scatter and bins of
+// fewer than kGatherTile coefficients.
 #pragma once
 
 #include <stddef.h>

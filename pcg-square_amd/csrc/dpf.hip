@@ -15,6 +15,11 @@
 // the previous level's CW (XORs only, no PRG call).  That removes a third of
 // the widest level's HBM traffic (16 of 48 bytes per node), and with it the
 // power that held that level's clock down.
+//
+// The leaf level comes in three forms: store the leaves (drop-in entry
+// points), write C|tau per leaf plus per-tree sums (convert), or, for the
+// PCG-OLE expansion, do the output layer's gather as well (leaf_gather_kernel:
+// per output coefficient, the sum of +-C and one tau bit per landing leaf).
 #include <stdio.h>
 #include <string.h>
 
