@@ -294,6 +294,20 @@ def test_ole_relation_full_size(ctx, pq):
     assert int((x0 != 0).sum()) > N // 2
 
 
+@pytest.mark.parametrize("t,w", [(64, 1), (128, 1), (32, 2)])
+def test_ole_relation_other_shapes(ctx, pq, t, w):
+    """N=2^20, c=2: x0*x1 == z0+z1 with t = 64 (the fused output layer's
+    largest, four tau words), t = 128 (beyond it: the unfused path) and
+    t = 32 at w = 2."""
+    N = 1 << 20
+    with pq.OlePlan(ctx, N, 2, t, w, pq.PRG_CHACHA8, pub_seed=1) as pl:
+        pl.deal(7)
+        pl.run()
+        x0, z0, x1, z1 = pl.outputs()
+    assert np.array_equal(O.poly_mul(x0, x1), (z0 + z1) % np.uint64(P))
+    assert int((x0 != 0).sum()) > N // 2
+
+
 @pytest.mark.parametrize("N,c", [(1 << 12, 2), (1 << 13, 3)])
 def test_triples_match_oracle(ctx, pq, N, c):
     """generate_triples (triple.h:45-110) on the device equals the oracle's."""
