@@ -48,7 +48,7 @@ __global__ void aes_ecb_kernel(const uint8_t* __restrict__ keys, const uint8_t* 
   for (int i = 0; i < 4; ++i)
     s[i] = (uint32_t)p[4 * i] | ((uint32_t)p[4 * i + 1] << 8) | ((uint32_t)p[4 * i + 2] << 16) |
            ((uint32_t)p[4 * i + 3] << 24);
-  aes128_encrypt_words(t0, rk, s[0], s[1], s[2], s[3]);
+  aes128_encrypt_words(AesTable{t0}, rk, s[0], s[1], s[2], s[3]);
   uint8_t* o = out + 16 * (size_t)b;
   for (int i = 0; i < 4; ++i)
     for (int k = 0; k < 4; ++k) o[4 * i + k] = (uint8_t)(s[i] >> (8 * k));
