@@ -395,6 +395,16 @@ def main() -> int:
             pl.run()
             pl.outputs()
         dt_pcie = (time.perf_counter() - t0) / 5
+        # ... and into page-locked host arrays that the caller keeps
+        pinned = tuple(pcgsq.pinned_array(N) for _ in range(4))
+        pl.run()
+        pl.outputs(out=pinned)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            pl.run()
+            pl.outputs(out=pinned)
+        dt_pinned = (time.perf_counter() - t0) / 5
+        del pinned
         # ---- live per-launch timing of every kernel (HIP events on the stream) ----
         ctx.profile(True)
         nprof = 2
@@ -464,6 +474,7 @@ def main() -> int:
                        "requests": len(lats)},
         "per_party_OLEs_per_s": N * args.steps / dt / 2,
         "pcie_inclusive_OLEs_per_s": N / dt_pcie,
+        "pcie_inclusive_pinned_OLEs_per_s": N / dt_pinned,
         "preprocessing_gmw": {
             "what": "position adder (Kogge-Stone) + DltSft circuit under GMW for both parties, "
                     "dealer AND triples; replaces the dealer's DltSft (same outputs)",

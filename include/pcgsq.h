@@ -91,6 +91,12 @@ int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s);
  * pointers). */
 int pcgsq_dev_alloc(pcgsq_ctx* ctx, size_t bytes, void** dev_out);
 int pcgsq_dev_free(pcgsq_ctx* ctx, void* dev);
+/* Page-locked host memory (no counterpart in the reference): outputs copied
+ * into it (pcgsq_ole_outputs and the like) move by DMA at PCIe speed instead
+ * of through the runtime's staging of pageable memory.  Free with
+ * pcgsq_host_free. */
+int pcgsq_host_alloc(size_t bytes, void** host_out);
+int pcgsq_host_free(void* host);
 int pcgsq_memcpy_h2d(pcgsq_ctx* ctx, void* dev_dst, const void* host_src,
                      size_t bytes);
 int pcgsq_memcpy_d2h(pcgsq_ctx* ctx, void* host_dst, const void* dev_src,

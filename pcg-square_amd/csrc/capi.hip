@@ -360,6 +360,21 @@ int pcgsq_dev_alloc(pcgsq_ctx* ctx, size_t bytes, void** dev_out) {
   });
 }
 
+int pcgsq_host_alloc(size_t bytes, void** host_out) {
+  return guarded([&] {
+    require(host_out, "host_alloc: null argument");
+    *host_out = nullptr;
+    hip_check(hipHostMalloc(host_out, bytes ? bytes : 1, hipHostMallocDefault),
+              "hipHostMalloc");
+  });
+}
+
+int pcgsq_host_free(void* host) {
+  return guarded([&] {
+    if (host) hip_check(hipHostFree(host), "hipHostFree");
+  });
+}
+
 int pcgsq_dev_free(pcgsq_ctx* ctx, void* dev) {
   return guarded([&] {
     require(ctx, "dev_free: null ctx");

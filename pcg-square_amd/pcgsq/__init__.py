@@ -100,6 +100,8 @@ def lib() -> C.CDLL:
     L.pcgsq_ole_plan_create.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_size_t,
                                         C.POINTER(vp)]
     L.pcgsq_ole_plan_destroy.argtypes = [vp]
+    L.pcgsq_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.pcgsq_host_free.argtypes = [vp]
     L.pcgsq_ole_plan_bytes.argtypes = [vp]
     L.pcgsq_ole_plan_bytes.restype = C.c_size_t
     L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
@@ -422,6 +424,19 @@ class DeviceBuffer:
             pass
 
 
+def pinned_array(n: int, dtype=np.uint64) -> np.ndarray:
+    """A numpy array of n elements in page-locked host memory (freed when the
+    array is collected): device-to-host copies into it run by DMA."""
+    import weakref
+    nbytes = max(1, int(n) * np.dtype(dtype).itemsize)
+    h = vp()
+    _check(lib().pcgsq_host_alloc(nbytes, C.byref(h)))
+    buf = (C.c_uint8 * nbytes).from_address(h.value)
+    arr = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dtype)[:n]
+    weakref.finalize(buf, lib().pcgsq_host_free, h.value)
+    return arr
+
+
 def poly_mul_supported(N: int, backend=NTT_FOURSTEP, prime: int = P) -> tuple[bool, str]:
     buf = C.create_string_buffer(256)
     ok = lib().pcgsq_poly_mul_supported(prime, N, backend, buf, 256)
@@ -497,8 +512,14 @@ class OlePlan:
     def run_blocks(self) -> None:
         _check(lib().pcgsq_ole_run_blocks(self.h))
 
-    def outputs(self):
-        x0, z0, x1, z1 = (np.empty(self.N, dtype=np.uint64) for _ in range(4))
+    def outputs(self, out=None):
+        """(x0, z0, x1, z1) in new arrays, or copied into `out` (four uint64
+        arrays of N, e.g. from pinned_array) and returned."""
+        if out is None:
+            out = tuple(np.empty(self.N, dtype=np.uint64) for _ in range(4))
+        x0, z0, x1, z1 = out
+        for a in out:
+            assert a.dtype == np.uint64 and a.size == self.N and a.flags.c_contiguous
         _check(lib().pcgsq_ole_outputs(self.h, _p(x0), _p(z0), _p(x1), _p(z1)))
         return x0, z0, x1, z1
 

@@ -294,6 +294,22 @@ def test_ole_relation_full_size(ctx, pq):
     assert int((x0 != 0).sum()) > N // 2
 
 
+def test_ole_outputs_into_pinned_arrays(ctx, pq):
+    """Outputs copied into page-locked host arrays equal the plain copies, on
+    repeated calls."""
+    N = 1 << 14
+    pinned = tuple(pq.pinned_array(N) for _ in range(4))
+    with pq.OlePlan(ctx, N, 2, 16, 1, pq.PRG_CHACHA8, pub_seed=2) as pl:
+        for seed in (3, 4):
+            pl.deal(seed)
+            pl.run()
+            plain = pl.outputs()
+            got = pl.outputs(out=pinned)
+            assert all(g is p for g, p in zip(got, pinned))
+            for a, b in zip(plain, pinned):
+                assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("t,w", [(64, 1), (128, 1), (32, 2)])
 def test_ole_relation_other_shapes(ctx, pq, t, w):
     """N=2^20, c=2: x0*x1 == z0+z1 with t = 64 (the fused output layer's
