@@ -103,6 +103,51 @@ def test_dpf_plateau_recipe_through_dropin_api(ctx, pq, golden):
     assert "%016x" % cks == case["checksum"]
 
 
+@pytest.mark.parametrize("n,w,B,prg", [(10, 1, 40, O.PRG_CHACHA8), (9, 3, 12, O.PRG_AES)])
+def test_dpf_two_parties_through_host_exchange(pq, n, w, B, prg):
+    """The reference's 2PC shape: each party expands its own trees on its own
+    context (one thread per party, as one process per party would), and the
+    per-level shares go through the host exchange callback (a queue pair
+    standing in for NetIO).  The leaves equal the in-device both-party path
+    and the oracle, and leaf0 ^ leaf1 is Delta at alpha only."""
+    import queue
+    import threading
+
+    rks = O.aes_tree_round_keys((1 << w) - 1) if prg == O.PRG_AES else None
+    alphas, r0, r1, dl0, dl1, delta = _dealt(n, w, B, seed=9)
+    inbox = (queue.Queue(), queue.Queue())
+    out, errs = [None, None], []
+
+    def party(p, roots, dlt):
+        try:
+            def exchange(level, mine):
+                inbox[1 - p].put((level, mine.copy()))
+                lv, peer = inbox[p].get(timeout=60)
+                assert lv == level
+                return peer
+            with pq.Context(0) as c:
+                out[p] = c.dpf_batch_full_eval(roots, dlt, n, w, exchange, party=p, prg=prg,
+                                               round_keys=rks)
+        except Exception as e:  # reported below
+            errs.append(e)
+            inbox[1 - p].put((-1, None))
+
+    th = [threading.Thread(target=party, args=(0, r0, dl0)),
+          threading.Thread(target=party, args=(1, r1, dl1))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs, errs
+    e0, e1 = O.dpf_full_eval_both(n, w, prg, r0, r1, dl0, dl1, rks=rks)
+    assert np.array_equal(out[0], e0)
+    assert np.array_equal(out[1], e1)
+    x = (out[0] ^ out[1]).reshape(B, 1 << n, 2)
+    for b in range(B):
+        assert list(np.nonzero(x[b].any(axis=1))[0]) == [alphas[b]]
+        assert np.array_equal(x[b, alphas[b]], delta[b])
+
+
 def test_host_exchange_callback_failure_reported(ctx, pq):
     roots = np.ones((2, 2), dtype=np.uint64)
     dlt = np.ones((2 * 6, 2), dtype=np.uint64)
