@@ -313,6 +313,31 @@ size_t pcgsq_ole_plan_bytes(const pcgsq_ole_plan* plan);
 /* Device bytes a plan of these parameters and budget would take, including
  * its context's NTT scratch (0 if the parameters are invalid). */
 size_t pcgsq_ole_plan_estimate(const pcgsq_params* p, size_t hbm_budget_bytes);
+
+/* ---- One party of the 2PC expansion ------------------------------------------
+ * PCG_OLE<P>::gen_and_expand as one party's process runs it
+ * (pcg_ole_impl.h:46-490, with emp::NetIO replaced by a callback): the party
+ * expands only its own DPF trees, and every exchange of the protocol goes
+ * through `fn`, in the order the reference makes them:
+ *   "dpf_gen"   per DPF level and block: the level's correction-word shares,
+ *               B * (2^w - 1) blocks of 16 bytes (dpf.cpp:520-551);
+ *   "beaver_de" per block: d and e of the first Beaver open, 2B u64;
+ *   "beaver_cw" per block: the CW shares of the second open, B u64
+ *               (pcg_ole_impl.h:231-295).
+ * (phase, level, batch = B, bytes) are the columns of the reference's
+ * comm_trace.h rows; level is the DPF level, or the block index ii*c + jj for
+ * the Beaver opens.  fn sends `bytes` from mine, receives as many from the
+ * other party into peer, and returns 0 (nonzero aborts the run with
+ * PCGSQ_ECALLBACK).  With several blocks per group, a level's rows of all
+ * the group's blocks come together.  Deal with pcgsq_ole_deal (the party
+ * reads only its own shares), run, then read x and z; the two parties' plans
+ * together give exactly the outputs of one both-party plan. */
+typedef int (*pcgsq_ole_exchange_fn)(void* user, const char* phase, int level, int batch,
+                                     const void* mine, void* peer, size_t bytes);
+int pcgsq_ole_party_create(pcgsq_ctx* ctx, const pcgsq_params* p, int party, uint64_t pub_seed,
+                           size_t hbm_budget_bytes, pcgsq_ole_plan** out);
+int pcgsq_ole_party_run(pcgsq_ole_plan* plan, pcgsq_ole_exchange_fn fn, void* user);
+int pcgsq_ole_party_outputs(pcgsq_ole_plan* plan, uint64_t* x, uint64_t* z);
 /* Deal the per-expansion inputs from `seed` (the seeded stand-in for the
  * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199):
  * the dealer runs on the device, asynchronously on the context stream. */

@@ -161,12 +161,12 @@ __global__ __launch_bounds__(kTpb) void gather_fix_kernel(GatherFixArgs a) {
   static_assert(kGatherMaxSlots <= kTpb, "one thread per landing slot");
   __shared__ uint64_t sc[kGatherMaxSlots];   // signed CW per slot, 0 past the last
   __shared__ uint64_t tab[kFixMaxWords][16][16];
-  const int bb = blockIdx.y, party = blockIdx.z;
+  const int bb = blockIdx.y, pz = blockIdx.z, party = a.first_party + pz;
   const uint32_t qt = blockIdx.x * (2 * kTpb);
   const uint32_t q0 = qt + 2 * threadIdx.x;
-  const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
-  const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
-  Blk* gq = reinterpret_cast<Blk*>(a.g + (size_t)party * a.g_party_stride + ((size_t)bb << a.N_log) +
+  const size_t set_off = ((size_t)pz * a.sets + bb) << a.N_log;
+  const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
+  Blk* gq = reinterpret_cast<Blk*>(a.g + (size_t)pz * a.g_party_stride + ((size_t)bb << a.N_log) +
                                    q0);
   const Blk g = *gq;
   Blk m[kFixMaxWords];
@@ -294,10 +294,12 @@ void launch_gather_g(hipStream_t st, const GatherArgs& a) {
 void launch_gather_fix(hipStream_t st, const GatherFixArgs& a) {
   require(a.sets >= 1 && a.sets <= 65535 && a.B > 0 && a.t > 0 && a.mask_words >= 1 &&
               a.bin_log >= 9 && a.N_log >= a.bin_log && 4 * a.t <= 64 * a.mask_words &&
-              a.mask_words <= kFixMaxWords,
+              a.mask_words <= kFixMaxWords && a.parties >= 1 &&
+              a.first_party + a.parties <= 2,
           "gather fix: bad shape");
   ProfZone pz(a.prof, st, "out.gather_fix");
-  const dim3 g3((unsigned)(((size_t)1 << a.N_log) / (2 * kTpb)), (unsigned)a.sets, 2);
+  const dim3 g3((unsigned)(((size_t)1 << a.N_log) / (2 * kTpb)), (unsigned)a.sets,
+                (unsigned)a.parties);
   hipLaunchKernelGGL(gather_fix_kernel, g3, dim3(kTpb), 0, st, a);
   hip_check(hipGetLastError(), "launch gather_fix_kernel");
 }

@@ -91,8 +91,9 @@ constexpr int kGatherMaxSlots = 256;
 
 // Second half of the fused output layer: g = (-1)^party (A + sum over the
 // landing slots whose tau bit is set of +-CW[inst]), in place over A.
-// A/g: g + p*g_party_stride + bb*N; masks [mask_words][2][sets][N]; CW
-// [sets][B] for the set's instances.  Needs bin >= 256.
+// For the launch's party p (party first_party + p): A/g at g +
+// p*g_party_stride + bb*N; masks [mask_words][parties][sets][N]; CW
+// [sets][B] for the set's instances.  Needs bin >= 512.
 struct GatherFixArgs {
   uint64_t* g = nullptr;
   size_t g_party_stride = 0;
@@ -100,6 +101,7 @@ struct GatherFixArgs {
   int mask_words = 0;
   const uint64_t* CW = nullptr;
   int sets = 0, B = 0, t = 0, N_log = 0, bin_log = 0;
+  int parties = 2, first_party = 0;
   Profiler* prof = nullptr;
 };
 void launch_gather_fix(hipStream_t st, const GatherFixArgs& a);

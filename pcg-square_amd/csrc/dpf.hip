@@ -70,7 +70,7 @@ struct LevelArgs {
   uint64_t* gA;
   size_t gA_party_stride;
   uint64_t* gmask;
-  int mask_words, t, B, N_log, bin_log, sets;
+  int mask_words, t, B, N_log, bin_log, sets, parties;
 };
 
 enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   constexpr int PPT = LPT / CPP;           // the thread's parents per slot
   __shared__ uint64_t red[kGatherMaxSlots][4][2];
   __shared__ uint64_t red_at[kGatherMaxSlots];
-  const int bb = blockIdx.y, party = blockIdx.z;
+  const int bb = blockIdx.y, party = blockIdx.z;  // party: index within this launch
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t qt = blockIdx.x * TILE;
   const uint32_t q0 = qt + LPT * threadIdx.x;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   const size_t S = (size_t)1 << a.S_log;
   const int nparts_log = a.S_log + W - (kTpbLog + ilog2_constexpr(LPT));
   const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
-  const size_t word_stride = ((size_t)2 * a.sets) << a.N_log;
+  const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
   // A: raw sums of C, reduced every third slot (three values below P added to
   // one below P stay below 2^64).  The slots with h = 0 (+C) all come before
   // those with h = 1 (-C): at the switch the +C sums go to gA, and at the end
@@ -498,7 +498,8 @@ void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
 
 template <int W, int LPT>
 void launch_leaf_gather_t(hipStream_t st, const LevelArgs& a) {
-  const dim3 grid((unsigned)(((size_t)1 << a.N_log) / ((size_t)kTpb * LPT)), (unsigned)a.sets, 2);
+  const dim3 grid((unsigned)(((size_t)1 << a.N_log) / ((size_t)kTpb * LPT)), (unsigned)a.sets,
+                  (unsigned)a.parties);
   hipLaunchKernelGGL((leaf_gather_kernel<W, LPT>), grid, dim3(kTpb), 0, st, a);
   hip_check(hipGetLastError(), "launch leaf_gather_kernel");
 }
@@ -610,6 +611,7 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
       a.N_log = r.N_log;
       a.bin_log = n - 1;
       a.sets = r.sets;
+      a.parties = r.parties;
       launch_leaf_gather<W>(st, a, r.prof);
     } else {
       a.ct = r.ct;
@@ -673,8 +675,9 @@ void dpf_expand(hipStream_t st, const DpfRun& r, DpfScratch& s) {
               (r.g_acc && r.g_mask && r.partials && r.n / r.w >= 2 &&
                ((size_t)1 << (r.n - 1)) >= (size_t)kGatherTile && r.t >= 1 && r.t <= 64 &&
                r.B >= 1 && r.B <= r.t * r.t && 4 * r.t <= 64 * r.mask_words &&
-               r.sets >= 1 && r.sets <= 65535 && r.ntrees == 2 * r.sets * r.B &&
-               r.N_log >= r.n - 1 && r.N_log <= 30 && !r.host_ex),
+               r.sets >= 1 && r.sets <= 65535 && (r.parties == 1 || r.parties == 2) &&
+               r.ntrees == r.parties * r.sets * r.B && r.N_log >= r.n - 1 && r.N_log <= 30 &&
+               (r.parties == 1 || !r.host_ex)),
           "dpf: fused gather needs bin >= 512, t <= 64, two levels and [party][sets][B] trees");
 #define PCGSQ_DISPATCH(W)                                                        \
   if (r.w == W) {                                                                \

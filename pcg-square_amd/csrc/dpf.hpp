@@ -53,15 +53,16 @@ struct DpfRun {
   uint64_t* ct = nullptr;         // kLeafConvert: [ntrees][2^n]
   uint64_t* partials = nullptr;   // kLeafConvert: [ntrees][nparts][2]; kLeafGather: nparts =
                                   // 2^n / kGatherTile
-  // kLeafGather: the trees are [party][sets][B] (sets blocks of B instances
-  // per party, t^2 >= B); block bb of party p sums into g_acc + p *
-  // g_party_stride + bb * N (N = 2^N_log) and writes its tau bits to
-  // g_mask [mask_words][2][sets][N].  Needs 2^(n-1) >= kGatherTile, t <= 64
-  // and at least two levels.
+  // kLeafGather: the trees are [parties][sets][B] (sets blocks of B
+  // instances per party, t^2 >= B; parties = 2 for both parties in one run,
+  // 1 for one party with a host exchange); block bb of the run's party p
+  // sums into g_acc + p * g_party_stride + bb * N (N = 2^N_log) and writes
+  // its tau bits to g_mask [mask_words][parties][sets][N].  Needs
+  // 2^(n-1) >= kGatherTile, t <= 64 and at least two levels.
   uint64_t* g_acc = nullptr;
   size_t g_party_stride = 0;
   uint64_t* g_mask = nullptr;
-  int mask_words = 0, t = 0, B = 0, N_log = 0, sets = 0;
+  int mask_words = 0, t = 0, B = 0, N_log = 0, sets = 0, parties = 2;
   Blk* cw_out = nullptr;          // optional [levels][ntrees][nk] copy of the shares
   int iters_max = 8;              // tiles of 256 parents per workgroup
   Profiler* prof = nullptr;       // optional per-launch timing

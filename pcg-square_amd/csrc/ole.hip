@@ -279,9 +279,9 @@ __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t*
 // X_s = sum_i A_i E_{s,i};  Z_s = sum_i A_i sum_j A_j G_{s,ij}  (NTT domain)
 __global__ void step4_dot_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ G,
                                  const uint64_t* __restrict__ E, uint64_t* __restrict__ out,
-                                 int c, int N) {
+                                 int c, int N, int s0) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = blockIdx.y;
+  const int s = s0 + blockIdx.y;  // party
   if (q >= (size_t)N) return;
   uint64_t x = 0, z = 0;
   for (int i = 0; i < c; ++i) {
@@ -346,6 +346,7 @@ struct pcgsq_ole_plan {
   pcgsq_ctx* ctx = nullptr;
   OleDims d{};
   uint64_t pub_seed = 0;
+  int party = -1;        // -1: both parties in one process; 0 / 1: that party alone
   int group_blocks = 1;
   bool dealt = false;
   DpfScratch dpf;
@@ -373,36 +374,37 @@ int leaf_nparts(const OleDims& d) {
   return fused_gather(d) ? dpf_gather_nparts(d.D_log) : dpf_leaf_nparts(d.D_log, d.w, 8);
 }
 
-size_t plan_group_bytes(const OleDims& d, int gb) {
-  const size_t trees = (size_t)2 * gb * d.B;
+// parties: 2 when one plan runs both parties, 1 for one party.
+size_t plan_group_bytes(const OleDims& d, int gb, int parties) {
+  const size_t trees = (size_t)parties * gb * d.B;
   const size_t maxS = (size_t)1 << (d.w * (d.levels - 1));
   const size_t fh = trees * maxS * (1 + d.nk) * sizeof(Blk) * 2;
   const size_t leaf = fused_gather(d)
-                          ? (size_t)gather_mask_words(d) * 2 * gb * d.N * sizeof(uint64_t)
+                          ? (size_t)gather_mask_words(d) * parties * gb * d.N * sizeof(uint64_t)
                           : trees * (size_t)d.D * sizeof(uint64_t);
   return fh + leaf;
 }
 
 // DPF blocks per group: as many as the budget holds, dividing c^2 (equal
 // groups keep every launch the same shape).
-int plan_group(const OleDims& d, size_t budget) {
+int plan_group(const OleDims& d, size_t budget, int parties) {
   if (budget == 0) budget = (size_t)96 << 30;
   int gb = d.nblk;
-  while (gb > 1 && plan_group_bytes(d, gb) > budget) --gb;
+  while (gb > 1 && plan_group_bytes(d, gb, parties) > budget) --gb;
   while (d.nblk % gb) --gb;
   return gb;
 }
 
 // Device bytes a plan (with its DPF scratch) and its context's NTT scratch
 // take, without allocating anything.
-size_t plan_estimate(const OleDims& d, size_t budget) {
-  const int gb = plan_group(d, budget);
-  const size_t gtrees = (size_t)2 * gb * d.B;
+size_t plan_estimate(const OleDims& d, size_t budget, int parties = 2) {
+  const int gb = plan_group(d, budget, parties);
+  const size_t gtrees = (size_t)parties * gb * d.B;
   const size_t trees_all = (size_t)2 * d.ninst;
   const int nparts = leaf_nparts(d);
   const size_t nfw = (size_t)2 * d.nblk + 2 * d.c;
-  size_t b = plan_group_bytes(d, gb);
-  b += (size_t)d.levels * gtrees * d.nk * sizeof(Blk);            // dpf cw shares
+  size_t b = plan_group_bytes(d, gb, parties);
+  b += (size_t)d.levels * gtrees * d.nk * sizeof(Blk) * (parties == 1 ? 2 : 1);  // cw (+ peer)
   b += trees_all * sizeof(Blk) + trees_all * d.levels * d.nk * sizeof(Blk);
   b += (size_t)8 * d.ninst * sizeof(uint64_t) + (size_t)4 * d.c * d.t * sizeof(uint64_t);
   b += (size_t)d.c * d.N * sizeof(uint64_t);                       // ahat
@@ -415,8 +417,10 @@ size_t plan_estimate(const OleDims& d, size_t budget) {
 void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   const OleDims& d = pl->d;
   if (budget == 0) budget = (size_t)96 << 30;
-  const int gb = plan_group(d, budget);
-  require(plan_group_bytes(d, gb) <= budget || gb == 1, "ole: HBM budget too small", kErrNoMemory);
+  const int parties = pl->party < 0 ? 2 : 1;
+  const int gb = plan_group(d, budget, parties);
+  require(plan_group_bytes(d, gb, parties) <= budget || gb == 1, "ole: HBM budget too small",
+          kErrNoMemory);
   pl->group_blocks = gb;
   const size_t trees_all = (size_t)2 * d.ninst;
   pl->roots.ensure(trees_all * sizeof(Blk), "ole roots");
@@ -424,19 +428,21 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   pl->bt.ensure((size_t)4 * 2 * d.ninst * sizeof(uint64_t), "ole beaver");
   pl->noise.ensure((size_t)2 * 2 * d.c * d.t * sizeof(uint64_t), "ole noise");
   pl->ahat.ensure((size_t)d.c * d.N * sizeof(uint64_t), "ole ahat");
-  const size_t gtrees = (size_t)2 * gb * d.B;
+  const size_t gtrees = (size_t)parties * gb * d.B;
   if (fused_gather(d))
-    pl->ct.ensure((size_t)gather_mask_words(d) * 2 * gb * d.N * sizeof(uint64_t), "ole tau masks");
+    pl->ct.ensure((size_t)gather_mask_words(d) * parties * gb * d.N * sizeof(uint64_t),
+                  "ole tau masks");
   else
     pl->ct.ensure(gtrees * d.D * sizeof(uint64_t), "ole ct");
   const int nparts = leaf_nparts(d);
   pl->part.ensure(gtrees * nparts * 2 * sizeof(uint64_t), "ole partials");
-  pl->cw.ensure((size_t)d.ninst * sizeof(uint64_t), "ole cw");
+  // CWs; a one-party plan also keeps its group's sums after them
+  pl->cw.ensure(((size_t)d.ninst + (parties == 1 ? 2 * gtrees : 0)) * sizeof(uint64_t), "ole cw");
   pl->fw.ensure(((size_t)2 * d.nblk + 2 * d.c) * d.N * sizeof(uint64_t), "ole ntt buffers");
   pl->out.ensure((size_t)4 * d.N * sizeof(uint64_t), "ole outputs");
   // the DPF frontier and the NTT scratch too, so that a run allocates nothing
   // (an allocation inside a run would stall other streams' work)
-  dpf_reserve(pl->dpf, d.D_log, d.w, gtrees, false);
+  dpf_reserve(pl->dpf, d.D_log, d.w, gtrees, parties == 1);
   (void)pl->ctx->ntt.scratch(((size_t)2 * d.nblk + 2 * d.c) * d.N, 0);
   if (d.prg == PCGSQ_PRG_AES) {
     uint32_t t0[256];
@@ -478,6 +484,7 @@ void plan_deal(pcgsq_ole_plan* pl, uint64_t seed, bool gmw = false) {
 // DPF blocks + output layer: leaves g_ij (folded) for both parties in fw.
 void plan_run_blocks(pcgsq_ole_plan* pl) {
   require(pl->dealt, "ole: run before deal");
+  require(pl->party < 0, "ole: a one-party plan runs with pcgsq_ole_party_run");
   const OleDims& d = pl->d;
   hipStream_t st = pl->ctx->stream;
   const int gb = pl->group_blocks;
@@ -558,6 +565,7 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
 }
 
 void plan_step4(pcgsq_ole_plan* pl) {
+  require(pl->party < 0, "ole: a one-party plan runs with pcgsq_ole_party_run");
   const OleDims& d = pl->d;
   hipStream_t st = pl->ctx->stream;
   uint64_t* G = pl->fw.as<uint64_t>();
@@ -575,9 +583,181 @@ void plan_step4(pcgsq_ole_plan* pl) {
   pl->ctx->ntt.forward(st, G, 2 * d.nblk + 2 * d.c, d.N, kNttFourStep);
   ProfZone pzd(&pl->ctx->prof, st, "ole.step4_dot");
   hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 2), dim3(256), 0, st,
-                     pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N);
+                     pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N, 0);
   hip_check(hipGetLastError(), "launch step4_dot");
   pl->ctx->ntt.inverse(st, pl->out.as<uint64_t>(), 4, d.N, kNttFourStep);
+}
+
+// ---- one party (pcg_ole_impl.h:46-490 as run by one process) --------------
+
+struct PartyExchange {
+  pcgsq_ole_exchange_fn fn;
+  void* user;
+  int B, nk, gb;
+};
+
+// The DPF's per-level exchange of a group of blocks, one call per block as
+// the reference exchanges one DPF batch per block (dpf.cpp:535-551).
+int party_dpf_exchange(void* user, int level, const Blk* mine, Blk* peer, size_t count) {
+  const PartyExchange* x = static_cast<const PartyExchange*>(user);
+  const size_t per = (size_t)x->B * x->nk;
+  if (count != per * (size_t)x->gb) return -1;
+  for (int bb = 0; bb < x->gb; ++bb) {
+    const int rc = x->fn(x->user, "dpf_gen", level, x->B, mine + bb * per, peer + bb * per,
+                         per * sizeof(Blk));
+    if (rc != 0) return rc;
+  }
+  return 0;
+}
+
+void party_exchange(const PartyExchange& x, const char* phase, int level, const void* mine,
+                    void* peer, size_t bytes) {
+  if (x.fn(x.user, phase, level, x.B, mine, peer, bytes) != 0)
+    throw Error(kErrCallback, std::string("ole: exchange callback failed at ") + phase);
+}
+
+// This party's blocks: the DPF with the per-level exchange, the sums, the
+// two Beaver opens per block on the host (pcg_ole_impl.h:213-295: d = A - r,
+// e = B - s; CW share = d s + e r + rs (+ d e for party 0)), and this
+// party's g.
+void plan_run_party(pcgsq_ole_plan* pl, pcgsq_ole_exchange_fn fn, void* user) {
+  require(pl->dealt, "ole: run before deal");
+  require(pl->party == 0 || pl->party == 1, "ole: not a one-party plan");
+  const OleDims& d = pl->d;
+  const int p = pl->party;
+  hipStream_t st = pl->ctx->stream;
+  const int gb = pl->group_blocks;
+  const size_t ni = (size_t)d.ninst, gi = (size_t)gb * d.B;
+  const int nparts = leaf_nparts(d);
+  const bool fused = fused_gather(d);
+  uint64_t* G = pl->fw.as<uint64_t>() + (size_t)p * d.nblk * d.N;  // this party's g [nblk][N]
+  const uint64_t* BT = pl->bt.as<uint64_t>();
+  PartyExchange px{fn, user, d.B, d.nk, gb};
+  std::vector<uint64_t> sums(2 * gi), bt(4 * gi), cw_full(gi);
+  std::vector<uint64_t> de_my(2 * (size_t)d.B), de_peer(2 * (size_t)d.B);
+  std::vector<uint64_t> cw_my(d.B), cw_peer(d.B);
+  uint64_t* dsums = pl->cw.as<uint64_t>() + ni;  // scratch after the CWs
+  for (int b0 = 0; b0 < d.nblk; b0 += gb) {
+    const size_t i0 = (size_t)b0 * d.B;
+    DpfRun r;
+    r.ntrees = (int)gi;
+    r.n = d.D_log;
+    r.w = d.w;
+    r.prg = d.prg;
+    r.roots = pl->roots.as<Blk>() + (size_t)p * ni + i0;
+    r.dltsft = pl->dlt.as<Blk>() + ((size_t)p * ni + i0) * d.levels * d.nk;
+    r.aes_t0 = pl->aes_t0.as<uint32_t>();
+    r.aes_rks = pl->aes_rks.as<uint8_t>();
+    r.host_ex = party_dpf_exchange;
+    r.host_user = &px;
+    r.partials = pl->part.as<uint64_t>();
+    r.prof = &pl->ctx->prof;
+    if (fused) {
+      r.mode = kLeafGather;
+      r.g_acc = G + (size_t)b0 * d.N;
+      r.g_mask = pl->ct.as<uint64_t>();
+      r.mask_words = gather_mask_words(d);
+      r.t = d.t;
+      r.B = d.B;
+      r.N_log = d.L;
+      r.sets = gb;
+      r.parties = 1;
+    } else {
+      r.mode = kLeafConvert;
+      r.ct = pl->ct.as<uint64_t>();
+    }
+    dpf_expand(st, r, pl->dpf);
+    launch_fold_sums(st, pl->part.as<uint64_t>(), (int)gi, nparts, dsums, dsums + gi);
+    hip_check(hipMemcpyAsync(sums.data(), dsums, 2 * gi * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, st), "ole party sums");
+    for (int k = 0; k < 4; ++k)  // beta, r, s, rs of this party
+      hip_check(hipMemcpyAsync(bt.data() + k * gi, BT + (size_t)k * 2 * ni + (size_t)p * ni + i0,
+                               gi * sizeof(uint64_t), hipMemcpyDeviceToHost, st),
+                "ole party beaver shares");
+    hip_check(hipStreamSynchronize(st), "ole party sums");
+    const uint64_t* sumC = sums.data();
+    const uint64_t* sumT = sums.data() + gi;
+    const uint64_t *beta = bt.data(), *rr = bt.data() + gi, *ss = bt.data() + 2 * gi,
+                   *rs = bt.data() + 3 * gi;
+    for (int bb = 0; bb < gb; ++bb) {
+      const size_t o = (size_t)bb * d.B;
+      for (int k = 0; k < d.B; ++k) {
+        uint64_t A = freduce64(sumT[o + k]);
+        if (p == 1) A = fneg(A);
+        uint64_t Bs = sumC[o + k];
+        if (p == 0) Bs = fneg(Bs);
+        Bs = fadd(Bs, beta[o + k]);
+        de_my[k] = fsub(A, rr[o + k]);
+        de_my[d.B + k] = fsub(Bs, ss[o + k]);
+      }
+      party_exchange(px, "beaver_de", b0 + bb, de_my.data(), de_peer.data(),
+                     de_my.size() * sizeof(uint64_t));
+      for (int k = 0; k < d.B; ++k) {
+        const uint64_t dv = fadd(de_my[k], de_peer[k]);
+        const uint64_t ev = fadd(de_my[d.B + k], de_peer[d.B + k]);
+        uint64_t c = fadd(fadd(fmul(dv, ss[o + k]), fmul(ev, rr[o + k])), rs[o + k]);
+        if (p == 0) c = fadd(c, fmul(dv, ev));
+        cw_my[k] = c;
+      }
+      party_exchange(px, "beaver_cw", b0 + bb, cw_my.data(), cw_peer.data(),
+                     cw_my.size() * sizeof(uint64_t));
+      for (int k = 0; k < d.B; ++k) cw_full[o + k] = fadd(cw_my[k], cw_peer[k]);
+    }
+    hip_check(hipMemcpyAsync(pl->cw.as<uint64_t>() + i0, cw_full.data(), gi * sizeof(uint64_t),
+                             hipMemcpyHostToDevice, st), "ole party cw");
+    if (fused) {
+      GatherFixArgs fa;
+      fa.g = G + (size_t)b0 * d.N;
+      fa.mask = pl->ct.as<uint64_t>();
+      fa.mask_words = gather_mask_words(d);
+      fa.CW = pl->cw.as<uint64_t>() + i0;
+      fa.sets = gb;
+      fa.B = d.B;
+      fa.t = d.t;
+      fa.N_log = d.L;
+      fa.bin_log = d.D_log - 1;
+      fa.parties = 1;
+      fa.first_party = p;
+      fa.prof = &pl->ctx->prof;
+      launch_gather_fix(st, fa);
+    } else {
+      GatherArgs ga;
+      ga.ct = pl->ct.as<uint64_t>();
+      ga.set_src_stride = (size_t)d.B * d.D;
+      ga.CW = pl->cw.as<uint64_t>() + i0;
+      ga.cw_stride = (size_t)d.B;
+      ga.g = G + (size_t)b0 * d.N;
+      ga.nsets = gb;
+      ga.sets_per_party = gb;
+      ga.first_party = p;
+      ga.B = d.B;
+      ga.t = d.t;
+      ga.N = d.N;
+      ga.D_log = d.D_log;
+      ga.prof = &pl->ctx->prof;
+      launch_gather_g(st, ga);
+    }
+  }
+  // step 4 for this party: x = sum_i a_i e_i, z = sum_ij a_i a_j g_ij
+  uint64_t* E = pl->fw.as<uint64_t>() + (size_t)2 * d.nblk * d.N + (size_t)p * d.c * d.N;
+  const size_t ne_words = (size_t)d.c * d.N;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ne_words + 255) / 256)), dim3(256), 0, st, E,
+                     ne_words);
+  const int ne = 2 * d.c * d.t;
+  hipLaunchKernelGGL(build_e_kernel, dim3((d.t + 255) / 256, d.c), dim3(256), 0, st, E,
+                     pl->noise.as<uint64_t>() + (size_t)p * d.c * d.t,
+                     pl->noise.as<uint64_t>() + ne + (size_t)p * d.c * d.t, d.c, d.t, d.N);
+  hip_check(hipGetLastError(), "launch build_e");
+  pl->ctx->ntt.prof = &pl->ctx->prof;
+  pl->ctx->ntt.forward(st, G, d.nblk, d.N, kNttFourStep);
+  pl->ctx->ntt.forward(st, E, d.c, d.N, kNttFourStep);
+  hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 1), dim3(256), 0, st,
+                     pl->ahat.as<uint64_t>(), pl->fw.as<uint64_t>(),
+                     pl->fw.as<uint64_t>() + (size_t)2 * d.nblk * d.N, pl->out.as<uint64_t>(),
+                     d.c, d.N, p);
+  hip_check(hipGetLastError(), "launch step4_dot");
+  pl->ctx->ntt.inverse(st, pl->out.as<uint64_t>() + (size_t)2 * p * d.N, 2, d.N, kNttFourStep);
+  hip_check(hipStreamSynchronize(st), "ole party run");
 }
 
 }  // namespace
@@ -595,6 +775,47 @@ int pcgsq_ole_plan_create(pcgsq_ctx* ctx, const pcgsq_params* p, uint64_t pub_se
     pl->pub_seed = pub_seed;
     plan_alloc(pl.get(), hbm_budget_bytes);
     *out = pl.release();
+  });
+}
+
+int pcgsq_ole_party_create(pcgsq_ctx* ctx, const pcgsq_params* p, int party, uint64_t pub_seed,
+                           size_t hbm_budget_bytes, pcgsq_ole_plan** out) {
+  return guarded([&] {
+    require(ctx && p && out, "ole_party_create: null argument");
+    require(party == 0 || party == 1, "ole_party_create: party must be 0 or 1");
+    CtxGuard g(ctx);
+    std::unique_ptr<pcgsq_ole_plan> pl(new pcgsq_ole_plan);
+    pl->ctx = ctx;
+    pl->d = make_dims(*p);
+    pl->pub_seed = pub_seed;
+    pl->party = party;
+    plan_alloc(pl.get(), hbm_budget_bytes);
+    *out = pl.release();
+  });
+}
+
+int pcgsq_ole_party_run(pcgsq_ole_plan* plan, pcgsq_ole_exchange_fn fn, void* user) {
+  return guarded([&] {
+    require(plan && fn, "ole_party_run: null argument");
+    CtxGuard g(plan->ctx);
+    plan_run_party(plan, fn, user);
+  });
+}
+
+int pcgsq_ole_party_outputs(pcgsq_ole_plan* plan, uint64_t* x, uint64_t* z) {
+  return guarded([&] {
+    require(plan, "ole_party_outputs: null plan");
+    require(plan->party == 0 || plan->party == 1, "ole_party_outputs: not a one-party plan");
+    CtxGuard g(plan->ctx);
+    const size_t N = (size_t)plan->d.N;
+    const uint64_t* o = plan->out.as<uint64_t>() + (size_t)2 * plan->party * N;
+    if (x)
+      hip_check(hipMemcpyAsync(x, o, N * 8, hipMemcpyDeviceToHost, plan->ctx->stream),
+                "ole party outputs");
+    if (z)
+      hip_check(hipMemcpyAsync(z, o + N, N * 8, hipMemcpyDeviceToHost, plan->ctx->stream),
+                "ole party outputs");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole party outputs");
   });
 }
 
@@ -678,6 +899,7 @@ int pcgsq_ole_outputs(pcgsq_ole_plan* plan, uint64_t* x0, uint64_t* z0, uint64_t
                       uint64_t* z1) {
   return guarded([&] {
     require(plan, "ole_outputs: null plan");
+    require(plan->party < 0, "ole_outputs: a one-party plan has pcgsq_ole_party_outputs");
     CtxGuard g(plan->ctx);
     const size_t N = (size_t)plan->d.N;
     uint64_t* dst[4] = {x0, z0, x1, z1};

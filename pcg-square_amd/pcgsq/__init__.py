@@ -43,6 +43,8 @@ class Params(C.Structure):
 
 
 EXCHANGE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+OLE_EXCHANGE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_size_t)
 
 _lib = None
 u64p = C.POINTER(C.c_uint64)
@@ -102,6 +104,10 @@ def lib() -> C.CDLL:
     L.pcgsq_ole_plan_destroy.argtypes = [vp]
     L.pcgsq_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
     L.pcgsq_host_free.argtypes = [vp]
+    L.pcgsq_ole_party_create.argtypes = [vp, C.POINTER(Params), C.c_int, C.c_uint64, C.c_size_t,
+                                         C.POINTER(vp)]
+    L.pcgsq_ole_party_run.argtypes = [vp, OLE_EXCHANGE_FN, vp]
+    L.pcgsq_ole_party_outputs.argtypes = [vp, u64p, u64p]
     L.pcgsq_ole_plan_bytes.argtypes = [vp]
     L.pcgsq_ole_plan_bytes.restype = C.c_size_t
     L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
@@ -539,6 +545,87 @@ class OlePlan:
         _check(lib().pcgsq_triples_generate(self.h, seed1, seed2, _p(a0), _p(b0), _p(c0),
                                             _p(a1), _p(b1), _p(c1)))
         return a0, b0, c0, a1, b1, c1
+
+
+class OleParty(OlePlan):
+    """One party of the 2PC expansion, as one party's process runs it
+    (pcgsq_ole_party_*; pcg_ole_impl.h:46-490 with emp::NetIO replaced by a
+    callable).  run(exchange) calls exchange(phase, level, batch, mine) for
+    every exchange of the protocol ("dpf_gen" per DPF level and block,
+    "beaver_de" and "beaver_cw" per block); mine is a uint8 array, and the
+    callable returns the other party's bytes of the same length."""
+
+    def __init__(self, ctx: Context, N: int, c: int, t: int, party: int, w: int = 1,
+                 prg: int = PRG_CHACHA8, pub_seed: int = 0, hbm_budget: int = 0):
+        self.ctx = ctx
+        self.N, self.c, self.t, self.w, self.prg, self.party = N, c, t, w, prg, party
+        p = Params(N, c, t, w, prg)
+        h = vp()
+        _check(lib().pcgsq_ole_party_create(ctx.h, C.byref(p), party, pub_seed, hbm_budget,
+                                            C.byref(h)))
+        self.h = h
+
+    def run(self, exchange) -> None:
+        err = []
+
+        def tramp(user, phase, level, batch, mine, peer, nbytes):
+            try:
+                m = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(mine)).copy()
+                got = np.ascontiguousarray(exchange(phase.decode(), level, batch, m),
+                                           dtype=np.uint8).reshape(-1)
+                if got.size != nbytes:
+                    raise ValueError(f"exchange returned {got.size} bytes, expected {nbytes}")
+                C.memmove(peer, got.ctypes.data, nbytes)
+                return 0
+            except BaseException as e:  # re-raised after the run
+                err.append(e)
+                return 1
+
+        cb = OLE_EXCHANGE_FN(tramp)
+        rc = lib().pcgsq_ole_party_run(self.h, cb, None)
+        if err:
+            raise err[0]
+        _check(rc)
+
+    def outputs(self, out=None):
+        """(x, z) of this party."""
+        if out is None:
+            out = tuple(np.empty(self.N, dtype=np.uint64) for _ in range(2))
+        x, z = out
+        _check(lib().pcgsq_ole_party_outputs(self.h, _p(x), _p(z)))
+        return x, z
+
+
+class CommTrace:
+    """Rows of the reference's comm_trace.h CSV (party, context, phase, level,
+    batch, send_bytes, recv_bytes, t_begin_ns, t_mid_ns, t_end_ns,
+    io_counter_delta) for the exchanges of an OleParty run: wrap(exchange)
+    returns a traced exchange."""
+
+    COLUMNS = ("party,context,phase,level,batch,send_bytes,recv_bytes,"
+               "t_begin_ns,t_mid_ns,t_end_ns,io_counter_delta")
+
+    def __init__(self, party: int, context: str = ""):
+        self.party, self.context, self.rows = party, context, []
+
+    def wrap(self, exchange):
+        import time as _t
+
+        def traced(phase, level, batch, mine):
+            t0 = _t.perf_counter_ns()
+            peer = exchange(phase, level, batch, mine)
+            t1 = _t.perf_counter_ns()
+            n = int(np.asarray(mine).nbytes)
+            self.rows.append((self.party, self.context, phase, level, batch, n,
+                              int(np.asarray(peer).nbytes), t0, t0, t1, n))
+            return peer
+        return traced
+
+    def to_csv(self, path: str) -> None:
+        with open(path, "w") as f:
+            f.write(self.COLUMNS + "\n")
+            for r in self.rows:
+                f.write(",".join(str(v) for v in r) + "\n")
 
 
 NTT_BATCH_FORWARD, NTT_BATCH_INVERSE = 1, 2

@@ -339,6 +339,65 @@ def test_ole_relation_full_size(ctx, pq):
     assert int((x0 != 0).sum()) > N // 2
 
 
+@pytest.mark.parametrize("N,c,t,w,prg,budget", [
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0),   # bins of 512: the fused output layer
+    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0),   # bins of 256: the unfused one
+    (1 << 14, 2, 8, 2, O.PRG_AES, 0),
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1)])  # one DPF block per group
+def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget):
+    """The two parties of the 2PC expansion on two contexts (two threads, as
+    two processes would be), every exchange through a queue pair standing in
+    for NetIO: each party's (x, z) equals the both-party plan's, and the
+    exchanges are the reference's comm_trace.h rows (per block: one
+    "dpf_gen" per DPF level, then "beaver_de" and "beaver_cw")."""
+    import queue
+    import threading
+
+    seed, pub = 13, 4
+    with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=pub) as pl:
+        pl.deal(seed)
+        pl.run()
+        expect_out = pl.outputs()
+    inbox = (queue.Queue(), queue.Queue())
+    res, errs = [None, None], []
+    traces = [pq.CommTrace(0, "test"), pq.CommTrace(1, "test")]
+
+    def party(p):
+        try:
+            def exchange(phase, level, batch, mine):
+                inbox[1 - p].put((phase, level, mine))
+                ph, lv, peer = inbox[p].get(timeout=60)
+                assert (ph, lv) == (phase, level)
+                return peer
+            with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, p, w, prg, pub_seed=pub,
+                                                    hbm_budget=budget) as op:
+                op.deal(seed)
+                op.run(traces[p].wrap(exchange))
+                res[p] = op.outputs()
+        except Exception as e:  # reported below
+            errs.append(e)
+            inbox[1 - p].put(("abort", -1, None))
+
+    th = [threading.Thread(target=party, args=(p,)) for p in (0, 1)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=120)
+    assert not errs, errs
+    for p in (0, 1):
+        assert np.array_equal(res[p][0], expect_out[2 * p]), f"x of party {p}"
+        assert np.array_equal(res[p][1], expect_out[2 * p + 1]), f"z of party {p}"
+    B, nk = t * t, (1 << w) - 1
+    levels = ((2 * N // t).bit_length() - 1) // w
+    rows = ([("dpf_gen", lv, B, B * nk * 16) for b in range(c * c) for lv in range(levels)] +
+            [("beaver_de", b, B, 16 * B) for b in range(c * c)] +
+            [("beaver_cw", b, B, 8 * B) for b in range(c * c)])
+    for p in (0, 1):
+        got = [(r[2], r[3], r[4], r[5]) for r in traces[p].rows]
+        assert sorted(got) == sorted(rows)
+        assert all(r[5] == r[6] for r in traces[p].rows)
+
+
 def test_ole_outputs_into_pinned_arrays(ctx, pq):
     """Outputs copied into page-locked host arrays equal the plain copies, on
     repeated calls."""
