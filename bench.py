@@ -251,6 +251,55 @@ def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
     return out
 
 
+def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
+    """The 2PC expansion as two party processes run it (pcgsq_ole_party_*):
+    two threads, each with its own context and plan for one party, every
+    exchange through an in-memory queue pair; both parties share this one
+    GPU (with a party per GPU each would have a GPU of its own)."""
+    import queue
+    import threading
+    inbox = (queue.Queue(), queue.Queue())
+    stats = [{"exchanges": 0, "bytes": 0}, {"exchanges": 0, "bytes": 0}]
+    times, errs = [], []
+    go = [threading.Barrier(2) for _ in range(reps + 1)]
+
+    def party(p):
+        try:
+            def exchange(phase, level, batch, mine):
+                stats[p]["exchanges"] += 1
+                stats[p]["bytes"] += int(mine.nbytes)
+                inbox[1 - p].put(mine)
+                return inbox[p].get(timeout=120)
+            with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, p) as op:
+                op.deal(1)
+                for k in range(reps + 1):   # the first round warms up
+                    go[k].wait()
+                    t0 = time.perf_counter()
+                    op.run(exchange)
+                    if p == 0 and k > 0:
+                        times.append(time.perf_counter() - t0)
+                op.outputs()
+        except Exception as e:  # reported in the result
+            errs.append(repr(e))
+            for b in go:
+                b.abort()
+
+    th = [threading.Thread(target=party, args=(p,)) for p in (0, 1)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    if errs or not times:
+        return {"error": errs[:1]}
+    ms = 1e3 * sum(times) / len(times)
+    return {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
+                        "as two threads with a context and one-party plan each, exchanges "
+                        "through a queue pair; both on this one GPU",
+            "ms": ms, "OLEs_per_s": N / (ms * 1e-3),
+            "exchanges_per_party_per_run": stats[0]["exchanges"] // (reps + 1),
+            "bytes_sent_per_party_per_run": stats[0]["bytes"] // (reps + 1)}
+
+
 def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     """Served PCG-OLE: Poisson arrivals (rate = 75% of the measured
     one-at-a-time capacity), N uniform over {2^16, 2^18, 2^20, 2^22}, c=4,
@@ -322,15 +371,16 @@ def main() -> int:
     ap.add_argument("--c", type=int, default=4)
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,arity,served",
-                    help="comma list of expand, ntt_batch, dpf_batch, arity, served; without "
-                         "expand only the named parts run and print (for profiling one workload)")
+    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,arity,parties,served",
+                    help="comma list of expand, ntt_batch, dpf_batch, arity, parties, served; "
+                         "without expand only the named parts run and print (for profiling "
+                         "one workload)")
     ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
                     help="batched NTT: run only this mode, untuned, with no comparisons "
                          "(for profiling it)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
-    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "served"}
+    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "parties", "served"}
     if unknown:
         ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
@@ -351,6 +401,8 @@ def main() -> int:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:
                 res["arity"] = bench_arity(ctx, pcgsq)
+        if "parties" in parts:
+            res["two_party_processes"] = bench_two_parties(pcgsq, N, c, t)
         if "served" in parts:
             res["served"] = bench_served(pcgsq)
         print(json.dumps(res), flush=True)
@@ -418,6 +470,7 @@ def main() -> int:
         ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
         arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
+    two_parties = bench_two_parties(pcgsq, N, c, t) if "parties" in parts else None
     served = bench_served(pcgsq) if "served" in parts else None
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
@@ -490,6 +543,7 @@ def main() -> int:
         "batched_ntt": ntt_batch,
         "batched_dpf": dpf_batch,
         "arity": arity,
+        "two_party_processes": two_parties,
         "served": served,
     }
     if not args.no_cpu:
