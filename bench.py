@@ -10,9 +10,11 @@ protocol, the Beaver-corrected output layer and step 4.  The per-step inputs
 come from a seeded dealer and are resident in HBM before the timed region.
 
 Prints one JSON line: OLEs/s (N OLEs per expansion), p50/p99 latency of one
-request per launch, the roofline of the dominant kernel (the fused leaf-level
-DPF + ChaCha8 out-hash kernel, VALU-bound), the NTT passes' HBM roofline,
-and the CPU oracle timed on this host.
+request per launch, the roofline of the dominant kernel (the DPF leaf level
+fused with the ChaCha8 out-hash and the output layer's gather, VALU-bound),
+the NTT passes' HBM roofline, and the CPU oracle timed on this host; then
+the other workloads (--parts): batched NTTs, batched DPF keys, DPF arity,
+the expansion as two party processes, and the continuous-batching server.
 """
 from __future__ import annotations
 
