@@ -339,12 +339,13 @@ def test_ole_relation_full_size(ctx, pq):
     assert int((x0 != 0).sum()) > N // 2
 
 
-@pytest.mark.parametrize("N,c,t,w,prg,budget", [
-    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0),   # bins of 512: the fused output layer
-    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0),   # bins of 256: the unfused one
-    (1 << 14, 2, 8, 2, O.PRG_AES, 0),
-    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1)])  # one DPF block per group
-def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget):
+@pytest.mark.parametrize("N,c,t,w,prg,budget,gmw", [
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, False),   # bins of 512: the fused output layer
+    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0, False),   # bins of 256: the unfused one
+    (1 << 14, 2, 8, 2, O.PRG_AES, 0, False),
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1, False),   # one DPF block per group
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, True)])   # DltSft from the GMW circuits
+def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, gmw):
     """The two parties of the 2PC expansion on two contexts (two threads, as
     two processes would be), every exchange through a queue pair standing in
     for NetIO: each party's (x, z) equals the both-party plan's, and the
@@ -355,7 +356,7 @@ def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget):
 
     seed, pub = 13, 4
     with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=pub) as pl:
-        pl.deal(seed)
+        pl.deal(seed, gmw)
         pl.run()
         expect_out = pl.outputs()
     inbox = (queue.Queue(), queue.Queue())
@@ -371,7 +372,7 @@ def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget):
                 return peer
             with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, p, w, prg, pub_seed=pub,
                                                     hbm_budget=budget) as op:
-                op.deal(seed)
+                op.deal(seed, gmw)
                 op.run(traces[p].wrap(exchange))
                 res[p] = op.outputs()
         except Exception as e:  # reported below
