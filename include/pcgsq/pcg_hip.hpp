@@ -293,6 +293,70 @@ class OleExpander {
   pcgsq_ole_plan* plan_ = nullptr;
 };
 
+// ---- One party of the 2PC expansion ------------------------------------------------
+// pcg_ole::PCG_OLE<P>::gen_and_expand as one party's process runs it
+// (pcg_ole_impl.h:46-490): the party expands only its own DPF trees, and each
+// exchange the reference makes over emp::NetIO goes through `exchange`, in the
+// reference's order, with the reference's comm_trace.h columns (phase, level,
+// batch, bytes).  `exchange` sends `bytes` from mine and receives as many from
+// the other party into peer; throwing aborts the expansion (rethrown as
+// std::runtime_error).  Each party owns its own device context, so the two
+// parties may run at once from two threads of one process, or as two
+// processes; the outputs of the two together equal OleExpander's.
+class OleParty {
+ public:
+  using ExchangeFn = std::function<void(const char* phase, int level, int batch,
+                                        const void* mine, void* peer, size_t bytes)>;
+  struct Output {
+    std::vector<uint64_t> x, z;   // this party's shares: z0 + z1 = x0 * x1 in R_p
+  };
+
+  OleParty(int party, int N, int c, int t, int w = 1, DpfGpuPrg prg = DpfGpuPrg::CHACHA8,
+           uint64_t pub_seed = 0, size_t hbm_budget = 0, int device = 0)
+      : N_(N) {
+    detail::check(pcgsq_ctx_create(device, &ctx_));
+    pcgsq_params p{N, c, t, w, detail::prg_code(prg)};
+    const int rc = pcgsq_ole_party_create(ctx_, &p, party, pub_seed, hbm_budget, &plan_);
+    if (rc != PCGSQ_OK) {
+      const std::string why = pcgsq_last_error();
+      pcgsq_ctx_destroy(ctx_);
+      throw std::runtime_error("pcg_hip: " + why);
+    }
+  }
+  ~OleParty() {
+    if (plan_) pcgsq_ole_plan_destroy(plan_);
+    if (ctx_) pcgsq_ctx_destroy(ctx_);
+  }
+  OleParty(const OleParty&) = delete;
+  OleParty& operator=(const OleParty&) = delete;
+
+  // gmw: take this party's DltSft share from the preprocessing circuits run
+  // under GMW (pcg_ole_impl.h:114-199) instead of from the dealer.
+  Output expand(uint64_t seed, const ExchangeFn& exchange, bool gmw = false) {
+    detail::check(gmw ? pcgsq_ole_deal_gmw(plan_, seed) : pcgsq_ole_deal(plan_, seed));
+    detail::check(pcgsq_ole_party_run(plan_, tramp, const_cast<ExchangeFn*>(&exchange)));
+    Output o;
+    o.x.resize(N_);
+    o.z.resize(N_);
+    detail::check(pcgsq_ole_party_outputs(plan_, o.x.data(), o.z.data()));
+    return o;
+  }
+
+ private:
+  static int tramp(void* user, const char* phase, int level, int batch, const void* mine,
+                   void* peer, size_t bytes) {
+    try {
+      (*static_cast<const ExchangeFn*>(user))(phase, level, batch, mine, peer, bytes);
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+  int N_;
+  pcgsq_ctx* ctx_ = nullptr;
+  pcgsq_ole_plan* plan_ = nullptr;
+};
+
 }  // namespace pcg_hip
 
 #endif  // PCGSQ_PCG_HIP_HPP

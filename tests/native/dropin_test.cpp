@@ -13,11 +13,19 @@
 //   4. OleExpander: x0 * x1 == z0 + z1 at N = 2^12
 //   5. OleExpander::triples: every slot c0 + c1 == (a0 + a1)(b0 + b1)
 //   6. OleExpander::expand(seed, gmw = true) == expand(seed)
+//   7. OleParty: the two parties in two threads, every exchange through an
+//      in-process channel standing in for emp::NetIO; each party's (x, z)
+//      equals OleExpander's, with one "dpf_gen" row per DPF level and block
 // Exit 0 = PASS, 1 = FAIL, 77 = no device (the reference's skip code).
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <random>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "pcgsq/pcg_hip.hpp"
@@ -46,6 +54,35 @@ static std::vector<uint64_t> schoolbook(const std::vector<uint64_t>& a,
 
 static int fails = 0;
 #define CHECK(c, msg) do { if (!(c)) { ++fails; fprintf(stderr, "FAIL: %s\n", msg); } } while (0)
+
+// A blocking two-party byte channel: send() posts one message for the other
+// party, recv() waits for the next message from it.
+struct Channel {
+  std::mutex m;
+  std::condition_variable cv;
+  std::vector<std::vector<uint8_t>> box[2];   // box[p]: messages waiting for party p
+  bool aborted = false;
+  void send(int to, const void* src, size_t n) {
+    std::lock_guard<std::mutex> lk(m);
+    box[to].emplace_back((const uint8_t*)src, (const uint8_t*)src + n);
+    cv.notify_all();
+  }
+  void recv(int me, void* dst, size_t n) {
+    std::unique_lock<std::mutex> lk(m);
+    if (!cv.wait_for(lk, std::chrono::seconds(60), [&] { return aborted || !box[me].empty(); }) ||
+        aborted)
+      throw std::runtime_error("channel closed");
+    std::vector<uint8_t> msg = std::move(box[me].front());
+    box[me].erase(box[me].begin());
+    if (msg.size() != n) throw std::runtime_error("message size mismatch");
+    memcpy(dst, msg.data(), n);
+  }
+  void abort() {
+    std::lock_guard<std::mutex> lk(m);
+    aborted = true;
+    cv.notify_all();
+  }
+};
 
 int main() {
   if (!pcg_cuda::is_cuda_available()) {
@@ -174,6 +211,38 @@ int main() {
     const bool same = og.x0 == o.x0 && og.z0 == o.z0 && og.x1 == o.x1 && og.z1 == o.z1;
     CHECK(same, "GMW-preprocessed expansion == dealer expansion");
     printf("GMW preprocessing N=%d: %s\n", N, same ? "same outputs" : "DIFFERS");
+    // 7. one party per thread, exchanges through the channel
+    Channel ch;
+    pcg_hip::OleParty::Output po[2];
+    int dpf_rows[2] = {0, 0}, rows[2] = {0, 0};
+    std::string err[2];
+    auto run = [&](int p) {
+      try {
+        pcg_hip::OleParty party(p, N, 2, 16);
+        po[p] = party.expand(3, [&](const char* phase, int, int, const void* mine, void* peer,
+                                    size_t bytes) {
+          ch.send(1 - p, mine, bytes);
+          ch.recv(p, peer, bytes);
+          ++rows[p];
+          dpf_rows[p] += strcmp(phase, "dpf_gen") == 0;
+        });
+      } catch (const std::exception& e) {
+        err[p] = e.what();
+        ch.abort();
+      }
+    };
+    std::thread t0(run, 0), t1(run, 1);
+    t0.join();
+    t1.join();
+    CHECK(err[0].empty() && err[1].empty(), "both parties finish");
+    const bool pok = po[0].x == o.x0 && po[0].z == o.z0 && po[1].x == o.x1 && po[1].z == o.z1;
+    CHECK(pok, "OleParty outputs == OleExpander outputs");
+    const int levels = 9, blocks = 4;   // log2(2N/t) DPF levels at w = 1; c*c blocks
+    CHECK(dpf_rows[0] == levels * blocks && dpf_rows[1] == levels * blocks,
+          "one dpf_gen exchange per DPF level and block");
+    CHECK(rows[0] == (levels + 2) * blocks && rows[1] == rows[0], "two Beaver opens per block");
+    printf("two parties N=%d: %s (%d exchanges each)%s%s\n", N, pok ? "same outputs" : "DIFFER",
+           rows[0], err[0].c_str(), err[1].c_str());
   }
   printf("%s\n", fails ? "RESULT: FAIL" : "RESULT: PASS");
   return fails ? 1 : 0;

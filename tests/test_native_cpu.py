@@ -70,6 +70,6 @@ def test_dropin_header_builds_with_gxx(tmp_path):
     exe = tmp_path / "dropin_test"
     r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
                         os.path.join(ROOT, "tests", "native", "dropin_test.cpp"),
-                        "-L" + os.path.dirname(LIB), "-lpcgsq", "-o", str(exe)],
+                        "-L" + os.path.dirname(LIB), "-lpcgsq", "-pthread", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
