@@ -532,7 +532,8 @@ def main() -> int:
         "pcie_inclusive_pinned_OLEs_per_s": N / dt_pinned,
         "preprocessing_gmw": {
             "what": "position adder (Kogge-Stone) + DltSft circuit under GMW for both parties, "
-                    "dealer AND triples; replaces the dealer's DltSft (same outputs)",
+                    "AND triples made from ideal OTs, beta and Beaver rs by Gilboa OT products; "
+                    "replaces the dealer's DltSft, beta and rs (same outputs)",
             "instances": c * c * t * t, "and_gates": c * c * t * t * gmw_and_gates(N, t),
             "deal_ms": deal_ms[True], "dealer_deal_ms": deal_ms[False],
             "and_gates_per_s": c * c * t * t * gmw_and_gates(N, t) / (deal_ms[True] * 1e-3)},

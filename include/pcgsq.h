@@ -343,18 +343,25 @@ int pcgsq_ole_party_outputs(pcgsq_ole_plan* plan, uint64_t* x, uint64_t* z);
  * the dealer runs on the device, asynchronously on the context stream. */
 int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed);
 /* The same deal with the 2PC preprocessing of pcg_ole_impl.h:114-199 run on
- * the device for both parties: alpha = off0 + off1 by the Kogge-Stone adder
- * circuit (bool_circuit/adder.h:21-63) and DltSft by the F_DeltaShiftShare
- * circuit (half_tree_dpf/dpf.cpp:172-240), both under GMW
- * (bool_circuit/circuit.cpp:245-380) with masks and AND triples drawn by the
- * dealer in place of emp::PRG and FerretCOT.  beta and the F_p Beaver
- * triples (Gilboa OT in the reference) stay with the dealer.  The DltSft
- * shares differ from pcgsq_ole_deal's but reconstruct to the same values, so
- * the expansion's outputs are identical. */
+ * the device for both parties:
+ *   alpha = off0 + off1 by the Kogge-Stone adder circuit
+ *   (bool_circuit/adder.h:21-63) and DltSft by the F_DeltaShiftShare circuit
+ *   (half_tree_dpf/dpf.cpp:172-240), both under GMW
+ *   (bool_circuit/circuit.cpp:245-380), each AND triple made from two OTs
+ *   as TripleGen::generate makes it (circuit.cpp:30-85);
+ *   beta = pay0 * pay1 and the F_p Beaver triples' rs by Gilboa OT products
+ *   (TripleGen::zp_multiply and zp_triple, circuit.cpp:91-200).
+ * Masks and the OT senders' randomness come from the dealer in place of
+ * emp::PRG, and the OTs are ideal in place of FerretCOT.  The DltSft, beta
+ * and rs shares differ from pcgsq_ole_deal's but reconstruct to the same
+ * values, so the expansion's outputs are identical. */
 int pcgsq_ole_deal_gmw(pcgsq_ole_plan* plan, uint64_t seed);
 /* Both parties' dealt DltSft shares, [2][c*c*t*t][levels][2^w - 1] blocks
  * (synchronizes). */
 int pcgsq_ole_dealt_dltsft(pcgsq_ole_plan* plan, pcgsq_blk* dltsft);
+/* Both parties' dealt F_p shares, out[8][c*c*t*t] = beta0, beta1, r0, r1,
+ * s0, s1, rs0, rs1 per instance (synchronizes). */
+int pcgsq_ole_dealt_products(pcgsq_ole_plan* plan, uint64_t* out);
 /* Run the whole expansion for both parties on the device-resident inputs
  * (DPF batches, output layer with Beaver opens, step 4), asynchronously on
  * the context stream. */

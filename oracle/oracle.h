@@ -194,15 +194,20 @@ size_t orc_dealer_sizes(const orc_params* p, size_t* n_pos, size_t* n_inst,
                         size_t* n_dlt);
 int orc_dealer(const orc_params* p, uint64_t seed, orc_party_inputs* in0,
                orc_party_inputs* in1, uint64_t* alpha /* [c*c][B] or NULL */);
-/* The dealer's inputs with DltSft produced instead by the GMW preprocessing
- * of pcg_ole_impl.h:114-199 (Kogge-Stone position adder, then the
- * F_DeltaShiftShare circuit of dpf.cpp:172-240), evaluated gate by gate for
- * both parties with dealer masks and AND triples (orc_gmw.c gives the spec).
- * The DltSft shares differ from orc_dealer's; their XOR, and everything
- * else, is the same.  alpha_sh0/1[c*c][B] (or NULL): the adder's XOR shares
- * of alpha. */
+/* The dealer's inputs with DltSft, beta and rs produced instead by the
+ * preprocessing of pcg_ole_impl.h:114-199: the Kogge-Stone position adder
+ * and the F_DeltaShiftShare circuit of dpf.cpp:172-240 evaluated gate by
+ * gate under GMW for both parties, with AND triples made from OTs; beta and
+ * the Beaver triples' rs by Gilboa OT products (orc_gmw.c gives the spec).
+ * The DltSft, beta and rs shares differ from orc_dealer's; what they
+ * reconstruct to, and everything else, is the same.  alpha_sh0/1[c*c][B]
+ * (or NULL): the adder's XOR shares of alpha. */
 int orc_gmw_preprocess(const orc_params* p, uint64_t seed, orc_party_inputs* in0,
                        orc_party_inputs* in1, uint64_t* alpha_sh0, uint64_t* alpha_sh1);
+/* Gilboa's OT product (bool_circuit/circuit.cpp:91-153), product q of
+ * instance g: party 0 holds x, party 1 holds y; *sh0 + *sh1 = x*y mod P. */
+void orc_gilboa(uint64_t x, uint64_t y, uint64_t seed, uint64_t g, uint32_t q, uint64_t* sh0,
+                uint64_t* sh1);
 /* AND ops of the nb-bit Kogge-Stone adder (1 + 2*ceil(log2 nb)). */
 int orc_gmw_adder_ops(int nb);
 /* AND gates of one instance's two circuits (adder of nb bits, DltSft of

@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
         L = _lib
         L.orc_fmul.restype = C.c_uint64
         L.orc_fmul.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_gilboa.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.orc_fpow.restype = C.c_uint64
         L.orc_fpow.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_finv.restype = C.c_uint64
@@ -365,7 +367,8 @@ def dealt_inputs(N, c, t, w=1, seed=1, gmw=False):
     """Both parties' dealt inputs: dict of pos, pay [2][c*t], root [2][ninst][2],
     dltsft [2][ninst][levels][nk][2], beta, r, s, rs [2][ninst], and alpha
     ([ninst], dealer) or alpha_sh ([2][ninst], the GMW adder's XOR shares).
-    gmw=True: DltSft from the GMW preprocessing circuits instead of the dealer."""
+    gmw=True: DltSft from the GMW preprocessing circuits, beta and rs from
+    Gilboa OT products, instead of the dealer."""
     p = Params(N, c, t, w, PRG_CHACHA8)
     npos, ninst, ndlt = C.c_size_t(), C.c_size_t(), C.c_size_t()
     lib().orc_dealer_sizes(C.byref(p), C.byref(npos), C.byref(ninst), C.byref(ndlt))
@@ -389,6 +392,14 @@ def dealt_inputs(N, c, t, w=1, seed=1, gmw=False):
     levels, nk = n // w, (1 << w) - 1
     out["dltsft"] = out["dltsft"].reshape(2, ninst, levels, nk, 2)
     return out
+
+
+def gilboa(x: int, y: int, seed: int, g: int, q: int) -> tuple:
+    """Gilboa's OT product (bool_circuit/circuit.cpp:91-153), product q of
+    instance g: both parties' additive shares of x * y mod P."""
+    s0, s1 = C.c_uint64(), C.c_uint64()
+    lib().orc_gilboa(x, y, seed, g, q, C.byref(s0), C.byref(s1))
+    return s0.value, s1.value
 
 
 def gmw_and_gates(nb: int, w: int, n: int) -> int:

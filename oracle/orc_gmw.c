@@ -10,23 +10,36 @@
  *       61-255): per level i and child k the indicator
  *       prod_c (alpha_{n-w(i+1)+c} xor not k_c) over an AND tree, then 128
  *       AND gates with Delta; outputs stay XOR-shared.
+ *   3b. beta = pay0 * pay1 and the F_p Beaver triples (r, s, rs) by
+ *       Gilboa's OT product (bool_circuit/circuit.cpp:91-200).
  * GMW rules (bool_circuit/circuit.cpp:245-380): an input owned by party o
  * is shared as (x ^ m, m) with o holding x ^ m; XOR is local; NOT flips
  * party 0's share; AND with the triple (a, b, c), c = a & b, opens
  * d = x ^ a, e = y ^ b and gives z_0 = d&e ^ d&b_0 ^ e&a_0 ^ c_0,
  * z_1 = d&b_1 ^ e&a_1 ^ c_1.
- * 3b (Gilboa OT for beta and the F_p Beaver triples) stays with the dealer.
+ * Every AND triple is made from two OTs as TripleGen::generate makes it
+ * (circuit.cpp:30-85): each party draws its a, b; party 0, the OT sender,
+ * draws r, s and offers (r, r ^ a_0) and (s, s ^ b_0); party 1 chooses with
+ * b_1 and a_1.  So c_0 = a_0&b_0 ^ r ^ s and
+ * c_1 = a_1&b_1 ^ (r ^ a_0&b_1) ^ (s ^ b_0&a_1).
+ * The OTs are ideal (the receiver gets the chosen message) where the
+ * reference runs FerretCOT.
  *
- * Randomness (the reference draws masks from emp::PRG and AND triples from
- * FerretCOT): dealer domain 14, word M(g, j) = U(seed, 14, g, j) for global
- * instance g.
+ * Randomness (the reference draws from emp::PRG): dealer domain 14, word
+ * M(g, j) = U(seed, 14, g, j) for global instance g.
  *   j = 0 / 1: masks of party 0's / party 1's adder input (bit i = input i)
  *   j = 2 / 3: masks of party 0's / party 1's alpha-share input of DltSft
  *   j = 4,5 / 6,7: masks of party 0's / party 1's Delta-share input
  *                  (lo, hi words; bit j of the block = input j)
- *   AND op o: a0 = M(8+5o), a1 = M(9+5o), b0 = M(10+5o), b1 = M(11+5o),
- *             c0 = M(12+5o), c1 = ((a0^a1) & (b0^b1)) ^ c0; the gate with
- *             slot (o, bit) uses bit `bit` of each word.
+ *   AND op o: a0, a1, b0, b1, r, s = M(8+6o+0..5); the gate with slot
+ *             (o, bit) uses bit `bit` of each word.
+ * Gilboa (circuit.cpp:91-153): party 0 holds x, party 1 holds y < P; for
+ * bit j < 62 of y party 0 offers (r_j, r_j + x 2^j mod P) with
+ * r_j = U(seed, 15, g, 64q + j) % P for product q of instance g; party 0's
+ * share is P - sum r_j, party 1's the sum of the messages it chose.
+ * Products: q = 0 beta (x = pay0, y = pay1); zp_triple (circuit.cpp:
+ * 156-200) with the dealer's r_s, s_s: q = 1 r0 * s1, q = 2 s0 * r1, and
+ * rs_s = r_s s_s + (share s of q = 1) + (share s of q = 2).
  *   Ops: adder op 0 = the generate ANDs G_i = a_i & b_i (bit i); prefix
  *        step s = 1.. (distance d = 2^(s-1) < nb): op 2s-1 = P_i & G_{i-d},
  *        op 2s = P_i & P_{i-d} (bit i, i >= d).  DltSft (level i, child k,
@@ -102,15 +115,14 @@ static void evaluate(const circuit* c, uint64_t seed, uint64_t g, const uint8_t*
             s1[i] = s1[x->in0];
             break;
         default: {
-            const uint64_t base = 8 + 5 * (uint64_t)x->op;
-            const uint64_t wa0 = M(seed, g, base), wa1 = M(seed, g, base + 1);
-            const uint64_t wb0 = M(seed, g, base + 2), wb1 = M(seed, g, base + 3);
-            const uint64_t wc0 = M(seed, g, base + 4);
-            const uint64_t wc1 = ((wa0 ^ wa1) & (wb0 ^ wb1)) ^ wc0;
+            const uint64_t base = 8 + 6 * (uint64_t)x->op;
             const int b = x->bit;
-            const uint8_t a0 = (wa0 >> b) & 1, a1 = (wa1 >> b) & 1;
-            const uint8_t b0 = (wb0 >> b) & 1, b1 = (wb1 >> b) & 1;
-            const uint8_t c0 = (wc0 >> b) & 1, c1 = (wc1 >> b) & 1;
+            const uint8_t a0 = (M(seed, g, base) >> b) & 1, a1 = (M(seed, g, base + 1) >> b) & 1;
+            const uint8_t b0 = (M(seed, g, base + 2) >> b) & 1, b1 = (M(seed, g, base + 3) >> b) & 1;
+            const uint8_t r = (M(seed, g, base + 4) >> b) & 1, sr = (M(seed, g, base + 5) >> b) & 1;
+            /* the two OTs: party 1 receives r ^ a0&b1 (choice b1), sr ^ b0&a1 (choice a1) */
+            const uint8_t ot1 = (uint8_t)(b1 ? r ^ a0 : r), ot2 = (uint8_t)(a1 ? sr ^ b0 : sr);
+            const uint8_t c0 = (uint8_t)((a0 & b0) ^ r ^ sr), c1 = (uint8_t)((a1 & b1) ^ ot1 ^ ot2);
             const uint8_t d = s0[x->in0] ^ a0 ^ s1[x->in0] ^ a1;
             const uint8_t e = s0[x->in1] ^ b0 ^ s1[x->in1] ^ b1;
             s0[i] = (uint8_t)((d & e) ^ (d & b0) ^ (e & a0) ^ c0);
@@ -199,6 +211,23 @@ static void build_dltsft(circuit* c, int n, int w, int op0) {
 
 int orc_gmw_adder_ops(int nb) { return 1 + 2 * steps_of(nb); }
 
+/* Gilboa's product for instance g, product q: both parties' shares. */
+void orc_gilboa(uint64_t x, uint64_t y, uint64_t seed, uint64_t g, uint32_t q, uint64_t* sh0,
+                uint64_t* sh1) {
+    int nbits = 0;
+    for (uint64_t v = ORC_P - 1; v; v >>= 1) ++nbits;
+    uint64_t rsum = 0, got = 0, xs = x % ORC_P;
+    for (int j = 0; j < nbits; ++j) {
+        const uint64_t r = orc_dealer_u64(seed, 15, g, 64 * (uint64_t)q + (uint64_t)j) % ORC_P;
+        const uint64_t m0 = r, m1 = orc_fadd(r, xs);   /* the sender's two messages */
+        rsum = orc_fadd(rsum, r);
+        got = orc_fadd(got, ((y % ORC_P) >> j) & 1 ? m1 : m0);
+        xs = orc_fadd(xs, xs);
+    }
+    *sh0 = orc_fsub(0, rsum);
+    *sh1 = got;
+}
+
 int orc_gmw_preprocess(const orc_params* p, uint64_t seed, orc_party_inputs* in0,
                        orc_party_inputs* in1, uint64_t* alpha_sh0, uint64_t* alpha_sh1) {
     if (orc_dealer(p, seed, in0, in1, NULL)) return -1;
@@ -272,6 +301,14 @@ int orc_gmw_preprocess(const orc_params* p, uint64_t seed, orc_party_inputs* in0
                         in0->dltsft[off] = x0;
                         in1->dltsft[off] = x1;
                     }
+                /* 3b: beta and the Beaver triple by Gilboa products */
+                orc_gilboa(in0->pay[ii * t + kk], in1->pay[jj * t + ll], seed, g, 0, &in0->beta[g],
+                           &in1->beta[g]);
+                uint64_t c10, c11, c20, c21;
+                orc_gilboa(in0->r[g], in1->s[g], seed, g, 1, &c10, &c11);
+                orc_gilboa(in0->s[g], in1->r[g], seed, g, 2, &c20, &c21);
+                in0->rs[g] = orc_fadd(orc_fadd(orc_fmul(in0->r[g], in0->s[g]), c10), c20);
+                in1->rs[g] = orc_fadd(orc_fadd(orc_fmul(in1->r[g], in1->s[g]), c11), c21);
             }
     free(s0); free(s1); free(sum);
     c_free(&add_c); c_free(&dlt_c);

@@ -29,10 +29,14 @@
 //   r_s = U(seed,6,g,s) % P, s_s = U(seed,7,g,s) % P,
 //   rs0 = U(seed,8,g,0) % P, rs1 = (r0+r1)(s0+s1) - rs0;
 //   public_a[i][j] = U(pub_seed,9,i,j) % P.
-// GMW preprocessing (pcgsq_ole_deal_gmw): the same draws, except that alpha
-// and DltSft come from the parties' Kogge-Stone adder and F_DeltaShiftShare
-// circuits under GMW with masks and AND triples from U(seed,14,g,j) (spec at
-// gmw_and below; oracle/orc_gmw.c restates it gate by gate).
+// Preprocessing as the parties run it (pcgsq_ole_deal_gmw): the same draws,
+// except that
+//   * alpha and DltSft come from the parties' Kogge-Stone adder and
+//     F_DeltaShiftShare circuits under GMW, with masks from U(seed,14,g,j)
+//     and every AND triple made from two OTs (spec at gmw_and below);
+//   * beta and rs come from Gilboa OT products (spec at gilboa below).
+// The OTs are ideal (the receiver gets the chosen message) where the
+// reference runs FerretCOT; oracle/orc_gmw.c restates all of it.
 #include <string.h>
 
 #include <algorithm>
@@ -69,13 +73,18 @@ struct Sh {
 };
 
 // GMW AND (bool_circuit/circuit.cpp:245-256): open d = x ^ a, e = y ^ b;
-// party 0 gets d&e ^ d&b0 ^ e&a0 ^ c0, party 1 d&b1 ^ e&a1 ^ c1.
+// party 0 gets d&e ^ d&b0 ^ e&a0 ^ c0, party 1 d&b1 ^ e&a1 ^ c1.  The
+// triple is made from two OTs as TripleGen::generate (circuit.cpp:30-85)
+// makes it: each party draws its a, b; party 0, the sender, draws r, s and
+// offers (r, r ^ a0) and (s, s ^ b0); party 1 chooses with b1 and a1.  Words
+// of op o: a0, a1, b0, b1, r, s = M(8 + 6o + 0..5).
 __device__ __forceinline__ Sh gmw_and(Sh x, Sh y, uint64_t seed, uint64_t g, uint32_t op) {
-  const uint64_t j = 8 + 5 * (uint64_t)op;
+  const uint64_t j = 8 + 6 * (uint64_t)op;
   const uint64_t a0 = dealer_hd(seed, 14, g, j), a1 = dealer_hd(seed, 14, g, j + 1);
   const uint64_t b0 = dealer_hd(seed, 14, g, j + 2), b1 = dealer_hd(seed, 14, g, j + 3);
-  const uint64_t c0 = dealer_hd(seed, 14, g, j + 4);
-  const uint64_t c1 = ((a0 ^ a1) & (b0 ^ b1)) ^ c0;
+  const uint64_t r = dealer_hd(seed, 14, g, j + 4), s = dealer_hd(seed, 14, g, j + 5);
+  const uint64_t ot1 = r ^ (a0 & b1), ot2 = s ^ (b0 & a1);   // what party 1 receives
+  const uint64_t c0 = (a0 & b0) ^ r ^ s, c1 = (a1 & b1) ^ ot1 ^ ot2;
   const uint64_t d = x.s0 ^ a0 ^ x.s1 ^ a1, e = y.s0 ^ b0 ^ y.s1 ^ b1;
   return Sh{(d & e) ^ (d & b0) ^ (e & a0) ^ c0, (d & b1) ^ (e & a1) ^ c1};
 }
@@ -144,6 +153,22 @@ __device__ void gmw_dltsft(Sh alpha, Blk d0, Blk d1, int n, int w, int aops, uin
       out0[i * nk + k] = Blk{lo.s0, hi.s0};
       out1[i * nk + k] = Blk{lo.s1, hi.s1};
     }
+}
+
+// Gilboa's OT product (TripleGen::zp_multiply, circuit.cpp:91-153): party 0,
+// the sender, holds x; party 1 holds y < P.  For bit j < 62 of y the sender
+// offers (r_j, r_j + x 2^j mod P), r_j = U(seed, 15, g, 64q + j) % P for
+// product q of instance g, and the receiver takes the one its bit selects.
+// Shares: party 0 P - sum r_j, party 1 the sum of what it took.
+__device__ Sh gilboa(uint64_t x, uint64_t y, uint64_t seed, uint64_t g, uint32_t q) {
+  uint64_t rsum = 0, got = 0, xs = x;
+  for (int j = 0; j < 62; ++j) {
+    const uint64_t r = dealer_hd(seed, 15, g, 64 * (uint64_t)q + (uint64_t)j) % kP;
+    rsum = fadd(rsum, r);
+    got = fadd(got, (y >> j) & 1 ? fadd(r, xs) : r);
+    xs = fadd(xs, xs);
+  }
+  return Sh{fsub(0, rsum), got};
 }
 
 // AND ops of the nb-bit adder: the generate layer and two per prefix step.
@@ -249,20 +274,31 @@ __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __rest
     }
   }
   const uint64_t pay0 = pay[(size_t)ii * t + kk], pay1 = pay[((size_t)c + jj) * t + ll];
-  const uint64_t beta = fmul(pay0, pay1);
   uint64_t* BETA = bt;
   uint64_t* R = bt + 2 * ni;
   uint64_t* S = bt + 4 * ni;
   uint64_t* RS = bt + 6 * ni;
-  BETA[g] = dealer_hd(seed, 5, G, 0) % kP;
-  BETA[ni + g] = fsub(beta, BETA[g]);
-  R[g] = dealer_hd(seed, 6, G, 0) % kP;
-  R[ni + g] = dealer_hd(seed, 6, G, 1) % kP;
-  S[g] = dealer_hd(seed, 7, G, 0) % kP;
-  S[ni + g] = dealer_hd(seed, 7, G, 1) % kP;
-  const uint64_t rs = fmul(fadd(R[g], R[ni + g]), fadd(S[g], S[ni + g]));
-  RS[g] = dealer_hd(seed, 8, G, 0) % kP;
-  RS[ni + g] = fsub(rs, RS[g]);
+  const uint64_t r0 = dealer_hd(seed, 6, G, 0) % kP, r1 = dealer_hd(seed, 6, G, 1) % kP;
+  const uint64_t s0 = dealer_hd(seed, 7, G, 0) % kP, s1 = dealer_hd(seed, 7, G, 1) % kP;
+  R[g] = r0;
+  R[ni + g] = r1;
+  S[g] = s0;
+  S[ni + g] = s1;
+  if (gmw) {
+    // beta by a Gilboa product (pcg_ole_impl.h:158-171); the Beaver triple's
+    // rs by zp_triple's two cross products (circuit.cpp:156-200)
+    const Sh b = gilboa(pay0, pay1, seed, G, 0);
+    BETA[g] = b.s0;
+    BETA[ni + g] = b.s1;
+    const Sh x1 = gilboa(r0, s1, seed, G, 1), x2 = gilboa(s0, r1, seed, G, 2);
+    RS[g] = fadd(fadd(fmul(r0, s0), x1.s0), x2.s0);
+    RS[ni + g] = fadd(fadd(fmul(r1, s1), x1.s1), x2.s1);
+  } else {
+    BETA[g] = dealer_hd(seed, 5, G, 0) % kP;
+    BETA[ni + g] = fsub(fmul(pay0, pay1), BETA[g]);
+    RS[g] = dealer_hd(seed, 8, G, 0) % kP;
+    RS[ni + g] = fsub(fmul(fadd(r0, r1), fadd(s0, s1)), RS[g]);
+  }
 }
 
 __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t* pay, int c, int t,
@@ -875,6 +911,19 @@ int pcgsq_ole_dealt_dltsft(pcgsq_ole_plan* plan, pcgsq_blk* dltsft) {
                              plan->ctx->stream),
               "ole dltsft");
     hip_check(hipStreamSynchronize(plan->ctx->stream), "ole dltsft");
+  });
+}
+
+int pcgsq_ole_dealt_products(pcgsq_ole_plan* plan, uint64_t* out) {
+  return guarded([&] {
+    require(plan && out, "ole_dealt_products: null argument");
+    require(plan->dealt, "ole_dealt_products: nothing dealt");
+    CtxGuard g(plan->ctx);
+    const size_t n = (size_t)8 * plan->d.ninst;
+    hip_check(hipMemcpyAsync(out, plan->bt.ptr, n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             plan->ctx->stream),
+              "ole products");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole products");
   });
 }
 

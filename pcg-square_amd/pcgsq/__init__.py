@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
     L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
     L.pcgsq_ole_deal_gmw.argtypes = [vp, C.c_uint64]
     L.pcgsq_ole_dealt_dltsft.argtypes = [vp, C.c_void_p]
+    L.pcgsq_ole_dealt_products.argtypes = [vp, C.c_void_p]
     L.pcgsq_ole_run.argtypes = [vp]
     L.pcgsq_ole_run_blocks.argtypes = [vp]
     L.pcgsq_ole_outputs.argtypes = [vp, u64p, u64p, u64p, u64p]
@@ -511,6 +512,12 @@ class OlePlan:
         out = np.empty((2, self.c * self.c * self.t * self.t, levels, nk, 2), dtype=np.uint64)
         _check(lib().pcgsq_ole_dealt_dltsft(self.h, out.ctypes.data))
         return out
+
+    def dealt_products(self) -> dict:
+        """Both parties' dealt F_p shares: beta, r, s, rs, each [2][c*c*t*t]."""
+        out = np.empty((4, 2, self.c * self.c * self.t * self.t), dtype=np.uint64)
+        _check(lib().pcgsq_ole_dealt_products(self.h, out.ctypes.data))
+        return dict(zip(("beta", "r", "s", "rs"), out))
 
     def run(self) -> None:
         _check(lib().pcgsq_ole_run(self.h))

@@ -469,18 +469,25 @@ def test_triples_full_size_are_triples(ctx, pq):
                                      (1 << 12, 2, 16, 3), (1 << 13, 3, 8, 1)])
 def test_gmw_preprocessing_matches_oracle(ctx, pq, N, c, t, w):
     """Both parties' DltSft shares from the device's bit-sliced GMW
-    evaluation equal the oracle's gate-by-gate evaluation, bit for bit, and
-    reconstruct to the dealer's DltSft."""
-    exp = O.dealt_inputs(N, c, t, w, seed=41, gmw=True)["dltsft"]
-    dealer = O.dealt_inputs(N, c, t, w, seed=41)["dltsft"]
+    evaluation (AND triples made from OTs) equal the oracle's gate-by-gate
+    evaluation, bit for bit, and reconstruct to the dealer's DltSft; beta and
+    the Beaver triples' shares from the device's Gilboa products equal the
+    oracle's."""
+    og = O.dealt_inputs(N, c, t, w, seed=41, gmw=True)
+    od = O.dealt_inputs(N, c, t, w, seed=41)
     with pq.OlePlan(ctx, N, c, t, w) as pl:
         pl.deal(41, gmw=True)
         got = pl.dealt_dltsft()
+        prod = pl.dealt_products()
         pl.deal(41)
         got_dealer = pl.dealt_dltsft()
-    assert np.array_equal(got, exp)
-    assert np.array_equal(got_dealer, dealer)
+        prod_dealer = pl.dealt_products()
+    assert np.array_equal(got, og["dltsft"])
+    assert np.array_equal(got_dealer, od["dltsft"])
     assert np.array_equal(got[0] ^ got[1], got_dealer[0] ^ got_dealer[1])
+    for k in ("beta", "r", "s", "rs"):
+        assert np.array_equal(prod[k], og[k]), k
+        assert np.array_equal(prod_dealer[k], od[k]), k
 
 
 @pytest.mark.parametrize("N,c,t,w,prg", [(1 << 12, 2, 16, 1, 1), (1 << 12, 2, 16, 3, 1),

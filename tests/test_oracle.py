@@ -321,8 +321,31 @@ def test_gmw_preprocessing_reconstructs(N, c, t, w):
     assert np.array_equal(g["dltsft"][0] ^ g["dltsft"][1], d["dltsft"][0] ^ d["dltsft"][1])
     assert not np.array_equal(g["dltsft"][0], d["dltsft"][0])
     assert not np.array_equal(g["alpha_sh"][0], d["alpha"])
-    for k in ("pos", "pay", "root", "beta", "r", "s", "rs"):
+    # beta and rs by Gilboa products: fresh shares of the same field values
+    # (beta = pay0 * pay1, rs = (r0 + r1)(s0 + s1), the dealer's closed forms)
+    for k in ("beta", "rs"):
+        assert np.all(g[k] < P), k
+        assert [(int(a) + int(b)) % P for a, b in zip(*g[k])] == \
+            [(int(a) + int(b)) % P for a, b in zip(*d[k])], k
+        assert not np.array_equal(g[k][0], d[k][0]), k
+    for k in ("pos", "pay", "root", "r", "s"):
         assert np.array_equal(g[k], d[k]), k
+
+
+def test_gilboa_product_shares():
+    """Gilboa's OT product (bool_circuit/circuit.cpp:91-153): the two shares
+    are in [0, P) and add up to x * y mod P, at the edges (zero, one, P - 1,
+    y with bit 61 set, the top bit the 62-bit decomposition must reach) and
+    on random operands; different products of one instance use different
+    sender randomness."""
+    rng = np.random.default_rng(5)
+    cases = [(0, 0), (1, 1), (P - 1, P - 1), (7, P - 1), (P - 1, 1 << 61), (1 << 61, 3)]
+    cases += [(int(a), int(b)) for a, b in rng.integers(0, P, (64, 2), dtype=np.uint64)]
+    for q, (x, y) in enumerate(cases):
+        s0, s1 = O.gilboa(x, y, 11, 5, q % 3)
+        assert s0 < P and s1 < P
+        assert (s0 + s1) % P == x * y % P, (x, y)
+    assert O.gilboa(3, 4, 11, 5, 0)[0] != O.gilboa(3, 4, 11, 5, 1)[0]
 
 
 def test_gmw_and_gate_counts():
