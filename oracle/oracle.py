@@ -137,7 +137,7 @@ def lib() -> C.CDLL:
 
 
 def ref():
-    """Compiled reference headers, or None when not built (GPU box)."""
+    """Compiled reference headers, or None where they are not built (no reference sources)."""
     global _ref
     if _ref is None:
         so = os.path.join(HERE, "_ref", "libpcgref.so")

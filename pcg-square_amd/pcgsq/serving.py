@@ -1,7 +1,7 @@
-"""Open-loop load driver for the expansion server (pcgsq.Server).
+"""Open-loop load generator for the expansion server (pcgsq.Server).
 
 Requests arrive as a Poisson process of a fixed rate; each draws its ring
-dimension uniformly from `sizes` and carries a fresh seed.  The driver submits
+dimension uniformly from `sizes` and carries a fresh seed.  The generator submits
 every request at its arrival time (so the server's submit timestamp is the
 arrival time), releases outputs as requests finish, and returns one record
 per request with the server-side times.  Latency = completion - arrival,

@@ -60,20 +60,20 @@ static int fails = 0;
 struct Channel {
   std::mutex m;
   std::condition_variable cv;
-  std::vector<std::vector<uint8_t>> box[2];   // box[p]: messages waiting for party p
+  std::vector<std::vector<uint8_t>> inbox[2];   // inbox[p]: messages waiting for party p
   bool aborted = false;
   void send(int to, const void* src, size_t n) {
     std::lock_guard<std::mutex> lk(m);
-    box[to].emplace_back((const uint8_t*)src, (const uint8_t*)src + n);
+    inbox[to].emplace_back((const uint8_t*)src, (const uint8_t*)src + n);
     cv.notify_all();
   }
   void recv(int me, void* dst, size_t n) {
     std::unique_lock<std::mutex> lk(m);
-    if (!cv.wait_for(lk, std::chrono::seconds(60), [&] { return aborted || !box[me].empty(); }) ||
+    if (!cv.wait_for(lk, std::chrono::seconds(60), [&] { return aborted || !inbox[me].empty(); }) ||
         aborted)
       throw std::runtime_error("channel closed");
-    std::vector<uint8_t> msg = std::move(box[me].front());
-    box[me].erase(box[me].begin());
+    std::vector<uint8_t> msg = std::move(inbox[me].front());
+    inbox[me].erase(inbox[me].begin());
     if (msg.size() != n) throw std::runtime_error("message size mismatch");
     memcpy(dst, msg.data(), n);
   }
