@@ -603,6 +603,32 @@ class OleParty(OlePlan):
         return x, z
 
 
+def dist_exchange(peer: int, group=None):
+    """An OleParty exchange over torch.distributed point-to-point messages,
+    for one party per process as the reference's 2PC runs: the pair of
+    emp::NetIO calls of pcg_ole_impl.h:247-260 / dpf.cpp:539-551, the lower
+    rank sending first.  The messages are CPU uint8 tensors, so the process
+    group is gloo (TCP between the parties' hosts, or between two processes
+    of one host)."""
+    import torch
+    import torch.distributed as dist
+
+    me = dist.get_rank(group)
+
+    def exchange(phase, level, batch, mine):
+        out = torch.from_numpy(np.ascontiguousarray(mine))
+        got = torch.empty_like(out)
+        if me < peer:
+            dist.send(out, peer, group=group)
+            dist.recv(got, peer, group=group)
+        else:
+            dist.recv(got, peer, group=group)
+            dist.send(out, peer, group=group)
+        return got.numpy()
+
+    return exchange
+
+
 class CommTrace:
     """Rows of the reference's comm_trace.h CSV (party, context, phase, level,
     batch, send_bytes, recv_bytes, t_begin_ns, t_mid_ns, t_end_ns,

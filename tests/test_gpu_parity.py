@@ -1,5 +1,7 @@
 """HIP path vs the CPU oracle, through the C ABI (libpcgsq.so).  Needs an
 MI355X.  Every comparison is bit-exact (all arithmetic is integer)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -397,6 +399,67 @@ def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, gmw):
         got = [(r[2], r[3], r[4], r[5]) for r in traces[p].rows]
         assert sorted(got) == sorted(rows)
         assert all(r[5] == r[6] for r in traces[p].rows)
+
+
+_PARTY_PROCESS = """
+import os, sys
+import numpy as np
+sys.path.insert(0, os.environ["PCGSQ_PKG"])
+import torch.distributed as dist
+import pcgsq as pq
+rank = int(os.environ["RANK"])
+N, c, t, seed, pub = (int(v) for v in os.environ["PCGSQ_CASE"].split(","))
+print("party", rank, "N", N, "c", c, "t", t, flush=True)
+dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MASTER_PORT"],
+                        rank=rank, world_size=2)
+with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, rank, pub_seed=pub) as op:
+    op.deal(seed)
+    op.run(pq.dist_exchange(1 - rank))
+    x, z = op.outputs()
+np.save(os.path.join(os.environ["PCGSQ_OUT"], "x%d.npy" % rank), x)
+np.save(os.path.join(os.environ["PCGSQ_OUT"], "z%d.npy" % rank), z)
+dist.destroy_process_group()
+"""
+
+
+def test_ole_parties_as_two_processes_over_gloo(ctx, pq, tmp_path):
+    """The reference's deployment: one party per OS process, every exchange
+    over a network transport (torch.distributed gloo on 127.0.0.1 in place of
+    emp::NetIO), both processes on this GPU.  Each party's (x, z) equals the
+    both-party plan's."""
+    import socket
+    import subprocess
+    import sys
+
+    N, c, t, seed, pub = 1 << 13, 2, 16, 21, 3
+    with pq.OlePlan(ctx, N, c, t, 1, pq.PRG_CHACHA8, pub_seed=pub) as pl:
+        pl.deal(seed)
+        pl.run()
+        expect = pl.outputs()
+    with socket.socket() as so:
+        so.bind(("127.0.0.1", 0))
+        port = so.getsockname()[1]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    procs = []
+    for rank in (0, 1):
+        env = dict(os.environ, RANK=str(rank), MASTER_PORT=str(port),
+                   PCGSQ_PKG=os.path.join(root, "pcg-square_amd"), PCGSQ_OUT=str(tmp_path),
+                   PCGSQ_CASE=f"{N},{c},{t},{seed},{pub}")
+        procs.append(subprocess.Popen([sys.executable, "-c", _PARTY_PROCESS], env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    logs = []
+    for p in procs:
+        try:
+            logs.append(p.communicate(timeout=120)[0])
+        except subprocess.TimeoutExpired:
+            for q in procs:
+                q.kill()
+            raise
+    for p, log in zip(procs, logs):
+        assert p.returncode == 0, log
+    for rank in (0, 1):
+        assert np.array_equal(np.load(tmp_path / f"x{rank}.npy"), expect[2 * rank]), f"x{rank}"
+        assert np.array_equal(np.load(tmp_path / f"z{rank}.npy"), expect[2 * rank + 1]), f"z{rank}"
 
 
 def test_ole_outputs_into_pinned_arrays(ctx, pq):

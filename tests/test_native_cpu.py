@@ -73,3 +73,53 @@ def test_dropin_header_builds_with_gxx(tmp_path):
                         "-L" + os.path.dirname(LIB), "-lpcgsq", "-pthread", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+_EXCHANGE_PROCESS = """
+import os, sys
+import numpy as np
+sys.path.insert(0, os.environ["PCGSQ_PKG"])
+import torch.distributed as dist
+import pcgsq as pq
+rank = int(os.environ["RANK"])
+dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MASTER_PORT"],
+                        rank=rank, world_size=2)
+ex = pq.dist_exchange(1 - rank)
+for k, n in enumerate((16, 4096, 1 << 20)):
+    mine = np.full(n, 17 * rank + k, dtype=np.uint8)
+    got = ex("dpf_gen", k, 1, mine)
+    assert got.dtype == np.uint8 and got.shape == (n,), (got.dtype, got.shape)
+    assert np.all(got == 17 * (1 - rank) + k), (rank, k)
+print("ok", rank, flush=True)
+dist.destroy_process_group()
+"""
+
+
+def test_dist_exchange_swaps_between_two_processes():
+    """pcgsq.dist_exchange, the OleParty exchange over torch.distributed
+    (gloo, CPU tensors): two processes on 127.0.0.1 each receive exactly the
+    other's bytes, in order, for messages of 16 B to 1 MiB."""
+    import socket
+    import subprocess
+    import sys
+
+    with socket.socket() as so:
+        so.bind(("127.0.0.1", 0))
+        port = so.getsockname()[1]
+    procs = []
+    for rank in (0, 1):
+        env = dict(os.environ, RANK=str(rank), MASTER_PORT=str(port),
+                   PCGSQ_PKG=os.path.join(ROOT, "pcg-square_amd"))
+        procs.append(subprocess.Popen([sys.executable, "-c", _EXCHANGE_PROCESS], env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    logs = []
+    for p in procs:
+        try:
+            logs.append(p.communicate(timeout=120)[0])
+        except subprocess.TimeoutExpired:
+            for q in procs:
+                q.kill()
+            raise
+    for rank, (p, log) in enumerate(zip(procs, logs)):
+        assert p.returncode == 0, log
+        assert f"ok {rank}" in log.splitlines(), log
