@@ -299,7 +299,72 @@ def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
                         "through a queue pair; both on this one GPU",
             "ms": ms, "OLEs_per_s": N / (ms * 1e-3),
             "exchanges_per_party_per_run": stats[0]["exchanges"] // (reps + 1),
-            "bytes_sent_per_party_per_run": stats[0]["bytes"] // (reps + 1)}
+            "bytes_sent_per_party_per_run": stats[0]["bytes"] // (reps + 1),
+            "as_os_processes_over_gloo": bench_two_processes(N, c, t, reps)}
+
+
+_PARTY_PROCESS = """
+import json, os, sys, time
+sys.path.insert(0, os.environ["PCGSQ_PKG"])
+import torch.distributed as dist
+import pcgsq
+rank = int(os.environ["RANK"])
+N, c, t, reps = (int(v) for v in os.environ["PCGSQ_CASE"].split(","))
+dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MASTER_PORT"],
+                        rank=rank, world_size=2)
+ex = pcgsq.dist_exchange(1 - rank)
+times = []
+with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, rank) as op:
+    op.deal(1)
+    for k in range(reps + 1):   # the first round warms up
+        dist.barrier()
+        t0 = time.perf_counter()
+        op.run(ex)
+        if k:
+            times.append(time.perf_counter() - t0)
+    op.outputs()
+dist.barrier()
+if rank == 0:
+    print("RESULT " + json.dumps(times), flush=True)
+dist.destroy_process_group()
+"""
+
+
+def bench_two_processes(N: int, c: int, t: int, reps: int = 3) -> dict:
+    """The 2PC expansion as the reference deploys it: one OS process per
+    party, every exchange over a network transport (torch.distributed gloo
+    on 127.0.0.1, pcgsq.dist_exchange, in place of emp::NetIO), both
+    processes on this GPU."""
+    import socket
+    import subprocess
+    with socket.socket() as so:
+        so.bind(("127.0.0.1", 0))
+        port = so.getsockname()[1]
+    here = os.path.dirname(os.path.abspath(__file__))
+    procs = []
+    for rank in (0, 1):
+        env = dict(os.environ, RANK=str(rank), MASTER_PORT=str(port),
+                   PCGSQ_PKG=os.path.join(here, "pcg-square_amd"),
+                   PCGSQ_CASE=f"{N},{c},{t},{reps}")
+        procs.append(subprocess.Popen([sys.executable, "-c", _PARTY_PROCESS], env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    logs = []
+    for p in procs:
+        try:
+            logs.append(p.communicate(timeout=240)[0])
+        except subprocess.TimeoutExpired:
+            for q in procs:
+                q.kill()
+            return {"error": "timed out"}
+    res = [ln for ln in logs[0].splitlines() if ln.startswith("RESULT ")]
+    if any(p.returncode for p in procs) or not res:
+        return {"error": (logs[0] + logs[1])[-400:]}
+    times = json.loads(res[0][len("RESULT "):])
+    ms = 1e3 * sum(times) / len(times)
+    return {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
+                        "as two OS processes, exchanges over torch.distributed gloo (TCP, "
+                        "127.0.0.1); both on this one GPU",
+            "ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
 
 
 def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
