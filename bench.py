@@ -14,7 +14,8 @@ request per launch, the roofline of the dominant kernel (the DPF leaf level
 fused with the ChaCha8 out-hash and the output layer's gather, VALU-bound),
 the NTT passes' HBM roofline, and the CPU oracle timed on this host; then
 the other workloads (--parts): batched NTTs, batched DPF keys, DPF arity,
-the expansion as two party processes, and the continuous-batching server.
+the expansion as two parties (threads, and OS processes over gloo), and the
+continuous-batching server.
 """
 from __future__ import annotations
 
@@ -256,7 +257,8 @@ def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
 def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
     """The 2PC expansion as two party processes run it (pcgsq_ole_party_*):
     two threads, each with its own context and plan for one party, every
-    exchange through an in-memory queue pair; both parties share this one
+    exchange through an in-memory queue pair; then the same as two OS
+    processes over gloo (bench_two_processes).  Both parties share this one
     GPU (with a party per GPU each would have a GPU of its own)."""
     import queue
     import threading
