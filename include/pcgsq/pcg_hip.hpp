@@ -275,9 +275,10 @@ class OleExpander {
     return t;
   }
 
-  // gmw: run the 2PC preprocessing circuits (position adder, DltSft) under
-  // GMW on the device instead of taking DltSft from the dealer (pcg_ole_impl.h:
-  // 114-199); the outputs are the same.
+  // gmw: run the 2PC preprocessing on the device instead of taking DltSft,
+  // beta and rs from the dealer (pcg_ole_impl.h:114-199): the position adder
+  // and DltSft circuits under GMW with AND triples made from OTs, and Gilboa
+  // OT products for beta and the Beaver triples; the outputs are the same.
   Output expand(uint64_t seed, bool gmw = false) {
     Output o;
     o.x0.resize(N_); o.z0.resize(N_); o.x1.resize(N_); o.z1.resize(N_);
@@ -330,8 +331,9 @@ class OleParty {
   OleParty(const OleParty&) = delete;
   OleParty& operator=(const OleParty&) = delete;
 
-  // gmw: take this party's DltSft share from the preprocessing circuits run
-  // under GMW (pcg_ole_impl.h:114-199) instead of from the dealer.
+  // gmw: take this party's DltSft, beta and rs shares from the preprocessing
+  // run on the device (GMW circuits, Gilboa products; pcg_ole_impl.h:114-199)
+  // instead of from the dealer.
   Output expand(uint64_t seed, const ExchangeFn& exchange, bool gmw = false) {
     detail::check(gmw ? pcgsq_ole_deal_gmw(plan_, seed) : pcgsq_ole_deal(plan_, seed));
     detail::check(pcgsq_ole_party_run(plan_, tramp, const_cast<ExchangeFn*>(&exchange)));

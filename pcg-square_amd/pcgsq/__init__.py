@@ -501,8 +501,10 @@ class OlePlan:
 
     def deal(self, seed: int, gmw: bool = False) -> None:
         """Deal the inputs of `seed`; gmw=True runs the parties' preprocessing
-        circuits (position adder, DltSft) under GMW on the device instead of
-        taking DltSft from the dealer (same outputs)."""
+        on the device (position adder and DltSft circuits under GMW with AND
+        triples made from OTs, Gilboa products for beta and the Beaver
+        triples) instead of taking DltSft, beta and rs from the dealer (same
+        outputs)."""
         _check((lib().pcgsq_ole_deal_gmw if gmw else lib().pcgsq_ole_deal)(self.h, seed))
 
     def dealt_dltsft(self) -> np.ndarray:
