@@ -401,6 +401,73 @@ def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, gmw):
         assert all(r[5] == r[6] for r in traces[p].rows)
 
 
+def test_ole_party_plans_reusable_after_a_failed_exchange(ctx, pq):
+    """An exchange that fails mid-protocol (a dropped connection) makes both
+    parties' runs raise; the same plans then deal and run again, and each
+    party's (x, z) equals the both-party plan's."""
+    import queue
+    import threading
+
+    N, c, t, seed, pub = 1 << 13, 2, 16, 9, 2
+    with pq.OlePlan(ctx, N, c, t, 1, pq.PRG_CHACHA8, pub_seed=pub) as pl:
+        pl.deal(seed)
+        pl.run()
+        expect = pl.outputs()
+    inbox = (queue.Queue(), queue.Queue())
+    go = [threading.Barrier(2) for _ in range(3)]
+    failed, res, errs = [False, False], [None, None], []
+
+    def party(p):
+        try:
+            with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, p, pub_seed=pub) as op:
+                calls = [0]
+
+                def flaky(phase, level, batch, mine):
+                    calls[0] += 1
+                    if p == 0 and calls[0] == 3:
+                        raise ConnectionError("link dropped")
+                    inbox[1 - p].put(mine)
+                    peer = inbox[p].get(timeout=60)
+                    if peer is None:
+                        raise ConnectionError("peer aborted")
+                    return peer
+
+                def good(phase, level, batch, mine):
+                    inbox[1 - p].put(mine)
+                    return inbox[p].get(timeout=60)
+
+                go[0].wait()
+                op.deal(seed)
+                try:
+                    op.run(flaky)
+                except Exception:
+                    failed[p] = True
+                    inbox[1 - p].put(None)   # tell the peer
+                go[1].wait()
+                while not inbox[p].empty():  # drop what the failed round left
+                    inbox[p].get_nowait()
+                go[2].wait()
+                op.deal(seed)
+                op.run(good)
+                res[p] = op.outputs()
+        except Exception as e:  # reported below
+            errs.append(repr(e))
+            for b in go:
+                b.abort()
+            inbox[1 - p].put(None)
+
+    th = [threading.Thread(target=party, args=(p,)) for p in (0, 1)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=180)
+    assert not errs, errs
+    assert failed == [True, True]
+    for p in (0, 1):
+        assert np.array_equal(res[p][0], expect[2 * p]), f"x of party {p}"
+        assert np.array_equal(res[p][1], expect[2 * p + 1]), f"z of party {p}"
+
+
 _PARTY_PROCESS = """
 import os, sys
 import numpy as np
