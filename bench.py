@@ -66,12 +66,40 @@ def gmw_and_gates(N: int, t: int, w: int = 1) -> int:
     return s + (n // w) * ((1 << w) - 1) * (w - 1 + 128)
 
 
+def host_cpus() -> dict:
+    """CPUs of this host as the OS reports them: `nproc` (the CPUs this
+    process may run on), os.cpu_count() (all online CPUs) and the
+    OMP_NUM_THREADS share the job was given."""
+    import subprocess
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = len(os.sched_getaffinity(0))
+    return {"nproc": nproc, "cpu_count": os.cpu_count(),
+            "omp_num_threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or None}
+
+
+def cpu_baseline_full(N: int, c: int, t: int, threads: int) -> dict:
+    """The CPU oracle (a C restatement of CPU_baseline's local compute: per-node
+    ChaCha8, u128 %P, radix-2 NTT with a cached plan) running one whole
+    expansion for both parties, dealt inputs included, on `threads` OpenMP
+    threads: the same work as one GPU step, timed in full."""
+    import oracle as O
+    t0 = time.perf_counter()
+    O.pcg_ole_expand(N, c, t, 1, O.PRG_CHACHA8, seed=1, pub_seed=0, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": N / dt, "unit": "OLEs/s", "cores": threads, "kind": "port",
+            "host": host_cpus(),
+            "sample": (f"one full expansion (both parties: {c * c} DPF blocks + output layer + "
+                       f"{c + 2 * c * c} poly-muls of step 4) by the oracle (C, u128 %P, per-node "
+                       f"ChaCha8) on {threads} OpenMP threads: {dt:.1f} s")}
+
+
 def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) -> dict:
-    """The CPU oracle (a C restatement of CPU_baseline's local compute) on a
-    bounded sample: one DPF block of the c^2 (both parties, DPF + output
-    layer) and the c + 2c^2 poly-muls of step 4 (or `polymul_sample` of them,
-    scaled), each with `threads` OpenMP threads; the full expansion is c^2
-    blocks + step 4."""
+    """The CPU oracle on a bounded sample: one DPF block of the c^2 (both
+    parties, DPF + output layer) and `polymul_sample` of the c + 2c^2
+    poly-muls of step 4 (all of them if 0), each with `threads` OpenMP
+    threads, scaled to the whole expansion (c^2 blocks + step 4)."""
     import numpy as np
 
     import oracle as O
@@ -89,10 +117,11 @@ def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) 
     full = c * c * t_block + t_step4
     pm = f"{npm} poly-muls" if ns == npm else f"{npm} poly-muls (timed {ns}, scaled)"
     return {"value": N / full, "unit": "OLEs/s", "cores": threads, "kind": "port",
+            "host": host_cpus(),
             "sample": (f"oracle (C, u128 %P, per-node ChaCha8) on {threads} host threads: 1 of "
                        f"{c * c} DPF blocks both parties ({t_block:.2f} s) + {pm} of "
                        f"step 4 ({t_step4:.2f} s); expansion = {c * c} x block + step 4 = "
-                       f"{full:.1f} s")}
+                       f"{full:.1f} s (extrapolated)")}
 
 
 NTT_BATCH_PROFILE = os.path.join(ROOT, "profiles", "ntt_batch_256_uneven_2e12_2e20.md")
@@ -535,7 +564,8 @@ def main() -> int:
         zones = ctx.profile_read()
         ctx.profile(False)
         ceiling = ctx.probe_chacha8()
-        copy_bps = ctx.probe_copy(1 << 30)
+        hbm = ctx.probe_hbm(1 << 30)
+        copy_bps = hbm["copy"]
         ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
         arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
@@ -610,6 +640,8 @@ def main() -> int:
         "tree_prg_rate_Gblocks_per_s": tree_blocks / (tree_ms * 1e-3) / 1e9,
         "chacha8_ceiling_Gblocks_per_s": ceiling / 1e9,
         "hbm_copy_GBps": copy_bps / 1e9,
+        "hbm_probe_GBps": {"copy": hbm["copy"] / 1e9, "read": hbm["read"] / 1e9,
+                           "write": hbm["write"] / 1e9, "copy_shape": hbm["copy_shape"]},
         "batched_ntt": ntt_batch,
         "batched_dpf": dpf_batch,
         "arity": arity,
@@ -617,8 +649,10 @@ def main() -> int:
         "served": served,
     }
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(N, c, t, threads)
+        # the host share this job was given (OMP_NUM_THREADS), else every CPU
+        # the process may run on; `host` records nproc and the online count
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        out["cpu_baseline"] = cpu_baseline_full(N, c, t, threads)
         # the reference-faithful mode: one thread per party, both parties
         out["cpu_baseline_one_thread_per_party"] = cpu_baseline(N, c, t, 2, polymul_sample=4)
         out["cpu_published_context"] = ("CPU_baseline 2PC at N=2^20, c=4, t=16 on a Xeon Gold "

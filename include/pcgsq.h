@@ -84,8 +84,15 @@ int pcgsq_probe_chacha8(pcgsq_ctx* ctx, int iters, double* blocks_per_s);
 int pcgsq_field_op(pcgsq_ctx* ctx, int op, size_t n, const uint64_t* a, const uint64_t* b,
                    const uint64_t* c, uint64_t* out);
 /* Plain device copy of `bytes` (read + write counted): achievable HBM
- * bytes/s, and the known-size kernel that HBM counters are scaled by. */
+ * bytes/s (the best of the shapes pcgsq_probe_hbm sweeps), and the
+ * known-size kernel that HBM counters are scaled by. */
 int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s);
+/* HBM probes over `bytes`: copy (read + write counted; best over 1..8
+ * 16-byte loads in flight per lane, plain or non-temporal, 2..32 workgroups
+ * per CU; copy_shape[3] = loads per lane, non-temporal, workgroups per CU),
+ * read-only and write-only bytes/s.  Any output pointer may be NULL. */
+int pcgsq_probe_hbm(pcgsq_ctx* ctx, size_t bytes, double* copy_bps, double* read_bps,
+                    double* write_bps, int* copy_shape);
 /* Device memory for ctypes callers that hold their own device-resident
  * buffers (no counterpart in the reference, whose plugin only takes host
  * pointers). */

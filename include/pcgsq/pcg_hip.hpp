@@ -21,6 +21,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <exception>
 #include <functional>
 #include <mutex>
 #include <stdexcept>
@@ -69,15 +70,31 @@ inline void check(int rc) {
   if (rc != PCGSQ_OK) throw std::runtime_error(std::string("pcg_hip: ") + pcgsq_last_error());
 }
 
+// The caller's exchange and the exception it threw: the trampoline keeps the
+// exception, the C call returns PCGSQ_ECALLBACK, and the wrapper rethrows the
+// exchange's own exception unchanged (as the reference's NetIO errors reach
+// its caller), not a generic runtime_error.
+struct ExchangeCall {
+  const DpfExchangeFn* fn;
+  std::exception_ptr err;
+};
+
 inline int exchange_tramp(void* user, int /*level*/, const pcgsq_blk* mine, pcgsq_blk* peer,
                           size_t count) {
+  ExchangeCall* x = static_cast<ExchangeCall*>(user);
   try {
-    (*static_cast<const DpfExchangeFn*>(user))(reinterpret_cast<const DpfBlk*>(mine),
-                                               reinterpret_cast<DpfBlk*>(peer), count);
+    (*x->fn)(reinterpret_cast<const DpfBlk*>(mine), reinterpret_cast<DpfBlk*>(peer), count);
     return 0;
   } catch (...) {
-    return 1;   // reported as PCGSQ_ECALLBACK, rethrown as runtime_error
+    x->err = std::current_exception();
+    return 1;
   }
+}
+
+// Status of a C call that took an exchange: the exchange's exception first.
+inline void check_call(int rc, const ExchangeCall& x) {
+  if (x.err) std::rethrow_exception(x.err);
+  check(rc);
 }
 
 inline int prg_code(DpfGpuPrg p) { return p == DpfGpuPrg::AES ? PCGSQ_PRG_AES : PCGSQ_PRG_CHACHA8; }
@@ -91,11 +108,12 @@ inline void dpf_gpu_batch_full_eval(int B, int n, int w, const DpfBlk* roots,
                                     DpfGpuPrg prg = DpfGpuPrg::AES) {
   if (B <= 0) return;
   std::lock_guard<std::mutex> lk(detail::mu());
-  detail::check(pcgsq_dpf_batch_full_eval(
+  detail::ExchangeCall call{&exchange, nullptr};
+  const int rc = pcgsq_dpf_batch_full_eval(
       detail::ctx(), B, n, w, reinterpret_cast<const pcgsq_blk*>(roots),
       reinterpret_cast<const pcgsq_blk*>(dltsft), round_keys, party, detail::exchange_tramp,
-      const_cast<DpfExchangeFn*>(&exchange), reinterpret_cast<pcgsq_blk*>(out_leaves),
-      detail::prg_code(prg)));
+      &call, reinterpret_cast<pcgsq_blk*>(out_leaves), detail::prg_code(prg));
+  detail::check_call(rc, call);
 }
 
 // dpf_gpu.h:69-75
@@ -107,10 +125,12 @@ inline const DpfBlk* dpf_gpu_batch_full_eval_device(int B, int n, int w, const D
   if (B <= 0) return nullptr;
   std::lock_guard<std::mutex> lk(detail::mu());
   const pcgsq_blk* d = nullptr;
-  detail::check(pcgsq_dpf_batch_full_eval_device(
+  detail::ExchangeCall call{&exchange, nullptr};
+  const int rc = pcgsq_dpf_batch_full_eval_device(
       detail::ctx(), B, n, w, reinterpret_cast<const pcgsq_blk*>(roots),
       reinterpret_cast<const pcgsq_blk*>(dltsft), round_keys, party, detail::exchange_tramp,
-      const_cast<DpfExchangeFn*>(&exchange), &d, detail::prg_code(prg)));
+      &call, &d, detail::prg_code(prg));
+  detail::check_call(rc, call);
   return reinterpret_cast<const DpfBlk*>(d);
 }
 

@@ -91,10 +91,110 @@ __global__ void field_op_kernel(const uint64_t* __restrict__ a, const uint64_t* 
 }
 
 // Plain 16-byte-per-thread copy: the known-size reference for HBM counters.
-__global__ void copy_probe_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// HBM probes.  Each workgroup walks chunks of U * blockDim 16-byte words
+// (grid-stride over chunks); a lane first issues its U loads, then its U
+// stores, so U loads per lane are in flight at once.  NT: non-temporal loads
+// and stores (no reuse expected, do not keep the lines in cache).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ void copy_probe_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                  size_t n) {
+  const size_t chunk = (size_t)U * blockDim.x;
+  for (size_t base = (size_t)blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+    u32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+      if (i < n) v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+      if (i < n) {
+        if (NT)
+          __builtin_nontemporal_store(v[u], dst + i);
+        else
+          dst[i] = v[u];
+      }
+    }
+  }
+}
+
+// Read-only: XOR of everything read, one word per lane written at the end
+// (out has gridDim * blockDim words) so that the loads stay live.
+template <int U>
+__global__ void read_probe_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ out,
+                                  size_t n) {
+  const size_t chunk = (size_t)U * blockDim.x;
+  u32x4 acc = {0u, 0u, 0u, 0u};
+  for (size_t base = (size_t)blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+    u32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+      v[u] = i < n ? __builtin_nontemporal_load(src + i) : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc ^= v[u];
+  }
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+// Write-only: a pattern derived from the index.
+template <int U>
+__global__ void write_probe_kernel(u32x4* __restrict__ dst, size_t n) {
+  const size_t chunk = (size_t)U * blockDim.x;
+  for (size_t base = (size_t)blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+      const unsigned x = (unsigned)i;
+      if (i < n) __builtin_nontemporal_store(u32x4{x, x ^ 1u, x ^ 2u, x ^ 3u}, dst + i);
+    }
+  }
+}
+
+// Best of `reps` timed launches (the first one untimed) of `launch` on st, ms.
+template <class F>
+float best_ms(hipStream_t st, int reps, F launch) {
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  float best = 1e30f;
+  for (int rep = 0; rep <= reps; ++rep) {
+    hip_check(hipEventRecord(e0, st), "event");
+    launch();
+    hip_check(hipGetLastError(), "launch probe");
+    hip_check(hipEventRecord(e1, st), "event");
+    hip_check(hipEventSynchronize(e1), "event");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+    if (rep > 0 && ms < best) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+// Copy rate (bytes read + written per second) of the best probe shape, and
+// that shape: unroll U in {1, 2, 4, 8}, plain or non-temporal, and 2..32
+// workgroups of 256 lanes per CU.
+struct CopyBest {
+  double bps = 0;
+  int U = 0, nt = 0, wg_per_cu = 0;
+};
+
+template <int U, bool NT>
+void copy_sweep_one(hipStream_t st, int cus, const u32x4* a, u32x4* b, size_t n, CopyBest& best) {
+  for (int wpc : {2, 4, 8, 16, 32}) {
+    const dim3 grid((unsigned)(cus * wpc));
+    const float ms = best_ms(st, 3, [&] {
+      hipLaunchKernelGGL((copy_probe_kernel<U, NT>), grid, dim3(256), 0, st, a, b, n);
+    });
+    const double bps = 2.0 * (double)n * 16 / (ms * 1e-3);
+    if (bps > best.bps) best = CopyBest{bps, U, NT ? 1 : 0, wpc};
+  }
 }
 
 void upload(DevBuf& buf, const void* src, size_t bytes, hipStream_t st, const char* what) {
@@ -319,35 +419,63 @@ int pcgsq_field_op(pcgsq_ctx* ctx, int op, size_t n, const uint64_t* a, const ui
 }
 
 int pcgsq_probe_copy(pcgsq_ctx* ctx, size_t bytes, double* bytes_per_s) {
+  return pcgsq_probe_hbm(ctx, bytes, bytes_per_s, nullptr, nullptr, nullptr);
+}
+
+int pcgsq_probe_hbm(pcgsq_ctx* ctx, size_t bytes, double* copy_bps, double* read_bps,
+                    double* write_bps, int* copy_shape) {
   return guarded([&] {
-    require(ctx && bytes_per_s && bytes >= 16, "probe_copy: bad argument");
+    require(ctx && bytes >= 16, "probe_hbm: bad argument");
     CtxGuard g(ctx);
     const size_t n = bytes / 16;
-    DevBuf a, b;
-    a.ensure(n * 16, "probe_copy");
-    b.ensure(n * 16, "probe_copy");
-    hip_check(hipMemsetAsync(a.ptr, 1, n * 16, ctx->stream), "probe_copy");
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, ctx->device), "props");
-    const int grid = prop.multiProcessorCount * 16;
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&e1), "event");
-    float best = 1e30f;
-    for (int rep = 0; rep < 4; ++rep) {
-      hip_check(hipEventRecord(e0, ctx->stream), "event");
-      hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, ctx->stream,
-                         a.as<uint4>(), b.as<uint4>(), n);
-      hip_check(hipGetLastError(), "launch copy probe");
-      hip_check(hipEventRecord(e1, ctx->stream), "event");
-      hip_check(hipEventSynchronize(e1), "event");
-      float ms = 0;
-      hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-      if (rep > 0 && ms < best) best = ms;
+    const int cus = prop.multiProcessorCount;
+    DevBuf a, b;
+    a.ensure(n * 16, "probe_hbm");
+    b.ensure(std::max<size_t>(n, (size_t)cus * 32 * 256) * 16, "probe_hbm");
+    hip_check(hipMemsetAsync(a.ptr, 1, n * 16, ctx->stream), "probe_hbm");
+    const u32x4* src = a.as<u32x4>();
+    u32x4* dst = b.as<u32x4>();
+    hipStream_t st = ctx->stream;
+    if (copy_bps) {
+      CopyBest best;
+      copy_sweep_one<1, false>(st, cus, src, dst, n, best);
+      copy_sweep_one<4, false>(st, cus, src, dst, n, best);
+      copy_sweep_one<8, false>(st, cus, src, dst, n, best);
+      copy_sweep_one<1, true>(st, cus, src, dst, n, best);
+      copy_sweep_one<2, true>(st, cus, src, dst, n, best);
+      copy_sweep_one<4, true>(st, cus, src, dst, n, best);
+      copy_sweep_one<8, true>(st, cus, src, dst, n, best);
+      *copy_bps = best.bps;
+      if (copy_shape) {
+        copy_shape[0] = best.U;
+        copy_shape[1] = best.nt;
+        copy_shape[2] = best.wg_per_cu;
+      }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *bytes_per_s = 2.0 * (double)n * 16 / (best * 1e-3);   // read + write
+    if (read_bps) {
+      double bestr = 0;
+      for (int wpc : {4, 8, 16, 32}) {
+        const float ms = best_ms(st, 3, [&] {
+          hipLaunchKernelGGL(read_probe_kernel<8>, dim3((unsigned)(cus * wpc)), dim3(256), 0, st,
+                             src, dst, n);
+        });
+        bestr = std::max(bestr, (double)n * 16 / (ms * 1e-3));
+      }
+      *read_bps = bestr;
+    }
+    if (write_bps) {
+      double bestw = 0;
+      for (int wpc : {4, 8, 16, 32}) {
+        const float ms = best_ms(st, 3, [&] {
+          hipLaunchKernelGGL(write_probe_kernel<8>, dim3((unsigned)(cus * wpc)), dim3(256), 0,
+                             st, dst, n);
+        });
+        bestw = std::max(bestw, (double)n * 16 / (ms * 1e-3));
+      }
+      *write_bps = bestw;
+    }
   });
 }
 

@@ -361,6 +361,11 @@ static OleDims make_dims(const pcgsq_params& p) {
   d.N = p.N; d.c = p.c; d.t = p.t; d.w = p.w; d.prg = p.prg;
   d.L = ilog((uint64_t)p.N);
   require(d.L >= 2 && d.L <= 24, "ole: N must be in [4, 2^24] (2N | P-1)", kErrUnsupported);
+  // instance counts and every device index derived from them are int: bound
+  // c^2 t^2 (both parties' trees, 2 c^2 t^2, too) in 64-bit arithmetic first
+  const uint64_t inst64 = (uint64_t)p.c * (uint64_t)p.c * (uint64_t)p.t * (uint64_t)p.t;
+  require(inst64 < ((uint64_t)1 << 30), "ole: c^2 t^2 DPF instances must be below 2^30",
+          kErrUnsupported);
   d.B = p.t * p.t;
   d.bin = p.N / p.t;
   d.D = 2 * d.bin;

@@ -4,10 +4,11 @@
 // time.  A worker thread keeps a pool of expansion plans, each with a context
 // of its own (NTT tables and scratch, all allocated when the plan is built),
 // and runs at most `slots` requests at once, one per slot stream: a running
-// plan's context borrows the stream of its slot.  The server owns exactly
-// `slots` streams, so that with slots <= the device's hardware queues
+// plan's context borrows the stream of its slot.  The server runs requests on
+// exactly `slots` streams, so that with slots <= the device's hardware queues
 // (GPU_MAX_HW_QUEUES, 4 by default) every running request has a hardware
-// queue of its own and never waits in order behind another one's kernels.  Waiting
+// queue of its own and never waits in order behind another one's kernels; one
+// more stream builds the plans that pcgsq_server_reserve asks for.  Waiting
 // requests are admitted oldest first, except that a request whose shape has
 // no idle plan is passed over (requests of one shape keep their order): a
 // slot going free is taken by the oldest request that can start now.  A
@@ -121,6 +122,7 @@ struct pcgsq_server {
   size_t budget = 0, used = 0, peak = 0, plan_cap = 0;
   std::vector<std::unique_ptr<PlanEntry>> pool;
   std::vector<hipStream_t> streams;        // one per slot
+  hipStream_t build_stream = nullptr;      // plans built by reserve (never runs requests)
   std::vector<char> slot_busy;
   std::multimap<size_t, void*> free_out;   // idle output buffers by size (counted in used)
   std::list<Request*> queue;               // list: iterators survive submits while unlocked
@@ -204,20 +206,34 @@ struct pcgsq_server {
     r->out = nullptr;
   }
 
-  // A new plan of this shape, built with the lock held (reserve); nullptr if
-  // memory does not allow.
-  PlanEntry* new_entry(const pcgsq_params& p, uint64_t pub_seed) {
+  // A new plan of this shape for reserve: room is made and charged under the
+  // lock, the plan is built with the lock released, on the build stream (no
+  // request runs there, so the build's synchronization waits only for
+  // itself, and the worker keeps polling and admitting meanwhile).
+  void reserve_one(std::unique_lock<std::mutex>& lk, const pcgsq_params& p, uint64_t pub_seed) {
     const size_t est = pcgsq_ole_plan_estimate(&p, plan_budget());
-    if (est == 0) throw Error(kErrInvalid, "invalid expansion parameters");
-    if (!make_room(est)) return nullptr;
-    const int k = free_slot();
-    std::unique_ptr<PlanEntry> e =
-        build_entry(device, p, pub_seed, plan_budget(), streams[(size_t)(k < 0 ? 0 : k)]);
-    e->bytes = est;
+    if (est == 0) throw Error(kErrInvalid, "server_reserve: invalid expansion parameters");
+    if (!make_room(est))
+      throw Error(kErrNoMemory, "server_reserve: the plans do not fit the memory budget");
     charge(est);
+    std::unique_ptr<PlanEntry> e;
+    std::string err;
+    lk.unlock();
+    try {
+      e = build_entry(device, p, pub_seed, plan_budget(), build_stream);
+    } catch (const std::exception& ex) {
+      err = ex.what();
+    }
+    lk.lock();
+    if (!e) {
+      used -= est;
+      throw Error(kErrDevice, "server_reserve: " + err);
+    }
+    e->bytes = est;
+    e->last_used = now_s();
     pool.push_back(std::move(e));
     ++plans_built;
-    return pool.back().get();
+    cv.notify_all();
   }
 
   void fail(Request* r, const std::string& why) {
@@ -410,6 +426,7 @@ int pcgsq_server_create(int device, size_t hbm_budget_bytes, int slots, pcgsq_se
     s->slot_busy.assign((size_t)slots, 0);
     for (auto& st : s->streams)
       hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "server stream");
+    hip_check(hipStreamCreateWithFlags(&s->build_stream, hipStreamNonBlocking), "server stream");
     s->worker = std::thread([p = s.get()] {
       (void)hipSetDevice(p->device);
       p->run();
@@ -434,6 +451,7 @@ int pcgsq_server_destroy(pcgsq_server* s) {
     s->pool.clear();
     for (hipStream_t st : s->streams)
       if (st) (void)hipStreamDestroy(st);
+    if (s->build_stream) (void)hipStreamDestroy(s->build_stream);
     for (auto& kv : s->reqs)
       if (kv.second->out) (void)hipFree(kv.second->out);
     for (auto& kv : s->free_out) (void)hipFree(kv.second);
@@ -583,13 +601,13 @@ int pcgsq_server_reserve(pcgsq_server* s, const pcgsq_params* p, uint64_t pub_se
   return guarded([&] {
     require(s && p, "server_reserve: null argument");
     require(count >= 0 && count <= 16, "server_reserve: count must be 0..16");
-    std::lock_guard<std::mutex> lk(s->mu);
-    int have = 0;
-    for (auto& e : s->pool)
-      have += same_params(e->pp, *p) && e->pub_seed == pub_seed;
-    for (; have < count; ++have)
-      require(s->new_entry(*p, pub_seed) != nullptr,
-              "server_reserve: the plans do not fit the memory budget", kErrNoMemory);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+      int have = 0;
+      for (auto& e : s->pool) have += same_params(e->pp, *p) && e->pub_seed == pub_seed;
+      if (have >= count) break;
+      s->reserve_one(lk, *p, pub_seed);
+    }
   });
 }
 

@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
     L.pcgsq_ctx_profile_read.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.pcgsq_probe_chacha8.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.pcgsq_probe_copy.argtypes = [vp, C.c_size_t, C.POINTER(C.c_double)]
+    L.pcgsq_probe_hbm.argtypes = [vp, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.pcgsq_field_op.argtypes = [vp, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]
     L.pcgsq_dev_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.pcgsq_dev_free.argtypes = [vp, vp]
@@ -193,6 +195,40 @@ def _blocks(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 2)
 
 
+def _out_array(a, n: int, name: str) -> np.ndarray:
+    """An output array the C side writes n uint64 words into: refuse anything
+    else (a short, strided or non-uint64 array would be written past)."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.uint64 or a.size != n \
+            or not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError(f"{name}: expected a writable C-contiguous uint64 array of {n} "
+                         f"elements")
+    return a
+
+
+def _dpf_inputs(roots, dltsft, n: int, w: int):
+    """roots [B] and dltsft [B][n/w][2^w - 1] blocks, checked against each
+    other (the C side reads B*(n/w)*(2^w-1) DltSft blocks)."""
+    roots, dltsft = _blocks(roots), _blocks(dltsft)
+    if w < 1 or w > 4 or n % w:
+        raise ValueError(f"dpf: w must be in [1, 4] and divide n (n={n}, w={w})")
+    want = len(roots) * (n // w) * ((1 << w) - 1)
+    if len(dltsft) != want:
+        raise ValueError(f"dpf: dltsft has {len(dltsft)} blocks, expected B*(n/w)*(2^w-1) = "
+                         f"{want}")
+    return roots, dltsft
+
+
+def _round_keys(round_keys, w: int, prg: int):
+    """AES tree round keys, nk*176 bytes (None for ChaCha8)."""
+    if round_keys is None:
+        return None
+    rk = np.ascontiguousarray(round_keys, dtype=np.uint8).reshape(-1)
+    if prg == PRG_AES and rk.size != ((1 << w) - 1) * 176:
+        raise ValueError(f"dpf: AES round keys must be (2^w - 1) * 176 = {((1 << w) - 1) * 176} "
+                         f"bytes, got {rk.size}")
+    return rk
+
+
 def device_available() -> bool:
     """pcg_cuda::is_cuda_available (GPU_baseline/common/poly_mul_cuda.h:9)."""
     return bool(lib().pcgsq_device_available())
@@ -261,6 +297,17 @@ class Context:
         _check(lib().pcgsq_probe_copy(self.h, nbytes, C.byref(v)))
         return v.value
 
+    def probe_hbm(self, nbytes: int = 1 << 30) -> dict:
+        """Achievable HBM rates: copy (read + write counted, best probe shape),
+        read-only and write-only, in bytes/s, and the copy's best shape."""
+        cp, rd, wr = C.c_double(), C.c_double(), C.c_double()
+        shape = (C.c_int * 3)()
+        _check(lib().pcgsq_probe_hbm(self.h, nbytes, C.byref(cp), C.byref(rd), C.byref(wr),
+                                     shape))
+        return {"copy": cp.value, "read": rd.value, "write": wr.value,
+                "copy_shape": {"loads_per_lane": shape[0], "nontemporal": bool(shape[1]),
+                               "wg_per_cu": shape[2]}}
+
     def field_op(self, op: int, a, b, c=None) -> np.ndarray:
         """Device F_p arithmetic elementwise (op 0 mulhi64, 1 the butterflies'
         Shoup multiply, 2 the general Shoup multiply, 3 a*b mod P)."""
@@ -288,37 +335,45 @@ class Context:
                             round_keys=None):
         """One party's batched full-domain half-tree DPF with a host exchange
         callable(level, mine: ndarray[(count,2)]) -> peer ndarray."""
-        roots, dltsft = _blocks(roots), _blocks(dltsft)
+        roots, dltsft = _dpf_inputs(roots, dltsft, n, w)
         B = len(roots)
         out = np.empty((B << n, 2), dtype=np.uint64)
-        cb = self._exchange(exchange)
-        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
-        _check(lib().pcgsq_dpf_batch_full_eval(
+        cb, err = self._exchange(exchange)
+        rk = _round_keys(round_keys, w, prg)
+        rc = lib().pcgsq_dpf_batch_full_eval(
             self.h, B, n, w, roots.ctypes.data, dltsft.ctypes.data,
-            None if rk is None else rk.ctypes.data, party, cb, None, out.ctypes.data, prg))
+            None if rk is None else rk.ctypes.data, party, cb, None, out.ctypes.data, prg)
+        if err:
+            raise err[0]
+        _check(rc)
         return out
 
     def dpf_batch_full_eval_device(self, roots, dltsft, n, w, exchange, party=0,
                                    prg=PRG_CHACHA8, round_keys=None) -> int:
-        roots, dltsft = _blocks(roots), _blocks(dltsft)
-        cb = self._exchange(exchange)
-        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
+        roots, dltsft = _dpf_inputs(roots, dltsft, n, w)
+        cb, err = self._exchange(exchange)
+        rk = _round_keys(round_keys, w, prg)
         d = vp()
-        _check(lib().pcgsq_dpf_batch_full_eval_device(
+        rc = lib().pcgsq_dpf_batch_full_eval_device(
             self.h, len(roots), n, w, roots.ctypes.data, dltsft.ctypes.data,
-            None if rk is None else rk.ctypes.data, party, cb, None, C.byref(d), prg))
+            None if rk is None else rk.ctypes.data, party, cb, None, C.byref(d), prg)
+        if err:
+            raise err[0]
+        _check(rc)
         return d.value
 
     def dpf_full_eval_both(self, roots0, roots1, dlt0, dlt1, n, w, prg=PRG_CHACHA8,
                            round_keys=None, want_cw=False):
-        roots0, roots1 = _blocks(roots0), _blocks(roots1)
-        dlt0, dlt1 = _blocks(dlt0), _blocks(dlt1)
+        roots0, dlt0 = _dpf_inputs(roots0, dlt0, n, w)
+        roots1, dlt1 = _dpf_inputs(roots1, dlt1, n, w)
+        if len(roots1) != len(roots0):
+            raise ValueError("dpf: the two parties' batches differ in size")
         B = len(roots0)
         l0 = np.empty((B << n, 2), dtype=np.uint64)
         l1 = np.empty((B << n, 2), dtype=np.uint64)
         nk = (1 << w) - 1
         cw = np.empty(((n // w) * B * nk, 2), dtype=np.uint64) if want_cw else None
-        rk = None if round_keys is None else np.ascontiguousarray(round_keys, dtype=np.uint8)
+        rk = _round_keys(round_keys, w, prg)
         _check(lib().pcgsq_dpf_batch_full_eval_both(
             self.h, B, n, w, roots0.ctypes.data, roots1.ctypes.data, dlt0.ctypes.data,
             dlt1.ctypes.data, None if rk is None else rk.ctypes.data, l0.ctypes.data,
@@ -326,18 +381,24 @@ class Context:
         return (l0, l1, cw) if want_cw else (l0, l1)
 
     def _exchange(self, fn):
+        """A C trampoline for the exchange callable, and the list that
+        receives the exception the callable raised (the caller re-raises it
+        after the C call, so the callable's own error reaches the user)."""
+        err = []
+
         def tramp(user, level, mine, peer, count):
             try:
                 m = np.ctypeslib.as_array(C.cast(mine, u64p), shape=(count * 2,)).reshape(-1, 2)
                 res = _blocks(fn(level, m.copy()))
                 if res.shape != (count, 2):
-                    return 1
+                    raise ValueError(f"exchange returned {res.shape[0]} blocks, expected {count}")
                 C.memmove(peer, res.ctypes.data, count * 16)
                 return 0
-            except Exception:
+            except BaseException as e:
+                err.append(e)
                 return 1
         self._cb = EXCHANGE_FN(tramp)   # keep alive for the call
-        return self._cb
+        return self._cb, err
 
     # ---- output layer (GPU_baseline/common/leaf_convert_cuda.h:43-52) -------
     def dpf_out_sums(self, d_leaves: int, B: int, D: int):
@@ -347,7 +408,9 @@ class Context:
         return sc, st
 
     def dpf_out_scatter_g(self, d_leaves: int, B: int, D: int, t: int, party: int, CW, N: int):
-        cw = _u64(CW)
+        cw = _u64(CW).reshape(-1)
+        if cw.size != B:
+            raise ValueError(f"dpf_out_scatter_g: CW has {cw.size} words, expected B = {B}")
         g = np.empty(N, dtype=np.uint64)
         _check(lib().pcgsq_dpf_out_scatter_g(self.h, d_leaves, B, D, t, party, P, _p(cw), _p(g),
                                              N))
@@ -356,6 +419,8 @@ class Context:
     # ---- poly-mul / NTT (poly_mul_cuda.h, CPU_baseline/ntt.h) --------------
     def poly_mul(self, a, b, backend=NTT_FOURSTEP):
         a, b = _u64(a), _u64(b)
+        if a.shape != b.shape or a.ndim == 0:
+            raise ValueError(f"poly_mul: operand shapes differ ({a.shape} vs {b.shape})")
         shape = a.shape
         a2, b2 = a.reshape(-1, shape[-1]), b.reshape(-1, shape[-1])
         out = np.empty_like(a2)
@@ -532,9 +597,7 @@ class OlePlan:
         arrays of N, e.g. from pinned_array) and returned."""
         if out is None:
             out = tuple(np.empty(self.N, dtype=np.uint64) for _ in range(4))
-        x0, z0, x1, z1 = out
-        for a in out:
-            assert a.dtype == np.uint64 and a.size == self.N and a.flags.c_contiguous
+        x0, z0, x1, z1 = (_out_array(a, self.N, "OlePlan.outputs") for a in out)
         _check(lib().pcgsq_ole_outputs(self.h, _p(x0), _p(z0), _p(x1), _p(z1)))
         return x0, z0, x1, z1
 
@@ -600,7 +663,7 @@ class OleParty(OlePlan):
         """(x, z) of this party."""
         if out is None:
             out = tuple(np.empty(self.N, dtype=np.uint64) for _ in range(2))
-        x, z = out
+        x, z = (_out_array(a, self.N, "OleParty.outputs") for a in out)
         _check(lib().pcgsq_ole_party_outputs(self.h, _p(x), _p(z)))
         return x, z
 

@@ -13,6 +13,7 @@
 //   4. OleExpander: x0 * x1 == z0 + z1 at N = 2^12
 //   5. OleExpander::triples: every slot c0 + c1 == (a0 + a1)(b0 + b1)
 //   6. OleExpander::expand(seed, gmw = true) == expand(seed)
+//      (plus an exchange that throws: its exception reaches the caller as is)
 //   7. OleParty: the two parties in two threads, every exchange through an
 //      in-process channel standing in for emp::NetIO; each party's (x, z)
 //      equals OleExpander's, with one "dpf_gen" row per DPF level and block
@@ -120,6 +121,24 @@ int main() {
     pcg_cuda::dpf_gpu_batch_full_eval(B, n, w, roots.data(), dltsft.data(), nullptr, 0, exchange,
                                       leaves.data(), pcg_cuda::DpfGpuPrg::CHACHA8);
     CHECK(leaves[5].lo != 0 || leaves[5].hi != 0, "leaves copied back");
+    // an exchange that throws: its own exception reaches the caller unchanged
+    struct PeerGone {
+      int level;
+    };
+    int thrown_at = -1;
+    try {
+      int k = 0;
+      pcg_cuda::dpf_gpu_batch_full_eval(
+          B, n, w, roots.data(), dltsft.data(), nullptr, 0,
+          [&](const DpfBlk*, DpfBlk*, size_t) {
+            if (++k == 3) throw PeerGone{k};
+          },
+          leaves.data(), pcg_cuda::DpfGpuPrg::CHACHA8);
+    } catch (const PeerGone& e) {
+      thrown_at = e.level;
+    } catch (...) {
+    }
+    CHECK(thrown_at == 3, "the exchange's exception propagates unchanged");
   }
   // 2. out-hash fixture shape: t=3, B=9, D=1024, leaves from mt19937_64(12345)
   {

@@ -258,6 +258,49 @@ def test_reference_ntt_checksums_at_2_20(ctx, pq, golden, backend):
     assert "%016x" % int(np.bitwise_xor.reduce(c.ravel())) == golden["ntt_4step_xor"]["by_logN"]["20"]
 
 
+@pytest.mark.parametrize("logN", [21, 22, 23, 24])
+def test_reference_ntt_checksums_2_21_to_2_24(ctx, pq, golden, logN):
+    """The reference's published poly-mul checksums at N = 2^21..2^24, batch
+    16 (results_l40s/ntt_naive.txt:2-5, ntt_4step.txt:14,21,28,35): the
+    naive recipe's FNV through the naive backend and the four-step recipe's
+    XOR through the four-step backend (the product is unique, so either
+    backend must give both)."""
+    N, batch = 1 << logN, 16
+    m = O.MT64(11)
+    a = m.take(batch * N, P).reshape(batch, N)
+    b = m.take(batch * N, P).reshape(batch, N)
+    c = ctx.poly_mul(a, b, pq.NTT_NAIVE)
+    assert "%016x" % O.fnv1a64(c) == golden["ntt_naive_fnv"]["by_logN"][str(logN)]
+    del a, b, c
+    j = np.arange(batch * N, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        a = ((np.uint64(0x9E3779B97F4A7C15) * (j + np.uint64(1))) % np.uint64(P)).reshape(batch, N)
+        b = ((np.uint64(0xC2B2AE3D27D4EB4F) * (j + np.uint64(7))) % np.uint64(P)).reshape(batch, N)
+    del j
+    c = ctx.poly_mul(a, b, pq.NTT_FOURSTEP)
+    assert "%016x" % int(np.bitwise_xor.reduce(c.ravel())) == golden["ntt_4step_xor"]["by_logN"][str(logN)]
+
+
+def test_headline_plan_matches_precomputed_cpu_hashes(ctx, pq):
+    """The metric's "(bit-exact)" for the plan bench.py times: a
+    default-budget OlePlan at N = 2^20, c = 4, t = 16, w = 1, ChaCha8 (its
+    own DPF block grouping) returns x0, z0, x1, z1 whose FNV-1a-64 hashes are
+    the CPU oracle's in tests/golden/served_hashes.json, for seeds 7006 and
+    7007, and again after another seed ran on the same plan."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "served_hashes.json")
+    want = {r["seed"]: r["fnv1a64"] for r in json.load(open(path))["requests"]
+            if r["N"] == 1 << 20 and r["c"] == 4 and r["t"] == 16}
+    assert sorted(want) == [7006, 7007]
+    with pq.OlePlan(ctx, 1 << 20, 4, 16) as pl:
+        for seed in (7006, 7007, 7006):
+            pl.deal(seed)
+            pl.run()
+            got = ["%016x" % O.fnv1a64(v) for v in pl.outputs()]
+            assert got == want[seed], f"seed {seed}: hashes differ from the CPU oracle"
+
+
 def test_aes_ecb_fips_and_random(ctx, pq, golden):
     for case in golden["aes128_fips197"]["cases"]:
         ct = ctx.aes128_ecb_encrypt(bytes.fromhex(case["key"]), bytes.fromhex(case["pt"]))

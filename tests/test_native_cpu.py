@@ -123,3 +123,78 @@ def test_dist_exchange_swaps_between_two_processes():
     for rank, (p, log) in enumerate(zip(procs, logs)):
         assert p.returncode == 0, log
         assert f"ok {rank}" in log.splitlines(), log
+
+
+def test_plan_estimate_rejects_instance_counts_past_int_range():
+    """c^2 t^2 DPF instances index device arrays as int: parameter sets whose
+    count would wrap (c = 9, t = 2^13 gives 5.4e9) are rejected, so the plan
+    estimate is 0 and no plan or served request is sized from a wrapped
+    count; a valid set right below the bound still gets an estimate."""
+    import pcgsq
+    assert pcgsq.ole_plan_estimate(1 << 24, 9, 1 << 13) == 0
+    assert pcgsq.ole_plan_estimate(1 << 20, 16, 1 << 12) == 0
+    assert pcgsq.ole_plan_estimate(1 << 24, 1, 1 << 15) == 0      # t^2 = 2^30
+    assert pcgsq.ole_plan_estimate(1 << 20, 4, 16) > 0
+    assert pcgsq.ole_plan_estimate(1 << 20, 2, 1 << 13) > 0         # 2^28 instances
+
+
+def test_oracle_aes128_matches_libcrypto():
+    """The oracle's AES-128 against OpenSSL's libcrypto (an independent
+    implementation, loaded through ctypes: EVP aes-128-ecb without padding)
+    on random keys and blocks; and the AES tree PRG's sub-key derivation
+    sub_k = AES_S(makeBlock(0, k)) xor makeBlock(0, k) and its CCR hash
+    H_k(x) = AES_{sub_k}(x) xor x (half_tree_dpf/tree_prg.h:42-68,
+    mkey_prp.h:24-79) recomputed from libcrypto's cipher."""
+    import ctypes
+    import ctypes.util
+
+    import numpy as np
+
+    import oracle as O
+    name = ctypes.util.find_library("crypto")
+    if not name:
+        pytest.skip("libcrypto not present")
+    L = ctypes.CDLL(name)
+    L.EVP_CIPHER_CTX_new.restype = ctypes.c_void_p
+    L.EVP_aes_128_ecb.restype = ctypes.c_void_p
+    L.EVP_EncryptInit_ex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_char_p, ctypes.c_char_p]
+    L.EVP_CIPHER_CTX_set_padding.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.EVP_EncryptUpdate.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
+    L.EVP_CIPHER_CTX_free.argtypes = [ctypes.c_void_p]
+
+    def ssl_aes(key: bytes, pt: bytes) -> bytes:
+        cx = L.EVP_CIPHER_CTX_new()
+        try:
+            assert L.EVP_EncryptInit_ex(cx, L.EVP_aes_128_ecb(), None, key, None) == 1
+            L.EVP_CIPHER_CTX_set_padding(cx, 0)
+            out = ctypes.create_string_buffer(len(pt) + 16)
+            n = ctypes.c_int()
+            assert L.EVP_EncryptUpdate(cx, out, ctypes.byref(n), pt, len(pt)) == 1
+            assert n.value == len(pt)
+            return out.raw[:len(pt)]
+        finally:
+            L.EVP_CIPHER_CTX_free(cx)
+
+    rng = np.random.default_rng(197)
+    for _ in range(64):
+        key = rng.integers(0, 256, 16, dtype=np.uint8).tobytes()
+        pt = rng.integers(0, 256, 16 * 8, dtype=np.uint8).tobytes()
+        rk = O.aes128_expand_key(key)
+        got = b"".join(O.aes128_encrypt(rk, pt[i:i + 16]) for i in range(0, len(pt), 16))
+        assert got == ssl_aes(key, pt)
+    master = O.FIPS197_MASTER
+    for nk in (1, 3, 15):
+        rks = O.aes_tree_round_keys(nk, master)
+        parents = rng.integers(0, 2**63, (5, 2), dtype=np.uint64)
+        out = np.empty((5 * nk, 2), dtype=np.uint64)
+        O.lib().orc_tree_hash_aes(rks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  out.ctypes.data, parents.ctypes.data, 5, nk)
+        for k in range(nk):
+            mk = k.to_bytes(8, "little") + bytes(8)
+            sub = bytes(a ^ b for a, b in zip(ssl_aes(master, mk), mk))
+            for j in range(5):
+                x = parents[j].tobytes()
+                want = bytes(a ^ b for a, b in zip(ssl_aes(sub, x), x))
+                assert out[j * nk + k].tobytes() == want, (nk, k, j)
