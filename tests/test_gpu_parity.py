@@ -151,15 +151,27 @@ def test_dpf_two_parties_through_host_exchange(pq, n, w, B, prg):
 
 
 def test_host_exchange_callback_failure_reported(ctx, pq):
+    """An exchange that raises aborts the expansion with the exchange's own
+    exception (as the reference's NetIO errors reach its caller); one that
+    returns the wrong number of blocks raises ValueError; the context stays
+    usable afterwards."""
     roots = np.ones((2, 2), dtype=np.uint64)
     dlt = np.ones((2 * 6, 2), dtype=np.uint64)
 
-    def bad(level, mine):
-        raise RuntimeError("peer went away")
+    class PeerGone(RuntimeError):
+        pass
 
-    with pytest.raises(pq.PcgsqError) as ei:
+    def bad(level, mine):
+        raise PeerGone("peer went away")
+
+    with pytest.raises(PeerGone, match="peer went away"):
         ctx.dpf_batch_full_eval(roots, dlt, 6, 1, bad)
-    assert ei.value.code == pq.ECALLBACK
+    with pytest.raises(ValueError, match="expected 2"):
+        ctx.dpf_batch_full_eval(roots, dlt, 6, 1, lambda level, mine: mine[:1])
+    with pytest.raises(ValueError, match="dltsft"):
+        ctx.dpf_batch_full_eval(roots, dlt[:-1], 6, 1, lambda level, mine: mine)
+    out = ctx.dpf_batch_full_eval(roots, dlt, 6, 1, lambda level, mine: np.zeros_like(mine))
+    assert out.shape == (2 << 6, 2)
 
 
 def test_out_hash_bitexact_fixture(ctx, pq):
