@@ -156,6 +156,19 @@ int pcgsq_dpf_batch_full_eval_both(pcgsq_ctx* ctx, int B, int n, int w,
 int pcgsq_dpf_out_sums(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B,
                        size_t D, uint64_t prime, uint64_t* sumC,
                        uint64_t* sumT);
+/* No counterpart in the reference: pcgsq_dpf_out_sums for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops or
+ * harnesses that forbid hipMalloc).  It allocates nothing and does not
+ * synchronize: the sums of the device leaves d_leaves [B][D] go to the device
+ * arrays d_sumC[B] (mod P) and d_sumT[B], enqueued on the context stream.
+ * Its partial sums live in a fixed device array of
+ * PCGSQ_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends):
+ * 16 * B * max(1, D / min(D, 2048)) bytes must fit, else PCGSQ_EINVAL.  All
+ * calls on one GPU share that array, so calls that overlap on one GPU (other
+ * contexts or streams) race; calls on one context are ordered by its stream. */
+#define PCGSQ_NOALLOC_SCRATCH_BYTES (4u << 20)
+int pcgsq_dpf_out_sums_noalloc(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B, size_t D,
+                               uint64_t prime, uint64_t* d_sumC, uint64_t* d_sumT);
 /* pcg_cuda::dpf_out_scatter_g (leaf_convert_cuda.h:49-52): host CW[B],
  * host g_out[N] overwritten with the folded y = +-(C + tau*CW). */
 int pcgsq_dpf_out_scatter_g(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B,

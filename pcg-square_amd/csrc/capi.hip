@@ -5,12 +5,16 @@
 #include <string.h>
 
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "context.hpp"
 #include "pcgsq.h"
 #include "prg.hpp"
+
+// The no-allocation entry point's scratch array (see scratch_address below).
+static __device__ __align__(256) uint64_t scratch[PCGSQ_NOALLOC_SCRATCH_BYTES / sizeof(uint64_t)];
 
 namespace pcgsq {
 
@@ -214,6 +218,31 @@ void stage_aes(pcgsq_ctx* ctx, const uint8_t* round_keys, int nk, DpfRun& r) {
   r.aes_rks = ctx->rks.as<uint8_t>();
 }
 
+// Scratch of the no-allocation entry point (pcgsq_dpf_out_sums_noalloc): the
+// leaf-sum partials, [B][nparts][2] words.  Each device holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: the device address comes from hipGetSymbolAddress, once
+// per device (scratch_address), and only that address is ever passed to a
+// kernel.
+constexpr size_t kNoAllocScratchBytes = (size_t)PCGSQ_NOALLOC_SCRATCH_BYTES;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch` (device current; looked up once).
+uint64_t* scratch_address(int device) {
+  require(device >= 0 && device < kMaxDevices, "scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch");
+    require(p != nullptr, "hipGetSymbolAddress returned null for scratch", kErrDevice);
+    g_scratch_ptr[device] = static_cast<uint64_t*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
 struct HostEx {
   pcgsq_exchange_fn fn;
   void* user;
@@ -309,6 +338,7 @@ int pcgsq_ctx_create(int device, pcgsq_ctx** out) {
     c->ntt.prof = &c->prof;
     c->activate();
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->noalloc_scratch = scratch_address(device);
     *out = c.release();
   });
 }
@@ -667,6 +697,31 @@ int pcgsq_dpf_out_scatter_g(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B, si
     launch_gather_g(ctx->stream, ga);
     hip_check(hipMemcpyAsync(g_out, ga.g, 8 * (size_t)N, hipMemcpyDeviceToHost, ctx->stream), "D2H g");
     hip_check(hipStreamSynchronize(ctx->stream), "out_scatter_g");
+  });
+}
+
+int pcgsq_dpf_out_sums_noalloc(pcgsq_ctx* ctx, const pcgsq_blk* d_leaves, int B, size_t D,
+                               uint64_t prime, uint64_t* d_sumC, uint64_t* d_sumT) {
+  return guarded([&] {
+    require(ctx && d_leaves && d_sumC && d_sumT, "out_sums_noalloc: null argument");
+    require(prime == kPrime, "out_sums_noalloc: only P = 2^62 - 3*2^25 + 1 is supported",
+            kErrUnsupported);
+    require(B > 0, "out_sums_noalloc: B must be positive");
+    const int D_log = ilog2_exact(D);
+    require(D_log >= 1 && D_log <= 30, "out_sums_noalloc: D must be a power of two");
+    const int nparts = leaf_sums_nparts(D_log);
+    const size_t need = sizeof(uint64_t) * 2 * (size_t)B * (size_t)nparts;
+    require(need <= kNoAllocScratchBytes,
+            "out_sums_noalloc: B * partials per tree needs " + std::to_string(need) +
+                " scratch bytes, above the fixed " + std::to_string(kNoAllocScratchBytes),
+            kErrInvalid);
+    CtxGuard g(ctx);
+    require(ctx->noalloc_scratch != nullptr, "out_sums_noalloc: context has no scratch",
+            kErrInternal);
+    // every partial of the B trees is written by exactly one workgroup of the
+    // sums kernel before the fold reads it: nothing must start at zero
+    launch_leaf_sums(ctx->stream, reinterpret_cast<const Blk*>(d_leaves), B, D_log,
+                     ctx->noalloc_scratch, nparts, d_sumC, d_sumT);
   });
 }
 

@@ -18,6 +18,9 @@ struct pcgsq_ctx {
   pcgsq::NttEngine ntt;
   pcgsq::DevBuf in_a, in_b, in_c, in_d, leaves, part, sums, cwbuf, gbuf, t0, rks;
   pcgsq::Profiler prof;
+  // device address of this device's copy of the library's no-allocation
+  // scratch array (capi.hip), taken when the context is created
+  uint64_t* noalloc_scratch = nullptr;
 
   void activate() { pcgsq::hip_check(hipSetDevice(device), "hipSetDevice"); }
 };

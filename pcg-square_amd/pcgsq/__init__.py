@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
     L.pcgsq_dpf_batch_full_eval_both.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
                                                  vp, vp, vp, vp, C.c_int]
     L.pcgsq_dpf_out_sums.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_uint64, u64p, u64p]
+    L.pcgsq_dpf_out_sums_noalloc.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_uint64, vp, vp]
     L.pcgsq_dpf_out_scatter_g.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int,
                                           C.c_uint64, u64p, u64p, C.c_int]
     L.pcgsq_poly_mul_supported.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
@@ -406,6 +407,13 @@ class Context:
         st = np.empty(B, dtype=np.uint64)
         _check(lib().pcgsq_dpf_out_sums(self.h, d_leaves, B, D, P, _p(sc), _p(st)))
         return sc, st
+
+    def dpf_out_sums_noalloc(self, d_leaves: int, B: int, D: int, d_sumC: int,
+                             d_sumT: int) -> None:
+        """dpf_out_sums into device arrays d_sumC[B], d_sumT[B], enqueued on
+        the context stream with no device allocation (partials in the
+        library's fixed per-GPU scratch array)."""
+        _check(lib().pcgsq_dpf_out_sums_noalloc(self.h, d_leaves, B, D, P, d_sumC, d_sumT))
 
     def dpf_out_scatter_g(self, d_leaves: int, B: int, D: int, t: int, party: int, CW, N: int):
         cw = _u64(CW).reshape(-1)
