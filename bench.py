@@ -298,11 +298,12 @@ def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
 
     def party(p):
         try:
-            def exchange(phase, level, batch, mine):
+            def exchange(phase, level, batch, send, nrecv):
                 stats[p]["exchanges"] += 1
-                stats[p]["bytes"] += int(mine.nbytes)
-                inbox[1 - p].put(mine)
-                return inbox[p].get(timeout=120)
+                stats[p]["bytes"] += int(send.nbytes)
+                if send.size:
+                    inbox[1 - p].put(send)
+                return inbox[p].get(timeout=120) if nrecv else None
             with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, p) as op:
                 op.deal(1)
                 for k in range(reps + 1):   # the first round warms up

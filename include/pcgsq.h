@@ -338,43 +338,101 @@ size_t pcgsq_ole_plan_estimate(const pcgsq_params* p, size_t hbm_budget_bytes);
  * PCG_OLE<P>::gen_and_expand as one party's process runs it
  * (pcg_ole_impl.h:46-490, with emp::NetIO replaced by a callback): the party
  * expands only its own DPF trees, and every exchange of the protocol goes
- * through `fn`, in the order the reference makes them:
- *   "dpf_gen"   per DPF level and block: the level's correction-word shares,
- *               B * (2^w - 1) blocks of 16 bytes (dpf.cpp:520-551);
- *   "beaver_de" per block: d and e of the first Beaver open, 2B u64;
- *   "beaver_cw" per block: the CW shares of the second open, B u64
- *               (pcg_ole_impl.h:231-295).
- * (phase, level, batch = B, bytes) are the columns of the reference's
- * comm_trace.h rows; level is the DPF level, or the block index ii*c + jj for
- * the Beaver opens.  fn sends `bytes` from mine, receives as many from the
- * other party into peer, and returns 0 (nonzero aborts the run with
- * PCGSQ_ECALLBACK).  With several blocks per group, a level's rows of all
- * the group's blocks come together.  Deal with pcgsq_ole_deal (the party
- * reads only its own shares), run, then read x and z; the two parties' plans
- * together give exactly the outputs of one both-party plan. */
+ * through `fn`, in the order the reference makes them.  Rows (phase, level,
+ * batch, send bytes, receive bytes are the columns of the reference's
+ * comm_trace.h rows):
+ *   "base_ot"     the first run of a plan dealt with pcgsq_ole_party_deal_own:
+ *                 the Chou-Orlandi base OTs, level 0 (33 bytes) and 1
+ *                 (128 * 33 bytes);
+ *   "ot_ext"      per run of such a plan: the IKNP matrix of the OTs this
+ *                 party receives (16 bytes per OT; the two parties' sizes
+ *                 differ);
+ *   "gilboa"      party 0's corrections of the Gilboa OTs (party 1 sends 0);
+ *   "circuit_and" per AND depth of the position adder, then of the DltSft
+ *                 circuit, and per DPF block: the opened d, e of the layer's
+ *                 AND gates (bool_circuit/circuit.cpp:306-330); level = the
+ *                 circuit's depth, batch = AND gates;
+ *   "dpf_gen"     per DPF level and block: the level's correction-word shares,
+ *                 B * (2^w - 1) blocks of 16 bytes (dpf.cpp:520-551);
+ *   "beaver_de"   per block: d and e of the first Beaver open, 2B u64;
+ *   "beaver_cw"   per block: the CW shares of the second open, B u64
+ *                 (pcg_ole_impl.h:231-295); level = block index ii*c + jj.
+ * fn sends send_bytes from `send` and receives recv_bytes from the other
+ * party into `recv` (either may be 0; party 0 sends first, party 1 receives
+ * first, as NetIO's send/flush/recv pairs run), and returns 0 (nonzero
+ * aborts the run with PCGSQ_ECALLBACK).
+ * Inputs:
+ *   pcgsq_ole_party_deal_own(own_seed): the party's own noise, Delta share
+ *     and Beaver randomness from its own seed; the run first computes alpha,
+ *     DltSft, beta and rs with the other party (OT extension from base OTs,
+ *     GMW, Gilboa products: the 2PC preprocessing of pcg_ole_impl.h:62-199).
+ *     No party ever holds the other's secrets.
+ *   pcgsq_ole_deal(seed): a trusted-dealer simulation: both parties' shares
+ *     derived from one shared seed (each party process can recompute the
+ *     other's secrets), the run starting at the DPF.
+ * Two parties' plans together give exactly the outputs of one both-party
+ * plan dealt alike (deal_own with seeds S, S equals pcgsq_ole_deal(S) and
+ * pcgsq_ole_deal_gmw(S)). */
 typedef int (*pcgsq_ole_exchange_fn)(void* user, const char* phase, int level, int batch,
-                                     const void* mine, void* peer, size_t bytes);
+                                     const void* send, size_t send_bytes, void* recv,
+                                     size_t recv_bytes);
+/* Exchange granularity: per DPF block, as the reference's serial arm
+ * (default), or one exchange per DPF level, Beaver open and circuit depth
+ * for a whole group of blocks, as its batched arm
+ * (GPU_baseline/e2e_full_bench.cu:128-153).  Results are the same. */
+enum { PCGSQ_EXCHANGE_PER_BLOCK = 0, PCGSQ_EXCHANGE_BATCHED = 1 };
 int pcgsq_ole_party_create(pcgsq_ctx* ctx, const pcgsq_params* p, int party, uint64_t pub_seed,
                            size_t hbm_budget_bytes, pcgsq_ole_plan** out);
 int pcgsq_ole_party_run(pcgsq_ole_plan* plan, pcgsq_ole_exchange_fn fn, void* user);
 int pcgsq_ole_party_outputs(pcgsq_ole_plan* plan, uint64_t* x, uint64_t* z);
+int pcgsq_ole_party_deal_own(pcgsq_ole_plan* plan, uint64_t own_seed);
+int pcgsq_ole_party_set_exchange_mode(pcgsq_ole_plan* plan, int mode);
+/* One party's base OTs (Chou-Orlandi over P-256, two "base_ot" exchanges)
+ * without a device: for each direction d the party that receives extended
+ * OTs there holds both 128-bit seeds of every column (k0, k1: direction
+ * 1 - party), the other its choice bits s and the chosen seeds ks
+ * (direction party), ks[i] = s_i ? k1[i] : k0[i] of the other party. */
+typedef struct pcgsq_base_ots {
+  pcgsq_blk k0[128], k1[128];
+  pcgsq_blk s;
+  pcgsq_blk ks[128];
+} pcgsq_base_ots;
+int pcgsq_base_ots_run(int party, uint64_t seed, pcgsq_ole_exchange_fn fn, void* user,
+                       pcgsq_base_ots* out);
+/* What one party's last preprocessing left on the device, for tests of the
+ * protocol's invariants (no counterpart in the reference): the choice words
+ * of the OTs it received, the 16-byte messages it received, the two it sent,
+ * its adder AND triples [ninst][ops][a, b, c] (u64 bit-sliced words), its
+ * Delta-AND triples [ninst][levels*nk][a, b.lo, b.hi, c.lo, c.hi] and its
+ * AND-tree triples [ninst][levels*nk] (u32: a bits 0..3, b 4..7, c 8..11).
+ * *needed receives the size; host may be NULL to query it.  Synchronizes. */
+enum {
+  PCGSQ_PRE_CHOICE = 0,
+  PCGSQ_PRE_RECEIVED = 1,
+  PCGSQ_PRE_SENT0 = 2,
+  PCGSQ_PRE_SENT1 = 3,
+  PCGSQ_PRE_ADDER_TRIPLES = 4,
+  PCGSQ_PRE_DELTA_TRIPLES = 5,
+  PCGSQ_PRE_TREE_TRIPLES = 6
+};
+int pcgsq_ole_preprocessing_read(pcgsq_ole_plan* plan, int party, int what, void* host,
+                                 size_t bytes, size_t* needed);
 /* Deal the per-expansion inputs from `seed` (the seeded stand-in for the
  * reference's noise sampling and 2PC preprocessing, pcg_ole_impl.h:62-199):
- * the dealer runs on the device, asynchronously on the context stream. */
+ * the dealer runs on the device, asynchronously on the context stream.  On a
+ * one-party plan this is the trusted-dealer simulation described above. */
 int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed);
-/* The same deal with the 2PC preprocessing of pcg_ole_impl.h:114-199 run on
- * the device for both parties:
- *   alpha = off0 + off1 by the Kogge-Stone adder circuit
- *   (bool_circuit/adder.h:21-63) and DltSft by the F_DeltaShiftShare circuit
- *   (half_tree_dpf/dpf.cpp:172-240), both under GMW
- *   (bool_circuit/circuit.cpp:245-380), each AND triple made from two OTs
- *   as TripleGen::generate makes it (circuit.cpp:30-85);
- *   beta = pay0 * pay1 and the F_p Beaver triples' rs by Gilboa OT products
- *   (TripleGen::zp_multiply and zp_triple, circuit.cpp:91-200).
- * Masks and the OT senders' randomness come from the dealer in place of
- * emp::PRG, and the OTs are ideal in place of FerretCOT.  The DltSft, beta
- * and rs shares differ from pcgsq_ole_deal's but reconstruct to the same
- * values, so the expansion's outputs are identical. */
+/* Both-party plans: the same draws, with alpha, DltSft, beta and rs computed
+ * by the two parties' 2PC preprocessing (pcg_ole_impl.h:114-199) run
+ * in-process: base OTs (once per plan), IKNP random OTs both ways, the
+ * Kogge-Stone adder (bool_circuit/adder.h:21-63) and the F_DeltaShiftShare
+ * circuit (half_tree_dpf/dpf.cpp:172-240) under GMW
+ * (bool_circuit/circuit.cpp:245-380) with AND triples from the OTs, and beta
+ * and the Beaver triples' rs by Gilboa OT products (circuit.cpp:91-200).
+ * Each party uses `seed` as its own seed; every message moves by a device
+ * copy between the parties' buffers.  The DltSft, beta and rs shares differ
+ * from pcgsq_ole_deal's but reconstruct to the same values, so the
+ * expansion's outputs are identical. */
 int pcgsq_ole_deal_gmw(pcgsq_ole_plan* plan, uint64_t seed);
 /* Both parties' dealt DltSft shares, [2][c*c*t*t][levels][2^w - 1] blocks
  * (synchronizes). */

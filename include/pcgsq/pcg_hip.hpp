@@ -295,10 +295,11 @@ class OleExpander {
     return t;
   }
 
-  // gmw: run the 2PC preprocessing on the device instead of taking DltSft,
-  // beta and rs from the dealer (pcg_ole_impl.h:114-199): the position adder
-  // and DltSft circuits under GMW with AND triples made from OTs, and Gilboa
-  // OT products for beta and the Beaver triples; the outputs are the same.
+  // gmw: run the two parties' 2PC preprocessing (pcg_ole_impl.h:114-199) in
+  // this process instead of taking DltSft, beta and rs from the dealer: base
+  // OTs and IKNP random OTs, the position adder and DltSft circuits under
+  // GMW with AND triples from the OTs, and Gilboa OT products for beta and
+  // the Beaver triples; the outputs are the same.
   Output expand(uint64_t seed, bool gmw = false) {
     Output o;
     o.x0.resize(N_); o.z0.resize(N_); o.x1.resize(N_); o.z1.resize(N_);
@@ -314,20 +315,20 @@ class OleExpander {
   pcgsq_ole_plan* plan_ = nullptr;
 };
 
-// ---- One party of the 2PC expansion ------------------------------------------------
-// pcg_ole::PCG_OLE<P>::gen_and_expand as one party's process runs it
-// (pcg_ole_impl.h:46-490): the party expands only its own DPF trees, and each
-// exchange the reference makes over emp::NetIO goes through `exchange`, in the
-// reference's order, with the reference's comm_trace.h columns (phase, level,
-// batch, bytes).  `exchange` sends `bytes` from mine and receives as many from
-// the other party into peer; throwing aborts the expansion (rethrown as
-// std::runtime_error).  Each party owns its own device context, so the two
-// parties may run at once from two threads of one process, or as two
+// One party of the 2PC expansion (pcg_ole_impl.h:46-490), as one party's
+// process runs it with emp::NetIO replaced by `exchange`: every message of
+// the protocol goes through it, in the reference's order, with the columns
+// of the reference's comm_trace.h rows (phase, level, batch, bytes sent,
+// bytes received).  `exchange` sends send_bytes from `send` and receives
+// recv_bytes from the other party into `recv` (either may be 0; party 0
+// sends first).  An exception thrown by `exchange` aborts the expansion and
+// reaches the caller unchanged.  Each party owns its own device context, so
+// the two parties may run at once from two threads of one process, or as two
 // processes; the outputs of the two together equal OleExpander's.
 class OleParty {
  public:
-  using ExchangeFn = std::function<void(const char* phase, int level, int batch,
-                                        const void* mine, void* peer, size_t bytes)>;
+  using ExchangeFn = std::function<void(const char* phase, int level, int batch, const void* send,
+                                        size_t send_bytes, void* recv, size_t recv_bytes)>;
   struct Output {
     std::vector<uint64_t> x, z;   // this party's shares: z0 + z1 = x0 * x1 in R_p
   };
@@ -351,26 +352,52 @@ class OleParty {
   OleParty(const OleParty&) = delete;
   OleParty& operator=(const OleParty&) = delete;
 
-  // gmw: take this party's DltSft, beta and rs shares from the preprocessing
-  // run on the device (GMW circuits, Gilboa products; pcg_ole_impl.h:114-199)
-  // instead of from the dealer.
-  Output expand(uint64_t seed, const ExchangeFn& exchange, bool gmw = false) {
-    detail::check(gmw ? pcgsq_ole_deal_gmw(plan_, seed) : pcgsq_ole_deal(plan_, seed));
-    detail::check(pcgsq_ole_party_run(plan_, tramp, const_cast<ExchangeFn*>(&exchange)));
+  // The reference's deployment: this party's noise, Delta share and Beaver
+  // randomness from its own seed; alpha, DltSft, beta and rs computed with
+  // the other party (base OTs on the first call, then OT extension, GMW and
+  // Gilboa products: "base_ot", "ot_ext", "gilboa", "circuit_and" rows before
+  // the expansion's own).
+  Output expand_own(uint64_t own_seed, const ExchangeFn& exchange) {
+    detail::check(pcgsq_ole_party_deal_own(plan_, own_seed));
+    return run(exchange);
+  }
+  // Trusted-dealer simulation: both parties' shares from one shared seed
+  // (either party could recompute the other's), the run starting at the DPF.
+  Output expand(uint64_t seed, const ExchangeFn& exchange) {
+    detail::check(pcgsq_ole_deal(plan_, seed));
+    return run(exchange);
+  }
+  // One exchange per DPF level, Beaver open and circuit depth for a whole
+  // group of DPF blocks (the reference's batched arm) instead of per block.
+  void set_batched(bool batched) {
+    detail::check(pcgsq_ole_party_set_exchange_mode(
+        plan_, batched ? PCGSQ_EXCHANGE_BATCHED : PCGSQ_EXCHANGE_PER_BLOCK));
+  }
+
+ private:
+  struct Call {
+    const ExchangeFn* fn;
+    std::exception_ptr err;
+  };
+  Output run(const ExchangeFn& exchange) {
+    Call call{&exchange, nullptr};
+    const int rc = pcgsq_ole_party_run(plan_, tramp, &call);
+    if (call.err) std::rethrow_exception(call.err);
+    detail::check(rc);
     Output o;
     o.x.resize(N_);
     o.z.resize(N_);
     detail::check(pcgsq_ole_party_outputs(plan_, o.x.data(), o.z.data()));
     return o;
   }
-
- private:
-  static int tramp(void* user, const char* phase, int level, int batch, const void* mine,
-                   void* peer, size_t bytes) {
+  static int tramp(void* user, const char* phase, int level, int batch, const void* send,
+                   size_t send_bytes, void* recv, size_t recv_bytes) {
+    Call* c = static_cast<Call*>(user);
     try {
-      (*static_cast<const ExchangeFn*>(user))(phase, level, batch, mine, peer, bytes);
+      (*c->fn)(phase, level, batch, send, send_bytes, recv, recv_bytes);
       return 0;
     } catch (...) {
+      c->err = std::current_exception();
       return 1;
     }
   }

@@ -19,7 +19,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "pcgsq", "libpcgsq.so")
-SOURCES = ["dpf.hip", "convert.hip", "ntt.hip", "ntt_batch.hip", "dpf_batch.hip", "ole.hip", "serve.hip", "capi.hip"]
+SOURCES = ["dpf.hip", "convert.hip", "ntt.hip", "ntt_batch.hip", "dpf_batch.hip", "ot.hip",
+           "preproc.hip", "ole.hip", "serve.hip", "capi.hip"]
+LIBS = ["-lcrypto"]   # base OTs: OpenSSL's P-256 group and SHA-256 (ot.hip)
 ARCH = os.environ.get("PCGSQ_ARCH", "gfx950")
 
 CFLAGS = [
@@ -52,7 +54,7 @@ def build(jobs: int = 8) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, *LIBS, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

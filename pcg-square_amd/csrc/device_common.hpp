@@ -58,6 +58,26 @@ PCGSQ_HD uint64_t mulP(uint64_t q) {
   return (q << 62) - (q << 26) - (q << 25) + q;
 }
 
+// ---- the seeded dealer's draws ----------------------------------------------
+// U(seed, domain, a, b) = sm(sm(sm(seed ^ domain*K) ^ a) ^ b), sm = the
+// splitmix64 finalizer, K = 0xD6E8FEB86659FD93: every seeded draw of the
+// dealer and of a party's own randomness (the oracle restates it).
+PCGSQ_HD uint64_t sm64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+PCGSQ_HD uint64_t dealer_hd(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
+  return sm64(sm64(sm64(seed ^ ((uint64_t)domain * 0xD6E8FEB86659FD93ULL)) ^ a) ^ b);
+}
+
+// The public block W of the DPF roots, root_s = Delta_s ^ W (the reference
+// draws it from emp::PRG(&F_RAND_SEED), half_tree_dpf/dpf.cpp:25-26,291; it
+// cancels between the parties, so any public constant gives the same CWs).
+constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
+
 // Shoup: x*w mod P given wq = floor(w * 2^64 / P), w < P.  Any x < 2^64;
 // result in [0, 2P).  x*w - q*P is formed as (x*w + 3q*2^25) - (q + q*2^62).
 PCGSQ_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wq) {

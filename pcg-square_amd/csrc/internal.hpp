@@ -108,4 +108,28 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Page-locked host buffer that grows on demand (staging of exchanged bytes).
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need, const char* what) {
+    if (need <= bytes) return;
+    release();
+    if (need == 0) return;
+    hip_check(hipHostMalloc(&ptr, need, hipHostMallocDefault), what);
+    bytes = need;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~PinnedBuf() { release(); }
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
+
 }  // namespace pcgsq

@@ -29,154 +29,32 @@
 //   r_s = U(seed,6,g,s) % P, s_s = U(seed,7,g,s) % P,
 //   rs0 = U(seed,8,g,0) % P, rs1 = (r0+r1)(s0+s1) - rs0;
 //   public_a[i][j] = U(pub_seed,9,i,j) % P.
-// Preprocessing as the parties run it (pcgsq_ole_deal_gmw): the same draws,
-// except that
-//   * alpha and DltSft come from the parties' Kogge-Stone adder and
-//     F_DeltaShiftShare circuits under GMW, with masks from U(seed,14,g,j)
-//     and every AND triple made from two OTs (spec at gmw_and below);
-//   * beta and rs come from Gilboa OT products (spec at gilboa below).
-// The OTs are ideal (the receiver gets the chosen message) where the
-// reference runs FerretCOT; oracle/orc_gmw.c restates all of it.
+// Preprocessing as the parties run it (preproc.hip): each party draws its own
+// positions, payloads, Delta share and r, s with the formulas above and its
+// own seed, and the parties compute alpha, DltSft, beta and rs together by
+// OT extension, GMW and Gilboa products, exchanging every message:
+//   * one party per process: pcgsq_ole_party_deal_own, then
+//     pcgsq_ole_party_run runs the protocol through the caller's exchange;
+//   * both parties in one plan: pcgsq_ole_deal_gmw(seed) runs the two
+//     parties' protocol in-process (each with `seed` as its own seed), the
+//     messages moved by device copies.
+// The shares differ from the dealer's but reconstruct to the same values, so
+// the expansion's outputs are identical.
 #include <string.h>
 
 #include <algorithm>
 #include <vector>
 
+#include <memory>
+
 #include "context.hpp"
+#include "ot.hpp"
 #include "pcgsq.h"
+#include "preproc.hpp"
 
 namespace pcgsq {
 
 namespace {
-
-PCGSQ_HD uint64_t sm64(uint64_t z) {
-  z += 0x9e3779b97f4a7c15ULL;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
-PCGSQ_HD uint64_t dealer_hd(uint64_t seed, uint32_t domain, uint64_t a, uint64_t b) {
-  return sm64(sm64(sm64(seed ^ ((uint64_t)domain * 0xD6E8FEB86659FD93ULL)) ^ a) ^ b);
-}
-
-constexpr uint64_t kWlo = 0x0123456789abcdefULL, kWhi = 0xdeadbeefcafe0001ULL;
-
-// ---- GMW preprocessing (pcg_ole_impl.h:114-199), both parties per thread ----
-// XOR-shared words: bit i of (s0, s1) are the two parties' shares of wire i
-// of a bit-sliced gate layer.  Randomness is dealer domain 14, word
-// M(g, j) = U(seed, 14, g, j) for global instance g: input masks j = 0..7,
-// AND op o takes a0, a1, b0, b1, c0 = M(8 + 5o + 0..4), c1 = (a0^a1)&(b0^b1)^c0
-// (a gate in bit i of a layer uses bit i of each word).
-struct Sh {
-  uint64_t s0, s1;
-};
-
-// GMW AND (bool_circuit/circuit.cpp:245-256): open d = x ^ a, e = y ^ b;
-// party 0 gets d&e ^ d&b0 ^ e&a0 ^ c0, party 1 d&b1 ^ e&a1 ^ c1.  The
-// triple is made from two OTs as TripleGen::generate (circuit.cpp:30-85)
-// makes it: each party draws its a, b; party 0, the sender, draws r, s and
-// offers (r, r ^ a0) and (s, s ^ b0); party 1 chooses with b1 and a1.  Words
-// of op o: a0, a1, b0, b1, r, s = M(8 + 6o + 0..5).
-__device__ __forceinline__ Sh gmw_and(Sh x, Sh y, uint64_t seed, uint64_t g, uint32_t op) {
-  const uint64_t j = 8 + 6 * (uint64_t)op;
-  const uint64_t a0 = dealer_hd(seed, 14, g, j), a1 = dealer_hd(seed, 14, g, j + 1);
-  const uint64_t b0 = dealer_hd(seed, 14, g, j + 2), b1 = dealer_hd(seed, 14, g, j + 3);
-  const uint64_t r = dealer_hd(seed, 14, g, j + 4), s = dealer_hd(seed, 14, g, j + 5);
-  const uint64_t ot1 = r ^ (a0 & b1), ot2 = s ^ (b0 & a1);   // what party 1 receives
-  const uint64_t c0 = (a0 & b0) ^ r ^ s, c1 = (a1 & b1) ^ ot1 ^ ot2;
-  const uint64_t d = x.s0 ^ a0 ^ x.s1 ^ a1, e = y.s0 ^ b0 ^ y.s1 ^ b1;
-  return Sh{(d & e) ^ (d & b0) ^ (e & a0) ^ c0, (d & b1) ^ (e & a1) ^ c1};
-}
-
-// The Kogge-Stone adder (bool_circuit/adder.h:21-63) on nb-bit offsets owned
-// by party 0 (off0) and party 1 (off1), one bit-sliced layer per AND depth:
-// op 0 the generate ANDs, prefix step s (distance d) ops 2s-1 = P & G<<d and
-// 2s = P & P<<d over bits >= d.  Returns XOR shares of the nb+1 sum bits.
-__device__ Sh gmw_adder(uint64_t off0, uint64_t off1, int nb, uint64_t seed, uint64_t g) {
-  const uint64_t mask = (1ULL << nb) - 1;
-  const uint64_t m0 = dealer_hd(seed, 14, g, 0) & mask, m1 = dealer_hd(seed, 14, g, 1) & mask;
-  const Sh A{off0 ^ m0, m0}, B{m1, off1 ^ m1};  // inputs shared by their owners
-  Sh G = gmw_and(A, B, seed, g, 0);
-  G.s0 &= mask;
-  G.s1 &= mask;
-  Sh P{A.s0 ^ B.s0, A.s1 ^ B.s1};
-  const Sh P0 = P;
-  uint32_t s = 1;
-  for (int d = 1; d < nb; d <<= 1, ++s) {
-    const uint64_t hi = mask & ~((1ULL << d) - 1);
-    const Sh X = gmw_and(P, Sh{G.s0 << d, G.s1 << d}, seed, g, 2 * s - 1);
-    const Sh Y = gmw_and(P, Sh{P.s0 << d, P.s1 << d}, seed, g, 2 * s);
-    G.s0 ^= X.s0 & hi;
-    G.s1 ^= X.s1 & hi;
-    P.s0 = (P.s0 & ~hi) | (Y.s0 & hi);
-    P.s1 = (P.s1 & ~hi) | (Y.s1 & hi);
-  }
-  return Sh{P0.s0 ^ (G.s0 << 1), P0.s1 ^ (G.s1 << 1)};
-}
-
-// The F_DeltaShiftShare circuit (half_tree_dpf/dpf.cpp:172-240) on the alpha
-// shares and the Delta shares (each party's input shared by its masks),
-// writing both parties' DltSft shares [levels][nk].  Level i, child k uses
-// ops base = aops + (i*nk + k)*(w+1): its AND-tree gates base + j (bit 0),
-// its 128 Delta ANDs base + w-1 (bits 0..63) and base + w (bits 64..127).
-__device__ void gmw_dltsft(Sh alpha, Blk d0, Blk d1, int n, int w, int aops, uint64_t seed,
-                           uint64_t g, Blk* __restrict__ out0, Blk* __restrict__ out1) {
-  const int nk = (1 << w) - 1, levels = n / w;
-  const uint64_t m2 = dealer_hd(seed, 14, g, 2), m3 = dealer_hd(seed, 14, g, 3);
-  const Sh A{(alpha.s0 ^ m2) ^ m3, m2 ^ (alpha.s1 ^ m3)};
-  const uint64_t m4 = dealer_hd(seed, 14, g, 4), m5 = dealer_hd(seed, 14, g, 5);
-  const uint64_t m6 = dealer_hd(seed, 14, g, 6), m7 = dealer_hd(seed, 14, g, 7);
-  const Sh Xlo{d0.lo ^ m4 ^ m6, m4 ^ d1.lo ^ m6}, Xhi{d0.hi ^ m5 ^ m7, m5 ^ d1.hi ^ m7};
-  for (int i = 0; i < levels; ++i)
-    for (int k = 0; k < nk; ++k) {
-      const uint32_t base = (uint32_t)(aops + (i * nk + k) * (w + 1));
-      Sh f[4];
-      for (int c = 0; c < w; ++c) {
-        const int pos = n - w * (i + 1) + c;
-        f[c] = Sh{(A.s0 >> pos) & 1, (A.s1 >> pos) & 1};
-        if (!((k >> c) & 1)) f[c].s0 ^= 1;  // NOT: party 0 flips its share
-      }
-      int nf = w, tree = 0;
-      while (nf > 1) {  // pairwise, odd one carried (dpf.cpp:219-226)
-        int nn = 0;
-        for (int j = 0; j + 1 < nf; j += 2) {
-          const Sh z = gmw_and(f[j], f[j + 1], seed, g, base + (uint32_t)tree++);
-          f[nn++] = Sh{z.s0 & 1, z.s1 & 1};
-        }
-        if (nf & 1) f[nn++] = f[nf - 1];
-        nf = nn;
-      }
-      const Sh I{0ULL - f[0].s0, 0ULL - f[0].s1};  // the indicator in every bit
-      const Sh lo = gmw_and(I, Xlo, seed, g, base + (uint32_t)w - 1);
-      const Sh hi = gmw_and(I, Xhi, seed, g, base + (uint32_t)w);
-      out0[i * nk + k] = Blk{lo.s0, hi.s0};
-      out1[i * nk + k] = Blk{lo.s1, hi.s1};
-    }
-}
-
-// Gilboa's OT product (TripleGen::zp_multiply, circuit.cpp:91-153): party 0,
-// the sender, holds x; party 1 holds y < P.  For bit j < 62 of y the sender
-// offers (r_j, r_j + x 2^j mod P), r_j = U(seed, 15, g, 64q + j) % P for
-// product q of instance g, and the receiver takes the one its bit selects.
-// Shares: party 0 P - sum r_j, party 1 the sum of what it took.
-__device__ Sh gilboa(uint64_t x, uint64_t y, uint64_t seed, uint64_t g, uint32_t q) {
-  uint64_t rsum = 0, got = 0, xs = x;
-  for (int j = 0; j < 62; ++j) {
-    const uint64_t r = dealer_hd(seed, 15, g, 64 * (uint64_t)q + (uint64_t)j) % kP;
-    rsum = fadd(rsum, r);
-    got = fadd(got, (y >> j) & 1 ? fadd(r, xs) : r);
-    xs = fadd(xs, xs);
-  }
-  return Sh{fsub(0, rsum), got};
-}
-
-// AND ops of the nb-bit adder: the generate layer and two per prefix step.
-int gmw_adder_ops(int nb) {
-  int s = 0;
-  for (int d = 1; d < nb; d <<= 1) ++s;
-  return 1 + 2 * s;
-}
 
 // The dealer of the spec above, on the device: the seed is all a request
 // carries to the GPU (t and bin are powers of two since t | N).  Noise: one
@@ -236,7 +114,7 @@ __global__ void public_a_kernel(uint64_t* __restrict__ a, uint64_t pub_seed, int
 __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __restrict__ roots,
                                  Blk* __restrict__ dlt, uint64_t* __restrict__ bt,
                                  uint64_t seed, int c, int t_log, int levels, int nk, int w,
-                                 int D_log, uint64_t bin, int ninst, int gmw, int gmw_aops) {
+                                 int D_log, uint64_t bin, int ninst) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = 1 << t_log, B = t << t_log;
   if (idx >= B) return;
@@ -257,12 +135,6 @@ __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __rest
   const size_t ni = (size_t)ninst;
   roots[g] = Blk{d0.lo ^ kWlo, d0.hi ^ kWhi};
   roots[ni + g] = Blk{d1.lo ^ kWlo, d1.hi ^ kWhi};
-  if (gmw) {
-    // alpha and DltSft by the parties' GMW circuits (pcg_ole_impl.h:114-199)
-    const Sh alpha = gmw_adder(p0 & (bin - 1), p1 & (bin - 1), D_log - 1, seed, G);
-    gmw_dltsft(alpha, d0, d1, D_log, w, gmw_aops, seed, G, dlt + (size_t)g * levels * nk,
-               dlt + (ni + g) * levels * nk);
-  } else
   for (int i = 0; i < levels; ++i) {
     const int digit = (int)((alpha >> (D_log - w * (i + 1))) & (uint64_t)((1 << w) - 1));
     for (int k = 0; k < nk; ++k) {
@@ -284,21 +156,10 @@ __global__ void deal_inst_kernel(const uint64_t* __restrict__ noise, Blk* __rest
   R[ni + g] = r1;
   S[g] = s0;
   S[ni + g] = s1;
-  if (gmw) {
-    // beta by a Gilboa product (pcg_ole_impl.h:158-171); the Beaver triple's
-    // rs by zp_triple's two cross products (circuit.cpp:156-200)
-    const Sh b = gilboa(pay0, pay1, seed, G, 0);
-    BETA[g] = b.s0;
-    BETA[ni + g] = b.s1;
-    const Sh x1 = gilboa(r0, s1, seed, G, 1), x2 = gilboa(s0, r1, seed, G, 2);
-    RS[g] = fadd(fadd(fmul(r0, s0), x1.s0), x2.s0);
-    RS[ni + g] = fadd(fadd(fmul(r1, s1), x1.s1), x2.s1);
-  } else {
-    BETA[g] = dealer_hd(seed, 5, G, 0) % kP;
-    BETA[ni + g] = fsub(fmul(pay0, pay1), BETA[g]);
-    RS[g] = dealer_hd(seed, 8, G, 0) % kP;
-    RS[ni + g] = fsub(fmul(fadd(r0, r1), fadd(s0, s1)), RS[g]);
-  }
+  BETA[g] = dealer_hd(seed, 5, G, 0) % kP;
+  BETA[ni + g] = fsub(fmul(pay0, pay1), BETA[g]);
+  RS[g] = dealer_hd(seed, 8, G, 0) % kP;
+  RS[ni + g] = fsub(fmul(fadd(r0, r1), fadd(s0, s1)), RS[g]);
 }
 
 __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t* pay, int c, int t,
@@ -399,6 +260,14 @@ struct pcgsq_ole_plan {
   // working set
   DevBuf ct, part, cw, fw, out;   // ct: C|tau per leaf, or the fused gather's tau masks
   DevBuf tri;   // triple generation: first OLE's outputs [4][N] + shares [6][N]
+  // the 2PC preprocessing (preproc.hpp): the run of each party this plan holds
+  PreRun pre[2];
+  bool have_base = false;   // base OTs done for this plan's party pair
+  uint64_t sid = 0;         // next IKNP session (both parties count alike)
+  bool own_deal = false;    // one-party plan: the next run starts with the preprocessing
+  uint64_t own_seed = 0;
+  int exchange_mode = PCGSQ_EXCHANGE_PER_BLOCK;
+  PinnedBuf hsend, hrecv;   // host staging of one party's messages
 };
 
 namespace {
@@ -505,9 +374,7 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   hip_check(hipStreamSynchronize(pl->ctx->stream), "ole ahat");
 }
 
-// gmw: alpha and DltSft by the GMW circuits instead of the dealer's formula
-// (same Delta shares and positions; DltSft shares differ, their XOR agrees).
-void plan_deal(pcgsq_ole_plan* pl, uint64_t seed, bool gmw = false) {
+void plan_deal(pcgsq_ole_plan* pl, uint64_t seed) {
   const OleDims& d = pl->d;
   hipStream_t st = pl->ctx->stream;
   const int t_log = ilog((uint64_t)d.t);
@@ -517,9 +384,64 @@ void plan_deal(pcgsq_ole_plan* pl, uint64_t seed, bool gmw = false) {
   hipLaunchKernelGGL(deal_inst_kernel, dim3((unsigned)((d.B + 255) / 256), d.c, d.c), dim3(256),
                      0, st, pl->noise.as<uint64_t>(), pl->roots.as<Blk>(), pl->dlt.as<Blk>(),
                      pl->bt.as<uint64_t>(), seed, d.c, t_log, d.levels, d.nk, d.w, d.D_log,
-                     (uint64_t)d.bin, d.ninst, gmw ? 1 : 0, gmw_adder_ops(d.D_log - 1));
+                     (uint64_t)d.bin, d.ninst);
   hip_check(hipGetLastError(), "launch dealer");
   pl->dealt = true;
+  pl->own_deal = false;
+}
+
+PreOut plan_pre_out(pcgsq_ole_plan* pl) {
+  PreOut o;
+  o.noise = pl->noise.as<uint64_t>();
+  o.roots = pl->roots.as<Blk>();
+  o.dlt = pl->dlt.as<Blk>();
+  o.bt = pl->bt.as<uint64_t>();
+  return o;
+}
+
+// Both parties' preprocessing in one plan, in-process: the two parties' runs
+// step in lockstep on the plan's stream and every message is a device copy
+// from one party's send buffer into the other's receive buffer.  Base OTs
+// (host, libcrypto) once per plan.
+void plan_deal_gmw(pcgsq_ole_plan* pl, uint64_t seed) {
+  require(pl->party < 0, "ole_deal_gmw: a one-party plan runs its preprocessing with the other "
+                         "party: pcgsq_ole_party_deal_own, then pcgsq_ole_party_run");
+  const OleDims& d = pl->d;
+  hipStream_t st = pl->ctx->stream;
+  if (!pl->have_base) {
+    const PreShape sh = make_pre_shape(d.N, d.c, d.t, d.w);
+    BaseOtParty a(0, seed), b(1, seed);
+    const std::vector<uint8_t> a1 = a.round1(), b1 = b.round1();
+    const std::vector<uint8_t> a2 = a.round2(b1.data(), b1.size());
+    const std::vector<uint8_t> b2 = b.round2(a1.data(), a1.size());
+    std::unique_ptr<BaseOts> ba(new BaseOts), bb(new BaseOts);
+    a.finish(b2.data(), b2.size(), ba.get());
+    b.finish(a2.data(), a2.size(), bb.get());
+    pre_prepare(pl->pre[0], sh, 0, *ba);
+    pre_prepare(pl->pre[1], sh, 1, *bb);
+    pl->have_base = true;
+  }
+  const PreOut out = plan_pre_out(pl);
+  const uint64_t sid = pl->sid++;
+  pre_begin(pl->pre[0], seed, sid, out, st);
+  pre_begin(pl->pre[1], seed, sid, out, st);
+  for (;;) {
+    PreMsg m0, m1;
+    const bool more0 = pre_step(pl->pre[0], &m0);
+    const bool more1 = pre_step(pl->pre[1], &m1);
+    require(more0 == more1, "ole_deal_gmw: the parties' protocols diverged", kErrInternal);
+    if (!more0) break;
+    require(m0.send_bytes == m1.recv_bytes && m1.send_bytes == m0.recv_bytes,
+            "ole_deal_gmw: message sizes disagree", kErrInternal);
+    if (m0.send_bytes)
+      hip_check(hipMemcpyAsync(m1.d_recv, m0.d_send, m0.send_bytes, hipMemcpyDeviceToDevice, st),
+                "ole_deal_gmw exchange");
+    if (m1.send_bytes)
+      hip_check(hipMemcpyAsync(m0.d_recv, m1.d_send, m1.send_bytes, hipMemcpyDeviceToDevice, st),
+                "ole_deal_gmw exchange");
+  }
+  pl->dealt = true;
+  pl->own_deal = false;
 }
 
 // DPF blocks + output layer: leaves g_ij (folded) for both parties in fw.
@@ -533,7 +455,6 @@ void plan_run_blocks(pcgsq_ole_plan* pl) {
   const int nparts = leaf_nparts(d);
   const bool fused = fused_gather(d);
   uint64_t* G = pl->fw.as<uint64_t>();                                  // [2][nblk][N]
-  uint64_t* E = G + (size_t)2 * d.nblk * d.N;                          // [2][c][N]
   const uint64_t* BT = pl->bt.as<uint64_t>();
   for (int b0 = 0; b0 < d.nblk; b0 += gb) {
     const size_t i0 = (size_t)b0 * d.B;   // first instance of the group
@@ -635,26 +556,94 @@ struct PartyExchange {
   pcgsq_ole_exchange_fn fn;
   void* user;
   int B, nk, gb;
+  int mode;   // PCGSQ_EXCHANGE_PER_BLOCK or PCGSQ_EXCHANGE_BATCHED
 };
 
-// The DPF's per-level exchange of a group of blocks, one call per block as
-// the reference exchanges one DPF batch per block (dpf.cpp:535-551).
+void party_call(const PartyExchange& x, const char* phase, int level, int batch, const void* send,
+                size_t send_bytes, void* recv, size_t recv_bytes) {
+  if (x.fn(x.user, phase, level, batch, send, send_bytes, recv, recv_bytes) != 0)
+    throw Error(kErrCallback, std::string("ole: exchange callback failed at ") + phase);
+}
+
+// The DPF's per-level exchange of a group of blocks: one call per block, as
+// the reference's serial arm exchanges one DPF batch per block
+// (dpf.cpp:535-551), or one call for the whole group (its batched arm,
+// GPU_baseline/e2e_full_bench.cu:128-153).
 int party_dpf_exchange(void* user, int level, const Blk* mine, Blk* peer, size_t count) {
   const PartyExchange* x = static_cast<const PartyExchange*>(user);
   const size_t per = (size_t)x->B * x->nk;
   if (count != per * (size_t)x->gb) return -1;
+  if (x->mode == PCGSQ_EXCHANGE_BATCHED)
+    return x->fn(x->user, "dpf_gen", level, x->B * x->gb, mine, count * sizeof(Blk), peer,
+                 count * sizeof(Blk));
   for (int bb = 0; bb < x->gb; ++bb) {
-    const int rc = x->fn(x->user, "dpf_gen", level, x->B, mine + bb * per, peer + bb * per,
-                         per * sizeof(Blk));
+    const int rc = x->fn(x->user, "dpf_gen", level, x->B, mine + bb * per, per * sizeof(Blk),
+                         peer + bb * per, per * sizeof(Blk));
     if (rc != 0) return rc;
   }
   return 0;
 }
 
-void party_exchange(const PartyExchange& x, const char* phase, int level, const void* mine,
-                    void* peer, size_t bytes) {
-  if (x.fn(x.user, phase, level, x.B, mine, peer, bytes) != 0)
-    throw Error(kErrCallback, std::string("ole: exchange callback failed at ") + phase);
+// One message of the preprocessing protocol through the host: copy out,
+// call the exchange (per DPF block when the message has per-block pieces
+// and the mode is per block), copy in.
+void party_msg(pcgsq_ole_plan* pl, const PartyExchange& x, const PreMsg& m) {
+  hipStream_t st = pl->ctx->stream;
+  pl->hsend.ensure(std::max<size_t>(m.send_bytes, 1), "ole party staging");
+  pl->hrecv.ensure(std::max<size_t>(m.recv_bytes, 1), "ole party staging");
+  if (m.send_bytes)
+    hip_check(hipMemcpyAsync(pl->hsend.ptr, m.d_send, m.send_bytes, hipMemcpyDeviceToHost, st),
+              "ole party message");
+  hip_check(hipStreamSynchronize(st), "ole party message");
+  const int nb = x.mode == PCGSQ_EXCHANGE_BATCHED ? 1 : m.nblocks;
+  require(m.send_bytes % (size_t)nb == 0 && m.recv_bytes % (size_t)nb == 0,
+          "ole: message does not split into blocks", kErrInternal);
+  const size_t sb = m.send_bytes / nb, rb = m.recv_bytes / nb;
+  const int batch = m.batch * (m.nblocks / nb);
+  for (int b = 0; b < nb; ++b)
+    party_call(x, m.phase, m.level, batch, pl->hsend.as<uint8_t>() + (size_t)b * sb, sb,
+               pl->hrecv.as<uint8_t>() + (size_t)b * rb, rb);
+  if (m.recv_bytes)
+    hip_check(hipMemcpyAsync(m.d_recv, pl->hrecv.ptr, m.recv_bytes, hipMemcpyHostToDevice, st),
+              "ole party message");
+}
+
+}  // namespace
+
+namespace pcgsq {
+
+// Base OTs of one party through an exchange: two "base_ot" rows (the
+// parties' A points, then their 128 B points).
+void base_ots_exchange(int party, uint64_t seed, pcgsq_ole_exchange_fn fn, void* user,
+                       BaseOts* out) {
+  BaseOtParty bo(party, seed);
+  const PartyExchange x{fn, user, 0, 0, 1, PCGSQ_EXCHANGE_PER_BLOCK};
+  const std::vector<uint8_t> m1 = bo.round1();
+  std::vector<uint8_t> p1(BaseOtParty::kMsg1Bytes), p2(BaseOtParty::kMsg2Bytes);
+  party_call(x, "base_ot", 0, 1, m1.data(), m1.size(), p1.data(), p1.size());
+  const std::vector<uint8_t> m2 = bo.round2(p1.data(), p1.size());
+  party_call(x, "base_ot", 1, kKappa, m2.data(), m2.size(), p2.data(), p2.size());
+  bo.finish(p2.data(), p2.size(), out);
+}
+
+}  // namespace pcgsq
+
+namespace {
+
+// One party's preprocessing (pcg_ole_impl.h:62-199) at the start of its run:
+// base OTs the first time, then every message of preproc.hpp's protocol.
+void party_preprocess(pcgsq_ole_plan* pl, const PartyExchange& x) {
+  const OleDims& d = pl->d;
+  const int p = pl->party;
+  if (!pl->have_base) {
+    std::unique_ptr<BaseOts> b(new BaseOts);
+    base_ots_exchange(p, pl->own_seed, x.fn, x.user, b.get());
+    pre_prepare(pl->pre[p], make_pre_shape(d.N, d.c, d.t, d.w), p, *b);
+    pl->have_base = true;
+  }
+  pre_begin(pl->pre[p], pl->own_seed, pl->sid++, plan_pre_out(pl), pl->ctx->stream);
+  PreMsg m;
+  while (pre_step(pl->pre[p], &m)) party_msg(pl, x, m);
 }
 
 // This party's blocks: the DPF with the per-level exchange, the sums, the
@@ -673,10 +662,10 @@ void plan_run_party(pcgsq_ole_plan* pl, pcgsq_ole_exchange_fn fn, void* user) {
   const bool fused = fused_gather(d);
   uint64_t* G = pl->fw.as<uint64_t>() + (size_t)p * d.nblk * d.N;  // this party's g [nblk][N]
   const uint64_t* BT = pl->bt.as<uint64_t>();
-  PartyExchange px{fn, user, d.B, d.nk, gb};
+  PartyExchange px{fn, user, d.B, d.nk, gb, pl->exchange_mode};
+  if (pl->own_deal) party_preprocess(pl, px);
   std::vector<uint64_t> sums(2 * gi), bt(4 * gi), cw_full(gi);
-  std::vector<uint64_t> de_my(2 * (size_t)d.B), de_peer(2 * (size_t)d.B);
-  std::vector<uint64_t> cw_my(d.B), cw_peer(d.B);
+  std::vector<uint64_t> de_my(2 * gi), de_peer(2 * gi), cw_my(gi), cw_peer(gi);
   uint64_t* dsums = pl->cw.as<uint64_t>() + ni;  // scratch after the CWs
   for (int b0 = 0; b0 < d.nblk; b0 += gb) {
     const size_t i0 = (size_t)b0 * d.B;
@@ -720,29 +709,37 @@ void plan_run_party(pcgsq_ole_plan* pl, pcgsq_ole_exchange_fn fn, void* user) {
     const uint64_t* sumT = sums.data() + gi;
     const uint64_t *beta = bt.data(), *rr = bt.data() + gi, *ss = bt.data() + 2 * gi,
                    *rs = bt.data() + 3 * gi;
-    for (int bb = 0; bb < gb; ++bb) {
-      const size_t o = (size_t)bb * d.B;
-      for (int k = 0; k < d.B; ++k) {
-        uint64_t A = freduce64(sumT[o + k]);
-        if (p == 1) A = fneg(A);
-        uint64_t Bs = sumC[o + k];
-        if (p == 0) Bs = fneg(Bs);
-        Bs = fadd(Bs, beta[o + k]);
-        de_my[k] = fsub(A, rr[o + k]);
-        de_my[d.B + k] = fsub(Bs, ss[o + k]);
-      }
-      party_exchange(px, "beaver_de", b0 + bb, de_my.data(), de_peer.data(),
-                     de_my.size() * sizeof(uint64_t));
-      for (int k = 0; k < d.B; ++k) {
-        const uint64_t dv = fadd(de_my[k], de_peer[k]);
-        const uint64_t ev = fadd(de_my[d.B + k], de_peer[d.B + k]);
-        uint64_t c = fadd(fadd(fmul(dv, ss[o + k]), fmul(ev, rr[o + k])), rs[o + k]);
-        if (p == 0) c = fadd(c, fmul(dv, ev));
-        cw_my[k] = c;
-      }
-      party_exchange(px, "beaver_cw", b0 + bb, cw_my.data(), cw_peer.data(),
-                     cw_my.size() * sizeof(uint64_t));
-      for (int k = 0; k < d.B; ++k) cw_full[o + k] = fadd(cw_my[k], cw_peer[k]);
+    // blocks [bb, bb + nbb) of the group: d, e of every instance ([block][d | e]),
+    // one "beaver_de" and one "beaver_cw" exchange
+    const int nbb = pl->exchange_mode == PCGSQ_EXCHANGE_BATCHED ? gb : 1;
+    for (int bb = 0; bb < gb; bb += nbb) {
+      const size_t o0 = (size_t)bb * d.B, n = (size_t)nbb * d.B;
+      for (int q = 0; q < nbb; ++q)
+        for (int k = 0; k < d.B; ++k) {
+          const size_t o = o0 + (size_t)q * d.B + k;
+          uint64_t A = freduce64(sumT[o]);
+          if (p == 1) A = fneg(A);
+          uint64_t Bs = sumC[o];
+          if (p == 0) Bs = fneg(Bs);
+          Bs = fadd(Bs, beta[o]);
+          de_my[2 * (size_t)q * d.B + k] = fsub(A, rr[o]);
+          de_my[(2 * (size_t)q + 1) * d.B + k] = fsub(Bs, ss[o]);
+        }
+      party_call(px, "beaver_de", b0 + bb, (int)n, de_my.data(), 2 * n * sizeof(uint64_t),
+                 de_peer.data(), 2 * n * sizeof(uint64_t));
+      for (int q = 0; q < nbb; ++q)
+        for (int k = 0; k < d.B; ++k) {
+          const size_t o = o0 + (size_t)q * d.B + k;
+          const size_t id = 2 * (size_t)q * d.B + k, ie = id + d.B;
+          const uint64_t dv = fadd(de_my[id], de_peer[id]);
+          const uint64_t ev = fadd(de_my[ie], de_peer[ie]);
+          uint64_t c = fadd(fadd(fmul(dv, ss[o]), fmul(ev, rr[o])), rs[o]);
+          if (p == 0) c = fadd(c, fmul(dv, ev));
+          cw_my[(size_t)q * d.B + k] = c;
+        }
+      party_call(px, "beaver_cw", b0 + bb, (int)n, cw_my.data(), n * sizeof(uint64_t),
+                 cw_peer.data(), n * sizeof(uint64_t));
+      for (size_t k = 0; k < n; ++k) cw_full[o0 + k] = fadd(cw_my[k], cw_peer[k]);
     }
     hip_check(hipMemcpyAsync(pl->cw.as<uint64_t>() + i0, cw_full.data(), gi * sizeof(uint64_t),
                              hipMemcpyHostToDevice, st), "ole party cw");
@@ -899,9 +896,75 @@ int pcgsq_ole_deal(pcgsq_ole_plan* plan, uint64_t seed) {
 int pcgsq_ole_deal_gmw(pcgsq_ole_plan* plan, uint64_t seed) {
   return guarded([&] {
     require(plan, "ole_deal_gmw: null plan");
-    require(plan->d.D_log >= 1 && plan->d.D_log <= 63, "ole_deal_gmw: domain out of range");
     CtxGuard g(plan->ctx);
-    plan_deal(plan, seed, true);
+    plan_deal_gmw(plan, seed);
+  });
+}
+
+int pcgsq_ole_party_deal_own(pcgsq_ole_plan* plan, uint64_t own_seed) {
+  return guarded([&] {
+    require(plan, "ole_party_deal_own: null plan");
+    require(plan->party == 0 || plan->party == 1, "ole_party_deal_own: not a one-party plan");
+    (void)make_pre_shape(plan->d.N, plan->d.c, plan->d.t, plan->d.w);   // shape check
+    plan->own_seed = own_seed;
+    plan->own_deal = true;
+    plan->dealt = true;
+  });
+}
+
+int pcgsq_ole_party_set_exchange_mode(pcgsq_ole_plan* plan, int mode) {
+  return guarded([&] {
+    require(plan, "ole_party_set_exchange_mode: null plan");
+    require(mode == PCGSQ_EXCHANGE_PER_BLOCK || mode == PCGSQ_EXCHANGE_BATCHED,
+            "ole_party_set_exchange_mode: unknown mode");
+    plan->exchange_mode = mode;
+  });
+}
+
+int pcgsq_ole_preprocessing_read(pcgsq_ole_plan* plan, int party, int what, void* host,
+                                 size_t bytes, size_t* needed) {
+  return guarded([&] {
+    require(plan && needed, "ole_preprocessing_read: null argument");
+    require(party == 0 || party == 1, "ole_preprocessing_read: party must be 0 or 1");
+    require(plan->party < 0 || plan->party == party,
+            "ole_preprocessing_read: this plan does not hold that party");
+    const PreRun& r = plan->pre[party];
+    require(r.stage >= 0, "ole_preprocessing_read: no preprocessing ran for that party");
+    const PreShape& sh = r.sh;
+    const size_t ni = (size_t)sh.ninst, le = (size_t)sh.levels * sh.nk;
+    const size_t mr = sh.m[1 - party], ms = sh.m[party];
+    const DevBuf* src = nullptr;
+    size_t n = 0;
+    switch (what) {
+      case PCGSQ_PRE_CHOICE: src = &r.choice; n = mr / 8; break;
+      case PCGSQ_PRE_RECEIVED: src = &r.mc; n = mr * sizeof(Blk); break;
+      case PCGSQ_PRE_SENT0: src = &r.m0; n = ms * sizeof(Blk); break;
+      case PCGSQ_PRE_SENT1: src = &r.m1; n = ms * sizeof(Blk); break;
+      case PCGSQ_PRE_ADDER_TRIPLES: src = &r.tri_add; n = ni * sh.ops * 3 * 8; break;
+      case PCGSQ_PRE_DELTA_TRIPLES: src = &r.tri_dlt; n = ni * le * 5 * 8; break;
+      case PCGSQ_PRE_TREE_TRIPLES: src = &r.tri_tree; n = sh.w >= 2 ? ni * le * 4 : 0; break;
+      default: require(false, "ole_preprocessing_read: unknown item");
+    }
+    *needed = n;
+    if (!host) return;
+    require(bytes >= n, "ole_preprocessing_read: host buffer too small");
+    CtxGuard g(plan->ctx);
+    if (n)
+      hip_check(hipMemcpyAsync(host, src->ptr, n, hipMemcpyDeviceToHost, plan->ctx->stream),
+                "ole preprocessing read");
+    hip_check(hipStreamSynchronize(plan->ctx->stream), "ole preprocessing read");
+  });
+}
+
+int pcgsq_base_ots_run(int party, uint64_t seed, pcgsq_ole_exchange_fn fn, void* user,
+                       pcgsq_base_ots* out) {
+  return guarded([&] {
+    require(fn && out, "base_ots_run: null argument");
+    require(party == 0 || party == 1, "base_ots_run: party must be 0 or 1");
+    static_assert(sizeof(pcgsq_base_ots) == sizeof(BaseOts), "pcgsq_base_ots layout");
+    std::unique_ptr<BaseOts> b(new BaseOts);
+    base_ots_exchange(party, seed, fn, user, b.get());
+    memcpy(out, b.get(), sizeof(BaseOts));
   });
 }
 

@@ -44,7 +44,13 @@ class Params(C.Structure):
 
 EXCHANGE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 OLE_EXCHANGE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p,
-                              C.c_void_p, C.c_size_t)
+                              C.c_size_t, C.c_void_p, C.c_size_t)
+EXCHANGE_PER_BLOCK, EXCHANGE_BATCHED = 0, 1
+
+
+class BaseOts(C.Structure):
+    """One party's base-OT results (pcgsq_base_ots)."""
+    _fields_ = [("k0", Blk * 128), ("k1", Blk * 128), ("s", Blk), ("ks", Blk * 128)]
 
 _lib = None
 u64p = C.POINTER(C.c_uint64)
@@ -111,6 +117,11 @@ def lib() -> C.CDLL:
                                          C.POINTER(vp)]
     L.pcgsq_ole_party_run.argtypes = [vp, OLE_EXCHANGE_FN, vp]
     L.pcgsq_ole_party_outputs.argtypes = [vp, u64p, u64p]
+    L.pcgsq_ole_party_deal_own.argtypes = [vp, C.c_uint64]
+    L.pcgsq_ole_party_set_exchange_mode.argtypes = [vp, C.c_int]
+    L.pcgsq_base_ots_run.argtypes = [C.c_int, C.c_uint64, OLE_EXCHANGE_FN, vp, C.POINTER(BaseOts)]
+    L.pcgsq_ole_preprocessing_read.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t,
+                                               C.POINTER(C.c_size_t)]
     L.pcgsq_ole_plan_bytes.argtypes = [vp]
     L.pcgsq_ole_plan_bytes.restype = C.c_size_t
     L.pcgsq_ole_deal.argtypes = [vp, C.c_uint64]
@@ -573,11 +584,11 @@ class OlePlan:
         return lib().pcgsq_ole_plan_bytes(self.h)
 
     def deal(self, seed: int, gmw: bool = False) -> None:
-        """Deal the inputs of `seed`; gmw=True runs the parties' preprocessing
-        on the device (position adder and DltSft circuits under GMW with AND
-        triples made from OTs, Gilboa products for beta and the Beaver
-        triples) instead of taking DltSft, beta and rs from the dealer (same
-        outputs)."""
+        """Deal the inputs of `seed`; gmw=True runs the two parties' 2PC
+        preprocessing in-process (base OTs, IKNP random OTs, the position
+        adder and DltSft circuits under GMW, Gilboa products for beta and the
+        Beaver triples, every message a device copy) instead of taking
+        DltSft, beta and rs from the dealer (same outputs)."""
         _check((lib().pcgsq_ole_deal_gmw if gmw else lib().pcgsq_ole_deal)(self.h, seed))
 
     def dealt_dltsft(self) -> np.ndarray:
@@ -593,6 +604,24 @@ class OlePlan:
         out = np.empty((4, 2, self.c * self.c * self.t * self.t), dtype=np.uint64)
         _check(lib().pcgsq_ole_dealt_products(self.h, out.ctypes.data))
         return dict(zip(("beta", "r", "s", "rs"), out))
+
+    PRE_ITEMS = {"choice": (0, np.uint64), "received": (1, np.uint64), "sent0": (2, np.uint64),
+                 "sent1": (3, np.uint64), "adder_triples": (4, np.uint64),
+                 "delta_triples": (5, np.uint64), "tree_triples": (6, np.uint32)}
+
+    def preprocessing(self, party: int, item: str) -> np.ndarray:
+        """What `party`'s last 2PC preprocessing left on the device (for
+        checking the protocol): "choice" words of the OTs it received,
+        "received" / "sent0" / "sent1" messages ([m][2] words), its
+        "adder_triples" [ninst][ops][3], "delta_triples" [ninst][L*nk][5],
+        "tree_triples" [ninst][L*nk] (uint32)."""
+        what, dt = self.PRE_ITEMS[item]
+        n = C.c_size_t()
+        _check(lib().pcgsq_ole_preprocessing_read(self.h, party, what, None, 0, C.byref(n)))
+        out = np.empty(n.value // np.dtype(dt).itemsize, dtype=dt)
+        _check(lib().pcgsq_ole_preprocessing_read(self.h, party, what, out.ctypes.data,
+                                                  out.nbytes, C.byref(n)))
+        return out
 
     def run(self) -> None:
         _check(lib().pcgsq_ole_run(self.h))
@@ -627,13 +656,41 @@ class OlePlan:
         return a0, b0, c0, a1, b1, c1
 
 
+def _ole_trampoline(exchange, err):
+    """A C exchange callback calling exchange(phase, level, batch, send,
+    nrecv) -> the nrecv bytes received; the callable's exception is kept in
+    `err` for the caller to re-raise after the C call."""
+    def tramp(user, phase, level, batch, send, send_bytes, recv, recv_bytes):
+        try:
+            m = (np.ctypeslib.as_array((C.c_uint8 * send_bytes).from_address(send)).copy()
+                 if send_bytes else np.empty(0, dtype=np.uint8))
+            got = exchange(phase.decode(), level, batch, m, recv_bytes)
+            got = (np.empty(0, dtype=np.uint8) if got is None else
+                   np.ascontiguousarray(got, dtype=np.uint8).reshape(-1))
+            if got.size != recv_bytes:
+                raise ValueError(f"exchange returned {got.size} bytes, expected {recv_bytes}")
+            if recv_bytes:
+                C.memmove(recv, got.ctypes.data, recv_bytes)
+            return 0
+        except BaseException as e:  # re-raised after the run
+            err.append(e)
+            return 1
+    return OLE_EXCHANGE_FN(tramp)
+
+
 class OleParty(OlePlan):
     """One party of the 2PC expansion, as one party's process runs it
     (pcgsq_ole_party_*; pcg_ole_impl.h:46-490 with emp::NetIO replaced by a
-    callable).  run(exchange) calls exchange(phase, level, batch, mine) for
-    every exchange of the protocol ("dpf_gen" per DPF level and block,
-    "beaver_de" and "beaver_cw" per block); mine is a uint8 array, and the
-    callable returns the other party's bytes of the same length."""
+    callable).  run(exchange) calls exchange(phase, level, batch, send, nrecv)
+    for every message of the protocol, in order ("base_ot", "ot_ext",
+    "gilboa", "circuit_and" when dealt with deal_own; "dpf_gen" per DPF
+    level and block, "beaver_de" and "beaver_cw" per block): send is a uint8
+    array for the other party, and the callable returns the nrecv bytes the
+    other party sent (party 0 sends first).
+
+    deal_own(own_seed) draws only this party's inputs, and the run computes
+    the shared ones with the other party (OT extension, GMW, Gilboa); deal(seed)
+    is the trusted-dealer simulation (both parties' shares from one seed)."""
 
     def __init__(self, ctx: Context, N: int, c: int, t: int, party: int, w: int = 1,
                  prg: int = PRG_CHACHA8, pub_seed: int = 0, hbm_budget: int = 0):
@@ -645,23 +702,20 @@ class OleParty(OlePlan):
                                             C.byref(h)))
         self.h = h
 
+    def deal_own(self, own_seed: int) -> None:
+        """This party's own draws from its own seed; the next run starts with
+        the 2PC preprocessing (pcg_ole_impl.h:62-199) through the exchange."""
+        _check(lib().pcgsq_ole_party_deal_own(self.h, own_seed))
+
+    def set_exchange_mode(self, mode: int) -> None:
+        """EXCHANGE_PER_BLOCK (the reference's serial arm) or EXCHANGE_BATCHED
+        (one exchange per DPF level, Beaver open and circuit depth for a whole
+        group of blocks, its batched arm)."""
+        _check(lib().pcgsq_ole_party_set_exchange_mode(self.h, mode))
+
     def run(self, exchange) -> None:
         err = []
-
-        def tramp(user, phase, level, batch, mine, peer, nbytes):
-            try:
-                m = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(mine)).copy()
-                got = np.ascontiguousarray(exchange(phase.decode(), level, batch, m),
-                                           dtype=np.uint8).reshape(-1)
-                if got.size != nbytes:
-                    raise ValueError(f"exchange returned {got.size} bytes, expected {nbytes}")
-                C.memmove(peer, got.ctypes.data, nbytes)
-                return 0
-            except BaseException as e:  # re-raised after the run
-                err.append(e)
-                return 1
-
-        cb = OLE_EXCHANGE_FN(tramp)
+        cb = _ole_trampoline(exchange, err)
         rc = lib().pcgsq_ole_party_run(self.h, cb, None)
         if err:
             raise err[0]
@@ -676,6 +730,19 @@ class OleParty(OlePlan):
         return x, z
 
 
+def base_ots_run(party: int, seed: int, exchange) -> BaseOts:
+    """One party's base OTs through exchange(phase, level, batch, send,
+    nrecv) (no device needed)."""
+    err = []
+    cb = _ole_trampoline(exchange, err)
+    out = BaseOts()
+    rc = lib().pcgsq_base_ots_run(party, seed, cb, None, C.byref(out))
+    if err:
+        raise err[0]
+    _check(rc)
+    return out
+
+
 def dist_exchange(peer: int, group=None):
     """An OleParty exchange over torch.distributed point-to-point messages,
     for one party per process as the reference's 2PC runs: the pair of
@@ -688,15 +755,19 @@ def dist_exchange(peer: int, group=None):
 
     me = dist.get_rank(group)
 
-    def exchange(phase, level, batch, mine):
-        out = torch.from_numpy(np.ascontiguousarray(mine))
-        got = torch.empty_like(out)
+    def exchange(phase, level, batch, send, nrecv):
+        out = torch.from_numpy(np.ascontiguousarray(send, dtype=np.uint8))
+        got = torch.empty(nrecv, dtype=torch.uint8)
         if me < peer:
-            dist.send(out, peer, group=group)
-            dist.recv(got, peer, group=group)
+            if out.numel():
+                dist.send(out, peer, group=group)
+            if nrecv:
+                dist.recv(got, peer, group=group)
         else:
-            dist.recv(got, peer, group=group)
-            dist.send(out, peer, group=group)
+            if nrecv:
+                dist.recv(got, peer, group=group)
+            if out.numel():
+                dist.send(out, peer, group=group)
         return got.numpy()
 
     return exchange
@@ -717,13 +788,13 @@ class CommTrace:
     def wrap(self, exchange):
         import time as _t
 
-        def traced(phase, level, batch, mine):
+        def traced(phase, level, batch, send, nrecv):
             t0 = _t.perf_counter_ns()
-            peer = exchange(phase, level, batch, mine)
+            peer = exchange(phase, level, batch, send, nrecv)
             t1 = _t.perf_counter_ns()
-            n = int(np.asarray(mine).nbytes)
-            self.rows.append((self.party, self.context, phase, level, batch, n,
-                              int(np.asarray(peer).nbytes), t0, t0, t1, n))
+            n = int(np.asarray(send).nbytes)
+            self.rows.append((self.party, self.context, phase, level, batch, n, int(nrecv),
+                              t0, t0, t1, n + int(nrecv)))
             return peer
         return traced
 

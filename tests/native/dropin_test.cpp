@@ -16,7 +16,10 @@
 //      (plus an exchange that throws: its exception reaches the caller as is)
 //   7. OleParty: the two parties in two threads, every exchange through an
 //      in-process channel standing in for emp::NetIO; each party's (x, z)
-//      equals OleExpander's, with one "dpf_gen" row per DPF level and block
+//      equals OleExpander's, with one "dpf_gen" row per DPF level and block;
+//      then each party from its own seed, the 2PC preprocessing (base OTs,
+//      OT extension, Gilboa, one "circuit_and" row per AND depth and block)
+//      run between the threads first
 // Exit 0 = PASS, 1 = FAIL, 77 = no device (the reference's skip code).
 #include <stdio.h>
 #include <string.h>
@@ -230,38 +233,54 @@ int main() {
     const bool same = og.x0 == o.x0 && og.z0 == o.z0 && og.x1 == o.x1 && og.z1 == o.z1;
     CHECK(same, "GMW-preprocessed expansion == dealer expansion");
     printf("GMW preprocessing N=%d: %s\n", N, same ? "same outputs" : "DIFFERS");
-    // 7. one party per thread, exchanges through the channel
-    Channel ch;
-    pcg_hip::OleParty::Output po[2];
-    int dpf_rows[2] = {0, 0}, rows[2] = {0, 0};
-    std::string err[2];
-    auto run = [&](int p) {
-      try {
-        pcg_hip::OleParty party(p, N, 2, 16);
-        po[p] = party.expand(3, [&](const char* phase, int, int, const void* mine, void* peer,
-                                    size_t bytes) {
-          ch.send(1 - p, mine, bytes);
-          ch.recv(p, peer, bytes);
-          ++rows[p];
-          dpf_rows[p] += strcmp(phase, "dpf_gen") == 0;
-        });
-      } catch (const std::exception& e) {
-        err[p] = e.what();
-        ch.abort();
-      }
-    };
-    std::thread t0(run, 0), t1(run, 1);
-    t0.join();
-    t1.join();
-    CHECK(err[0].empty() && err[1].empty(), "both parties finish");
-    const bool pok = po[0].x == o.x0 && po[0].z == o.z0 && po[1].x == o.x1 && po[1].z == o.z1;
-    CHECK(pok, "OleParty outputs == OleExpander outputs");
-    const int levels = 9, blocks = 4;   // log2(2N/t) DPF levels at w = 1; c*c blocks
-    CHECK(dpf_rows[0] == levels * blocks && dpf_rows[1] == levels * blocks,
-          "one dpf_gen exchange per DPF level and block");
-    CHECK(rows[0] == (levels + 2) * blocks && rows[1] == rows[0], "two Beaver opens per block");
-    printf("two parties N=%d: %s (%d exchanges each)%s%s\n", N, pok ? "same outputs" : "DIFFER",
-           rows[0], err[0].c_str(), err[1].c_str());
+    // 7. one party per thread, exchanges through the channel: the
+    // trusted-dealer simulation (deal), then each party from its own seed
+    // with the 2PC preprocessing run between the threads (deal_own)
+    for (int own = 0; own < 2; ++own) {
+      Channel ch;
+      pcg_hip::OleParty::Output po[2];
+      int dpf_rows[2] = {0, 0}, and_rows[2] = {0, 0}, pre_rows[2] = {0, 0}, rows[2] = {0, 0};
+      std::string err[2];
+      auto run = [&](int p) {
+        try {
+          pcg_hip::OleParty party(p, N, 2, 16);
+          auto ex = [&](const char* phase, int, int, const void* send, size_t sb, void* recv,
+                        size_t rb) {
+            if (sb) ch.send(1 - p, send, sb);
+            if (rb) ch.recv(p, recv, rb);
+            ++rows[p];
+            dpf_rows[p] += strcmp(phase, "dpf_gen") == 0;
+            and_rows[p] += strcmp(phase, "circuit_and") == 0;
+            pre_rows[p] += strcmp(phase, "base_ot") == 0 || strcmp(phase, "ot_ext") == 0 ||
+                           strcmp(phase, "gilboa") == 0;
+          };
+          po[p] = own ? party.expand_own(3, ex) : party.expand(3, ex);
+        } catch (const std::exception& e) {
+          err[p] = e.what();
+          ch.abort();
+        }
+      };
+      std::thread t0(run, 0), t1(run, 1);
+      t0.join();
+      t1.join();
+      CHECK(err[0].empty() && err[1].empty(), "both parties finish");
+      const bool pok = po[0].x == o.x0 && po[0].z == o.z0 && po[1].x == o.x1 && po[1].z == o.z1;
+      CHECK(pok, "OleParty outputs == OleExpander outputs");
+      const int levels = 9, blocks = 4;   // log2(2N/t) DPF levels at w = 1; c*c blocks
+      // adder over log2(N/t) = 8 bits: generate layer + 3 prefix steps; one Delta layer
+      const int and_layers = own ? 5 : 0;
+      CHECK(dpf_rows[0] == levels * blocks && dpf_rows[1] == levels * blocks,
+            "one dpf_gen exchange per DPF level and block");
+      CHECK(and_rows[0] == and_layers * blocks && and_rows[1] == and_rows[0],
+            "one circuit_and exchange per AND depth and block");
+      CHECK(pre_rows[0] == (own ? 4 : 0) && pre_rows[1] == pre_rows[0],
+            "two base_ot, one ot_ext, one gilboa exchange");
+      CHECK(rows[0] == (levels + 2) * blocks + and_rows[0] + pre_rows[0] && rows[1] == rows[0],
+            "two Beaver opens per block");
+      printf("two parties (%s) N=%d: %s (%d exchanges each)%s%s\n",
+             own ? "own seeds, 2PC preprocessing" : "dealer", N, pok ? "same outputs" : "DIFFER",
+             rows[0], err[0].c_str(), err[1].c_str());
+    }
   }
   printf("%s\n", fails ? "RESULT: FAIL" : "RESULT: PASS");
   return fails ? 1 : 0;

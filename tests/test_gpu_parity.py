@@ -396,42 +396,72 @@ def test_ole_relation_full_size(ctx, pq):
     assert int((x0 != 0).sum()) > N // 2
 
 
-@pytest.mark.parametrize("N,c,t,w,prg,budget,gmw", [
-    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, False),   # bins of 512: the fused output layer
-    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0, False),   # bins of 256: the unfused one
-    (1 << 14, 2, 8, 2, O.PRG_AES, 0, False),
-    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1, False),   # one DPF block per group
-    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, True)])   # DltSft from the GMW circuits
-def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, gmw):
-    """The two parties of the 2PC expansion on two contexts (two threads, as
-    two processes would be), every exchange through a queue pair standing in
-    for NetIO: each party's (x, z) equals the both-party plan's, and the
-    exchanges are the reference's comm_trace.h rows (per block: one
-    "dpf_gen" per DPF level, then "beaver_de" and "beaver_cw")."""
+def _adder_steps(nb):
+    s, d = 0, 1
+    while d < nb:
+        s, d = s + 1, d * 2
+    return s
+
+
+def _expected_rows(N, c, t, w, own, batched, groups, first_run=True):
+    """(phase, level, batch, send bytes) of one party's exchanges (both
+    parties' are the same except the sizes of "ot_ext" and "gilboa", which
+    are left out here)."""
+    B, nk, nblk = t * t, (1 << w) - 1, c * c
+    levels = ((2 * N // t).bit_length() - 1) // w
+    le = levels * nk
+    per = [1] * groups if batched else [1] * nblk      # blocks per exchange
+    unit = nblk // len(per)                             # blocks in one exchange
+    rows = []
+    if own:
+        nb = (N // t).bit_length() - 1
+        S = _adder_steps(nb)
+        # (level, AND gates per instance, words per instance) of every AND depth
+        layers = [(1, nb, 2)] + [(k + 1, 2 * (nb - (1 << (k - 1))), 4) for k in range(1, S + 1)]
+        tree = [(1, le * (w // 2), le)] if w >= 2 else []
+        if w >= 3:
+            tree.append((2, le, le))
+        dl = tree + [(len(tree) + 1, le * 128, 3 * le)]
+        for lv, gates, wpi in layers + dl:
+            rows += [("circuit_and", lv, B * gates * unit, B * wpi * 8 * unit)] * len(per)
+        if first_run:
+            rows += [("base_ot", 0, 1, 33), ("base_ot", 1, 128, 128 * 33)]
+    for blk in range(len(per)):
+        rows += [("dpf_gen", lv, B * unit, B * nk * 16 * unit) for lv in range(levels)]
+    rows += [("beaver_de", b * unit, B * unit, 16 * B * unit) for b in range(len(per))]
+    rows += [("beaver_cw", b * unit, B * unit, 8 * B * unit) for b in range(len(per))]
+    return rows
+
+
+def _queue_exchange(inbox, p):
+    """An exchange between two threads through a queue pair (the stand-in
+    for emp::NetIO): post what this party sends, wait for what it receives."""
+    def exchange(phase, level, batch, send, nrecv):
+        if send.size:
+            inbox[1 - p].put((phase, level, send))
+        if not nrecv:
+            return None
+        ph, lv, peer = inbox[p].get(timeout=60)
+        if ph == "abort":
+            raise ConnectionError("peer aborted")
+        assert (ph, lv) == (phase, level)
+        return peer
+    return exchange
+
+
+def _run_two_parties(pq, N, c, t, w, prg, pub, budget, body):
+    """body(p, op, exchange) in two threads, each with its own context and
+    one-party plan; returns body's results."""
     import queue
     import threading
-
-    seed, pub = 13, 4
-    with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=pub) as pl:
-        pl.deal(seed, gmw)
-        pl.run()
-        expect_out = pl.outputs()
     inbox = (queue.Queue(), queue.Queue())
     res, errs = [None, None], []
-    traces = [pq.CommTrace(0, "test"), pq.CommTrace(1, "test")]
 
     def party(p):
         try:
-            def exchange(phase, level, batch, mine):
-                inbox[1 - p].put((phase, level, mine))
-                ph, lv, peer = inbox[p].get(timeout=60)
-                assert (ph, lv) == (phase, level)
-                return peer
             with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, p, w, prg, pub_seed=pub,
                                                     hbm_budget=budget) as op:
-                op.deal(seed, gmw)
-                op.run(traces[p].wrap(exchange))
-                res[p] = op.outputs()
+                res[p] = body(p, op, _queue_exchange(inbox, p))
         except Exception as e:  # reported below
             errs.append(e)
             inbox[1 - p].put(("abort", -1, None))
@@ -440,23 +470,73 @@ def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, gmw):
     for x in th:
         x.start()
     for x in th:
-        x.join(timeout=120)
+        x.join(timeout=300)
     assert not errs, errs
+    return res
+
+
+@pytest.mark.parametrize("N,c,t,w,prg,budget,mode", [
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, "dealer"),   # bins of 512: the fused output layer
+    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0, "dealer"),   # bins of 256: the unfused one
+    (1 << 14, 2, 8, 2, O.PRG_AES, 0, "dealer"),
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1, "dealer"),   # one DPF block per group
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, "own"),      # each party's own seed + 2PC preprocessing
+    (1 << 12, 2, 16, 1, O.PRG_CHACHA8, 0, "own"),
+    (1 << 14, 2, 8, 2, O.PRG_AES, 0, "own"),           # DltSft AND tree, one depth
+    (1 << 12, 2, 16, 3, O.PRG_CHACHA8, 0, "own"),      # two depths
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 0, "own_batched"),
+    (1 << 13, 2, 16, 1, O.PRG_CHACHA8, 1, "dealer_batched")])
+def test_ole_two_parties_match_one_plan(ctx, pq, N, c, t, w, prg, budget, mode):
+    """The two parties of the 2PC expansion on two contexts (two threads, as
+    two processes would be), every exchange through a queue pair standing in
+    for NetIO: each party's (x, z) equals the both-party plan's, and the
+    exchanges are the reference's comm_trace.h rows: per block one "dpf_gen"
+    per DPF level, then "beaver_de" and "beaver_cw"; with each party's own
+    seed first the base OTs, the OT extension, the Gilboa corrections and one
+    "circuit_and" per AND depth and block; in the batched arm one exchange
+    per level, open and depth for the whole group."""
+    seed, pub = 13, 4
+    own, batched = mode.startswith("own"), mode.endswith("batched")
+    with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=pub) as pl:
+        pl.deal(seed)
+        pl.run()
+        expect_out = pl.outputs()
+    traces = [pq.CommTrace(0, "test"), pq.CommTrace(1, "test")]
+
+    def body(p, op, exchange):
+        if batched:
+            op.set_exchange_mode(pq.EXCHANGE_BATCHED)
+        if own:
+            op.deal_own(seed)
+        else:
+            op.deal(seed)
+        op.run(traces[p].wrap(exchange))
+        return op.outputs()
+
+    res = _run_two_parties(pq, N, c, t, w, prg, pub, budget, body)
     for p in (0, 1):
         assert np.array_equal(res[p][0], expect_out[2 * p]), f"x of party {p}"
         assert np.array_equal(res[p][1], expect_out[2 * p + 1]), f"z of party {p}"
-    B, nk = t * t, (1 << w) - 1
-    levels = ((2 * N // t).bit_length() - 1) // w
-    rows = ([("dpf_gen", lv, B, B * nk * 16) for b in range(c * c) for lv in range(levels)] +
-            [("beaver_de", b, B, 16 * B) for b in range(c * c)] +
-            [("beaver_cw", b, B, 8 * B) for b in range(c * c)])
+    groups = c * c if budget == 1 else 1
+    rows = _expected_rows(N, c, t, w, own, batched, groups)
     for p in (0, 1):
-        got = [(r[2], r[3], r[4], r[5]) for r in traces[p].rows]
+        got = [(r[2], r[3], r[4], r[5]) for r in traces[p].rows if r[2] not in ("ot_ext", "gilboa")]
         assert sorted(got) == sorted(rows)
-        assert all(r[5] == r[6] for r in traces[p].rows)
+        sym = [r for r in traces[p].rows if r[2] not in ("ot_ext", "gilboa")]
+        assert all(r[5] == r[6] for r in sym)
+        one = [r for r in traces[p].rows if r[2] in ("ot_ext", "gilboa")]
+        assert [r[2] for r in one] == (["ot_ext", "gilboa"] if own else [])
+    if own:   # the one-way and uneven messages: what one party sends the other receives
+        for ph in ("ot_ext", "gilboa"):
+            r0 = [r for r in traces[0].rows if r[2] == ph][0]
+            r1 = [r for r in traces[1].rows if r[2] == ph][0]
+            assert r0[5] == r1[6] and r1[5] == r0[6]
+        g0 = [r for r in traces[0].rows if r[2] == "gilboa"][0]
+        assert g0[6] == 0 and g0[5] == c * c * t * t * 3 * 64 * 8
 
 
-def test_ole_party_plans_reusable_after_a_failed_exchange(ctx, pq):
+@pytest.mark.parametrize("mode", ["dealer", "own"])
+def test_ole_party_plans_reusable_after_a_failed_exchange(ctx, pq, mode):
     """An exchange that fails mid-protocol (a dropped connection) makes both
     parties' runs raise; the same plans then deal and run again, and each
     party's (x, z) equals the both-party plan's."""
@@ -477,32 +557,43 @@ def test_ole_party_plans_reusable_after_a_failed_exchange(ctx, pq):
             with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, p, pub_seed=pub) as op:
                 calls = [0]
 
-                def flaky(phase, level, batch, mine):
+                def flaky(phase, level, batch, send, nrecv):
                     calls[0] += 1
                     if p == 0 and calls[0] == 3:
                         raise ConnectionError("link dropped")
-                    inbox[1 - p].put(mine)
+                    if send.size:
+                        inbox[1 - p].put(send)
+                    if not nrecv:
+                        return None
                     peer = inbox[p].get(timeout=60)
                     if peer is None:
                         raise ConnectionError("peer aborted")
                     return peer
 
-                def good(phase, level, batch, mine):
-                    inbox[1 - p].put(mine)
-                    return inbox[p].get(timeout=60)
+                def deal():
+                    if mode == "own":
+                        op.deal_own(seed)
+                    else:
+                        op.deal(seed)
 
                 go[0].wait()
-                op.deal(seed)
+                deal()
                 try:
                     op.run(flaky)
-                except Exception:
+                except ConnectionError:
                     failed[p] = True
                     inbox[1 - p].put(None)   # tell the peer
                 go[1].wait()
                 while not inbox[p].empty():  # drop what the failed round left
                     inbox[p].get_nowait()
                 go[2].wait()
-                op.deal(seed)
+                deal()
+
+                def good(phase, level, batch, send, nrecv):
+                    if send.size:
+                        inbox[1 - p].put(send)
+                    return inbox[p].get(timeout=60) if nrecv else None
+
                 op.run(good)
                 res[p] = op.outputs()
         except Exception as e:  # reported below
@@ -534,21 +625,30 @@ N, c, t, seed, pub = (int(v) for v in os.environ["PCGSQ_CASE"].split(","))
 print("party", rank, "N", N, "c", c, "t", t, flush=True)
 dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MASTER_PORT"],
                         rank=rank, world_size=2)
+trace = pq.CommTrace(rank, "gloo")
 with pq.Context(0) as cx, pq.OleParty(cx, N, c, t, rank, pub_seed=pub) as op:
-    op.deal(seed)
-    op.run(pq.dist_exchange(1 - rank))
+    if os.environ["PCGSQ_MODE"] == "own":
+        op.deal_own(seed)
+    else:
+        op.deal(seed)
+    op.run(trace.wrap(pq.dist_exchange(1 - rank)))
     x, z = op.outputs()
 np.save(os.path.join(os.environ["PCGSQ_OUT"], "x%d.npy" % rank), x)
 np.save(os.path.join(os.environ["PCGSQ_OUT"], "z%d.npy" % rank), z)
+trace.to_csv(os.path.join(os.environ["PCGSQ_OUT"], "trace%d.csv" % rank))
 dist.destroy_process_group()
 """
 
 
-def test_ole_parties_as_two_processes_over_gloo(ctx, pq, tmp_path):
+@pytest.mark.parametrize("mode", ["dealer", "own"])
+def test_ole_parties_as_two_processes_over_gloo(ctx, pq, tmp_path, mode):
     """The reference's deployment: one party per OS process, every exchange
     over a network transport (torch.distributed gloo on 127.0.0.1 in place of
-    emp::NetIO), both processes on this GPU.  Each party's (x, z) equals the
-    both-party plan's."""
+    emp::NetIO), both processes on this GPU; with "own" each process draws
+    only its own inputs and the two run the 2PC preprocessing (base OTs, OT
+    extension, GMW, Gilboa) over gloo first.  Each party's (x, z) equals the
+    both-party plan's, and the processes' comm_trace CSVs hold the rows."""
+    import csv
     import socket
     import subprocess
     import sys
@@ -566,7 +666,7 @@ def test_ole_parties_as_two_processes_over_gloo(ctx, pq, tmp_path):
     for rank in (0, 1):
         env = dict(os.environ, RANK=str(rank), MASTER_PORT=str(port),
                    PCGSQ_PKG=os.path.join(root, "pcg-square_amd"), PCGSQ_OUT=str(tmp_path),
-                   PCGSQ_CASE=f"{N},{c},{t},{seed},{pub}")
+                   PCGSQ_CASE=f"{N},{c},{t},{seed},{pub}", PCGSQ_MODE=mode)
         procs.append(subprocess.Popen([sys.executable, "-c", _PARTY_PROCESS], env=env,
                                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
     logs = []
@@ -582,6 +682,53 @@ def test_ole_parties_as_two_processes_over_gloo(ctx, pq, tmp_path):
     for rank in (0, 1):
         assert np.array_equal(np.load(tmp_path / f"x{rank}.npy"), expect[2 * rank]), f"x{rank}"
         assert np.array_equal(np.load(tmp_path / f"z{rank}.npy"), expect[2 * rank + 1]), f"z{rank}"
+        with open(tmp_path / f"trace{rank}.csv") as f:
+            rows = list(csv.DictReader(f))
+        phases = [r["phase"] for r in rows]
+        want = _expected_rows(N, c, t, 1, mode == "own", False, 1)
+        assert len(rows) == len(want) + (2 if mode == "own" else 0)
+        assert phases.count("circuit_and") == sum(1 for r in want if r[0] == "circuit_and")
+        assert phases.count("dpf_gen") == sum(1 for r in want if r[0] == "dpf_gen")
+
+
+def test_two_party_preprocessing_full_size(ctx, pq):
+    """N = 2^20, c = 4, t = 16: two one-party plans in two threads, each from
+    its own seed with the 2PC preprocessing run between them, return outputs
+    whose FNV-1a-64 hashes are the CPU oracle's for that seed
+    (tests/golden/served_hashes.json); the exchanges have the reference's
+    round structure: 96 "circuit_and" rows (16 blocks x 6 AND depths: the
+    16-bit adder's 5 and the DltSft's 1; comm_calib_l40s.txt:15), 272
+    "dpf_gen", 16 + 16 Beaver opens, plus the base OTs, the OT extension and
+    the Gilboa corrections.  A second run in the batched arm (seed 7007)
+    needs 6 + 17 + 2 + 2 exchanges and matches its hashes too."""
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "served_hashes.json")
+    want = {r["seed"]: r["fnv1a64"] for r in json.load(open(path))["requests"]
+            if r["N"] == 1 << 20}
+    traces = [[pq.CommTrace(p, "run1"), pq.CommTrace(p, "run2")] for p in (0, 1)]
+
+    def body(p, op, exchange):
+        out = []
+        for k, seed in enumerate((7006, 7007)):
+            if k == 1:
+                op.set_exchange_mode(pq.EXCHANGE_BATCHED)
+            op.deal_own(seed)
+            op.run(traces[p][k].wrap(exchange))
+            out.append(["%016x" % O.fnv1a64(v) for v in op.outputs()])
+        return out
+
+    res = _run_two_parties(pq, 1 << 20, 4, 16, 1, pq.PRG_CHACHA8, 0, 0, body)
+    for k, seed in enumerate((7006, 7007)):
+        assert [res[0][k][0], res[0][k][1], res[1][k][0], res[1][k][1]] == want[seed], seed
+    for p in (0, 1):
+        ph = [r[2] for r in traces[p][0].rows]
+        assert ph.count("circuit_and") == 96 and ph.count("dpf_gen") == 272
+        assert ph.count("beaver_de") == 16 and ph.count("beaver_cw") == 16
+        assert ph.count("base_ot") == 2 and ph.count("ot_ext") == 1 and ph.count("gilboa") == 1
+        ph = [r[2] for r in traces[p][1].rows]
+        assert ph.count("circuit_and") == 6 and ph.count("dpf_gen") == 17
+        assert ph.count("beaver_de") == 1 and ph.count("beaver_cw") == 1
+        assert ph.count("base_ot") == 0 and len(ph) == 27
 
 
 def test_ole_outputs_into_pinned_arrays(ctx, pq):
@@ -648,31 +795,68 @@ def test_triples_full_size_are_triples(ctx, pq):
     assert all(p % P == int(v) for p, v in zip(prod, cc))
 
 
-# ---- GMW preprocessing on the device (pcg_ole_impl.h:114-199) -----------------
+# ---- 2PC preprocessing on the device (pcg_ole_impl.h:62-199) -------------------
 
 @pytest.mark.parametrize("N,c,t,w", [(1 << 11, 2, 16, 1), (1 << 11, 2, 16, 2), (1 << 11, 2, 16, 4),
                                      (1 << 12, 2, 16, 3), (1 << 13, 3, 8, 1)])
-def test_gmw_preprocessing_matches_oracle(ctx, pq, N, c, t, w):
-    """Both parties' DltSft shares from the device's bit-sliced GMW
-    evaluation (AND triples made from OTs) equal the oracle's gate-by-gate
-    evaluation, bit for bit, and reconstruct to the dealer's DltSft; beta and
-    the Beaver triples' shares from the device's Gilboa products equal the
-    oracle's."""
-    og = O.dealt_inputs(N, c, t, w, seed=41, gmw=True)
+def test_gmw_preprocessing_reconstructs(ctx, pq, N, c, t, w):
+    """The two parties' preprocessing (in one plan, every message a device
+    copy): the DltSft shares XOR to the oracle's [digit_i(alpha) = k] Delta,
+    beta's shares add to pay0 * pay1 and rs's to (r0 + r1)(s0 + s1) (the
+    oracle dealer's reconstructions, same seed); r and s are the dealer's;
+    the shares themselves are fresh (not the dealer's)."""
     od = O.dealt_inputs(N, c, t, w, seed=41)
     with pq.OlePlan(ctx, N, c, t, w) as pl:
         pl.deal(41, gmw=True)
         got = pl.dealt_dltsft()
         prod = pl.dealt_products()
-        pl.deal(41)
-        got_dealer = pl.dealt_dltsft()
-        prod_dealer = pl.dealt_products()
-    assert np.array_equal(got, og["dltsft"])
-    assert np.array_equal(got_dealer, od["dltsft"])
-    assert np.array_equal(got[0] ^ got[1], got_dealer[0] ^ got_dealer[1])
-    for k in ("beta", "r", "s", "rs"):
-        assert np.array_equal(prod[k], og[k]), k
-        assert np.array_equal(prod_dealer[k], od[k]), k
+    assert np.array_equal(got[0] ^ got[1], od["dltsft"][0] ^ od["dltsft"][1])
+    assert not np.array_equal(got[0], od["dltsft"][0])
+    for k in ("beta", "rs"):
+        s_got = [(int(a) + int(b)) % P for a, b in zip(prod[k][0], prod[k][1])]
+        s_exp = [(int(a) + int(b)) % P for a, b in zip(od[k][0], od[k][1])]
+        assert s_got == s_exp, k
+        assert not np.array_equal(prod[k][0], od[k][0]), k
+    for k in ("r", "s"):
+        assert np.array_equal(prod[k], od[k]), k
+
+
+@pytest.mark.parametrize("N,c,t,w", [(1 << 12, 2, 16, 1), (1 << 12, 2, 16, 3), (1 << 11, 2, 16, 4)])
+def test_preprocessing_random_ots_and_triples(ctx, pq, N, c, t, w):
+    """The OT extension and the triples built on it, read back from both
+    parties: in each direction the receiver's message of every OT is the
+    sender's message its choice bit selects (and the sender's two differ);
+    every AND triple satisfies (a0 ^ a1) & (b0 ^ b1) = c0 ^ c1 (adder words on
+    their nb bits, the shared-a Delta triples on 128 bits, the AND-tree bits)."""
+    nb = (N // t).bit_length() - 1
+    mask = np.uint64((1 << nb) - 1)
+    with pq.OlePlan(ctx, N, c, t, w) as pl:
+        pl.deal(5, gmw=True)
+        pre = [{k: pl.preprocessing(p, k) for k in pq.OlePlan.PRE_ITEMS} for p in (0, 1)]
+    for d in (0, 1):   # direction d: party d sends, party 1 - d receives
+        snd, rcv = pre[d], pre[1 - d]
+        m0 = snd["sent0"].reshape(-1, 2)
+        m1 = snd["sent1"].reshape(-1, 2)
+        got = rcv["received"].reshape(-1, 2)
+        ch = np.unpackbits(rcv["choice"].view(np.uint8), bitorder="little").astype(bool)
+        assert len(m0) == len(got) == len(ch) and len(m0) % 512 == 0
+        assert np.array_equal(got, np.where(ch[:, None], m1, m0)), d
+        assert not np.any(np.all(m0 == m1, axis=1))
+    a0, b0, c0 = pre[0]["adder_triples"].reshape(-1, 3).T
+    a1, b1, c1 = pre[1]["adder_triples"].reshape(-1, 3).T
+    assert np.array_equal(((a0 ^ a1) & (b0 ^ b1)) & mask, (c0 ^ c1) & mask)
+    assert np.any(a0 & mask) and np.any(b1 & mask)
+    t0 = pre[0]["delta_triples"].reshape(-1, 5)
+    t1 = pre[1]["delta_triples"].reshape(-1, 5)
+    a = t0[:, 0] ^ t1[:, 0]
+    assert set(np.unique(t0[:, 0]).tolist()) <= {0, 2**64 - 1}
+    for h in (0, 1):
+        assert np.array_equal(a & (t0[:, 1 + h] ^ t1[:, 1 + h]), t0[:, 3 + h] ^ t1[:, 3 + h])
+    if w >= 2:
+        x = pre[0]["tree_triples"] ^ pre[1]["tree_triples"]
+        for j in range(w - 1):
+            aa, bb, cc = (x >> j) & 1, (x >> (4 + j)) & 1, (x >> (8 + j)) & 1
+            assert np.array_equal(aa & bb, cc), j
 
 
 @pytest.mark.parametrize("N,c,t,w,prg", [(1 << 12, 2, 16, 1, 1), (1 << 12, 2, 16, 3, 1),

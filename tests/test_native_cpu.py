@@ -85,11 +85,27 @@ rank = int(os.environ["RANK"])
 dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MASTER_PORT"],
                         rank=rank, world_size=2)
 ex = pq.dist_exchange(1 - rank)
-for k, n in enumerate((16, 4096, 1 << 20)):
-    mine = np.full(n, 17 * rank + k, dtype=np.uint8)
-    got = ex("dpf_gen", k, 1, mine)
-    assert got.dtype == np.uint8 and got.shape == (n,), (got.dtype, got.shape)
+# equal sizes, then uneven ones (the IKNP matrices), then one-way messages
+sizes = [(16, 16), (4096, 4096), (1 << 20, 1 << 20), (100, 3000), (0, 64), (64, 0)]
+for k, (n0, n1) in enumerate(sizes):
+    ns, nr = (n0, n1) if rank == 0 else (n1, n0)
+    mine = np.full(ns, 17 * rank + k, dtype=np.uint8)
+    got = ex("dpf_gen", k, 1, mine, nr)
+    assert got.dtype == np.uint8 and got.shape == (nr,), (got.dtype, got.shape)
     assert np.all(got == 17 * (1 - rank) + k), (rank, k)
+# base OTs between the two processes: the chosen seeds are the other party's
+b = pq.base_ots_run(rank, 1000 + rank, ex)
+def blk(x):
+    return int(x.lo) | (int(x.hi) << 64)
+mine = [blk(v) for v in b.k0] + [blk(v) for v in b.k1] + [blk(b.s)] + [blk(v) for v in b.ks]
+buf = np.frombuffer(b"".join(v.to_bytes(16, "little") for v in mine), dtype=np.uint8)
+got = ex("check", 0, 1, buf, buf.size)
+peer = [int.from_bytes(got[16 * i:16 * i + 16].tobytes(), "little") for i in range(len(mine))]
+pk0, pk1, ps, pks = peer[:128], peer[128:256], peer[256], peer[257:]
+mk0, mk1, ms, mks = mine[:128], mine[128:256], mine[256], mine[257:]
+for i in range(128):
+    assert mks[i] == (pk1[i] if (ms >> i) & 1 else pk0[i]), ("mine", i)
+    assert pks[i] == (mk1[i] if (ps >> i) & 1 else mk0[i]), ("peer's", i)
 print("ok", rank, flush=True)
 dist.destroy_process_group()
 """
@@ -98,7 +114,10 @@ dist.destroy_process_group()
 def test_dist_exchange_swaps_between_two_processes():
     """pcgsq.dist_exchange, the OleParty exchange over torch.distributed
     (gloo, CPU tensors): two processes on 127.0.0.1 each receive exactly the
-    other's bytes, in order, for messages of 16 B to 1 MiB."""
+    other's bytes, in order, for messages of 16 B to 1 MiB, of uneven sizes
+    and one-way; and the base OTs run over it between the processes give
+    each party, in each direction, the column seed its choice bit selects
+    from the other party's pair."""
     import socket
     import subprocess
     import sys
@@ -198,3 +217,58 @@ def test_oracle_aes128_matches_libcrypto():
                 x = parents[j].tobytes()
                 want = bytes(a ^ b for a, b in zip(ssl_aes(sub, x), x))
                 assert out[j * nk + k].tobytes() == want, (nk, k, j)
+
+
+def test_base_ots_two_threads_choose_the_right_seeds():
+    """Chou-Orlandi base OTs of the two parties (host code, libcrypto P-256)
+    in two threads over a queue pair: for every column i of both directions
+    the receiver's seed is the sender's k_{s_i}, the other seed differs from
+    it, seeds are distinct across columns, the choice bits are not constant,
+    and the same seeds give the same results (seeded draws)."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    import pcgsq
+
+    def run(seeds):
+        inbox = (queue.Queue(), queue.Queue())
+        out, errs = [None, None], []
+
+        def party(p):
+            try:
+                def ex(phase, level, batch, send, nrecv):
+                    inbox[1 - p].put(bytes(send))
+                    got = inbox[p].get(timeout=30)
+                    assert len(got) == nrecv
+                    return np.frombuffer(got, dtype=np.uint8)
+                out[p] = pcgsq.base_ots_run(p, seeds[p], ex)
+            except Exception as e:
+                errs.append(e)
+        th = [threading.Thread(target=party, args=(p,)) for p in (0, 1)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join(60)
+        assert not errs, errs
+        return out
+
+    def blk(x):
+        return int(x.lo) | (int(x.hi) << 64)
+
+    a = run((11, 22))
+    for snd, rcv in ((a[1], a[0]), (a[0], a[1])):   # direction 0: party 0 sends
+        s = blk(rcv.s)
+        assert 0 < s < (1 << 128) - 1
+        k0 = [blk(v) for v in snd.k0]
+        k1 = [blk(v) for v in snd.k1]
+        ks = [blk(v) for v in rcv.ks]
+        assert len(set(k0 + k1)) == 256
+        for i in range(128):
+            want, other = (k1[i], k0[i]) if (s >> i) & 1 else (k0[i], k1[i])
+            assert ks[i] == want and ks[i] != other, i
+    b = run((11, 22))
+    assert [blk(v) for v in b[0].ks] == [blk(v) for v in a[0].ks]
+    c = run((12, 22))
+    assert [blk(v) for v in c[0].ks] != [blk(v) for v in a[0].ks]
