@@ -283,12 +283,23 @@ def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
     return out
 
 
-def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
-    """The 2PC expansion as two party processes run it (pcgsq_ole_party_*):
-    two threads, each with its own context and plan for one party, every
-    exchange through an in-memory queue pair; then the same as two OS
-    processes over gloo (bench_two_processes).  Both parties share this one
-    GPU (with a party per GPU each would have a GPU of its own)."""
+def _spin_us(us: float) -> None:
+    """Busy-wait `us` microseconds (an injected network round trip)."""
+    if us <= 0:
+        return
+    end = time.perf_counter_ns() + int(us * 1e3)
+    while time.perf_counter_ns() < end:
+        pass
+
+
+def _two_party_threads(pcgsq, N, c, t, mode: str, rtt_us: float, reps: int) -> dict:
+    """The expansion as two party threads, each with its own context and
+    one-party plan, exchanging through an in-memory queue pair; party 0
+    waits rtt_us more per exchange (a network round trip: the reference's
+    PCG_RTT_US, GPU_baseline/dpf_real_bench.cu:64-81).  mode: "dealer" (the
+    trusted-dealer simulation: the run starts at the DPF), "own" (each party
+    from its own seed, the 2PC preprocessing first; per-block exchanges, the
+    reference's serial arm), "own_batched" (the same in its batched arm)."""
     import queue
     import threading
     inbox = (queue.Queue(), queue.Queue())
@@ -303,11 +314,21 @@ def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
                 stats[p]["bytes"] += int(send.nbytes)
                 if send.size:
                     inbox[1 - p].put(send)
-                return inbox[p].get(timeout=120) if nrecv else None
+                got = inbox[p].get(timeout=120) if nrecv else None
+                if p == 0:
+                    _spin_us(rtt_us)
+                return got
             with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, p) as op:
-                op.deal(1)
-                for k in range(reps + 1):   # the first round warms up
+                if mode.endswith("batched"):
+                    op.set_exchange_mode(pcgsq.EXCHANGE_BATCHED)
+                for k in range(reps + 1):   # the first round warms up (and runs the base OTs)
+                    if mode.startswith("own"):
+                        op.deal_own(1)
+                    else:
+                        op.deal(1)
                     go[k].wait()
+                    if k == 1:
+                        stats[p]["exchanges"] = stats[p]["bytes"] = 0
                     t0 = time.perf_counter()
                     op.run(exchange)
                     if p == 0 and k > 0:
@@ -326,13 +347,30 @@ def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
     if errs or not times:
         return {"error": errs[:1]}
     ms = 1e3 * sum(times) / len(times)
-    return {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
-                        "as two threads with a context and one-party plan each, exchanges "
-                        "through a queue pair; both on this one GPU",
-            "ms": ms, "OLEs_per_s": N / (ms * 1e-3),
-            "exchanges_per_party_per_run": stats[0]["exchanges"] // (reps + 1),
-            "bytes_sent_per_party_per_run": stats[0]["bytes"] // (reps + 1),
-            "as_os_processes_over_gloo": bench_two_processes(N, c, t, reps)}
+    return {"ms": ms, "OLEs_per_s": N / (ms * 1e-3),
+            "exchanges_per_party_per_run": stats[0]["exchanges"] // reps,
+            "bytes_sent_per_party_per_run": [stats[p]["bytes"] // reps for p in (0, 1)]}
+
+
+def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
+    """The 2PC expansion as two party processes run it (pcgsq_ole_party_*):
+    two threads with a context and one-party plan each, both on this one GPU
+    (with a party per GPU each would have a GPU of its own): the
+    trusted-dealer simulation, then each party from its own seed with the 2PC
+    preprocessing (base OTs, IKNP OT extension, GMW adder and DltSft, Gilboa)
+    in the serial and the batched arm, at injected round trips of 0, 5 (the
+    reference's tier), 50 and 500 us; and the own-seed serial arm as two OS
+    processes over gloo (bench_two_processes)."""
+    res = {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
+                       "as two threads with a context and one-party plan each, exchanges "
+                       "through a queue pair; both on this one GPU"}
+    res["dealer_serial"] = _two_party_threads(pcgsq, N, c, t, "dealer", 0, reps)
+    for rtt in (0, 5, 50, 500):
+        for mode, key in (("own", "own_serial"), ("own_batched", "own_batched")):
+            r = _two_party_threads(pcgsq, N, c, t, mode, rtt, 2 if rtt >= 50 else reps)
+            res.setdefault(key, {})[f"rtt_{rtt}us"] = r
+    res["as_os_processes_over_gloo"] = bench_two_processes(N, c, t, reps)
+    return res
 
 
 _PARTY_PROCESS = """
@@ -347,8 +385,8 @@ dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MAS
 ex = pcgsq.dist_exchange(1 - rank)
 times = []
 with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, rank) as op:
-    op.deal(1)
-    for k in range(reps + 1):   # the first round warms up
+    for k in range(reps + 1):   # the first round warms up (and runs the base OTs)
+        op.deal_own(1)
         dist.barrier()
         t0 = time.perf_counter()
         op.run(ex)
@@ -394,7 +432,8 @@ def bench_two_processes(N: int, c: int, t: int, reps: int = 3) -> dict:
     times = json.loads(res[0][len("RESULT "):])
     ms = 1e3 * sum(times) / len(times)
     return {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
-                        "as two OS processes, exchanges over torch.distributed gloo (TCP, "
+                        "as two OS processes, each from its own seed with the 2PC "
+                        "preprocessing, serial arm, exchanges over torch.distributed gloo (TCP, "
                         "127.0.0.1); both on this one GPU",
             "ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
 
@@ -629,9 +668,11 @@ def main() -> int:
         "pcie_inclusive_OLEs_per_s": N / dt_pcie,
         "pcie_inclusive_pinned_OLEs_per_s": N / dt_pinned,
         "preprocessing_gmw": {
-            "what": "position adder (Kogge-Stone) + DltSft circuit under GMW for both parties, "
-                    "AND triples made from ideal OTs, beta and Beaver rs by Gilboa OT products; "
-                    "replaces the dealer's DltSft, beta and rs (same outputs)",
+            "what": "both parties' 2PC preprocessing in one plan (pcgsq_ole_deal_gmw): IKNP random "
+                    "OTs both ways (base OTs once per plan), position adder (Kogge-Stone) and "
+                    "DltSft circuit under GMW with AND triples from the OTs, beta and Beaver rs "
+                    "by Gilboa OT products, every message a device copy; replaces the dealer's "
+                    "DltSft, beta and rs (same outputs)",
             "instances": c * c * t * t, "and_gates": c * c * t * t * gmw_and_gates(N, t),
             "deal_ms": deal_ms[True], "dealer_deal_ms": deal_ms[False],
             "and_gates_per_s": c * c * t * t * gmw_and_gates(N, t) / (deal_ms[True] * 1e-3)},
