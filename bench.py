@@ -613,26 +613,39 @@ def main() -> int:
     served = bench_served(pcgsq) if "served" in parts else None
 
     leaves = 2 * c * c * t * t * (2 * N // t)        # both parties, all instances
-    # the leaf level: with the output layer's gather fused in at this shape
-    leaf_zone = "dpf.leaf_gather" if "dpf.leaf_gather" in zones else "dpf.leaf_convert"
+    trees = 2 * c * c * t * t
+    tree_blocks = trees * ((2 * N // t) - 1)         # internal nodes, one ChaCha8 block each
+    if "dpf.leaf_deep" in zones:
+        # the leaf level with the last three tree levels fused in: per tree
+        # 2^(n-3) + 2^(n-2) + 2^(n-1) tree blocks and 2^n out-hash blocks
+        leaf_zone = "dpf.leaf_deep"
+        leaf_blocks = leaves + trees * 7 * (2 * N // t) // 8
+        kernel = ("deep_leaf_kernel (last three tree levels + leaf level + ChaCha8 out-hash + "
+                  "output-layer gather, in registers)")
+        algorithmic = (f"{leaves} out-hash + {leaf_blocks - leaves} tree ChaCha8 blocks x "
+                       f"{CHACHA8_OPS} ops per launch")
+    else:
+        leaf_zone = "dpf.leaf_gather" if "dpf.leaf_gather" in zones else "dpf.leaf_convert"
+        leaf_blocks = leaves
+        kernel = ("leaf_gather_kernel<w=1> (leaf level + ChaCha8 out-hash + output-layer "
+                  "gather)" if leaf_zone == "dpf.leaf_gather" else
+                  "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + out-hash)")
+        algorithmic = f"{leaves} ChaCha8 out-hash blocks x {CHACHA8_OPS} ops per launch"
     n_leaf, ms_leaf = zones[leaf_zone]
     avg_leaf = ms_leaf / n_leaf
-    ach = leaves * CHACHA8_OPS / (avg_leaf * 1e-3)
+    ach = leaf_blocks * CHACHA8_OPS / (avg_leaf * 1e-3)
     peak_meas = ceiling * CHACHA8_OPS
     peak = max(VALU_NOMINAL, peak_meas)
     traffic_file = os.path.join(ROOT, "profiles", "pcg_ole_expand_N20_c4_t16_hbm_traffic.md")
     traffic = pmc_traffic(traffic_file)
-    roofline = {"bound": "valu",
-                "kernel": ("leaf_gather_kernel<w=1> (leaf level + ChaCha8 out-hash + output-layer "
-                           "gather)" if leaf_zone == "dpf.leaf_gather" else
-                           "dpf_level_kernel<w=1,ChaCha8,convert> (leaf level + out-hash)"),
+    roofline = {"bound": "valu", "kernel": kernel,
                 "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "Tops/s",
                 "frac": ach / peak, "traffic": traffic,
                 "peak_nominal": VALU_NOMINAL / 1e12, "peak_measured": peak_meas / 1e12,
-                "algorithmic": f"{leaves} ChaCha8 out-hash blocks x {CHACHA8_OPS} ops per launch",
-                "avg_launch_ms": avg_leaf}
-    tree_ms = sum(v[1] for k, v in zones.items() if k.startswith("dpf.level.")) / nprof
-    tree_blocks = 2 * c * c * t * t * ((2 * N // t) - 1)
+                "algorithmic": algorithmic, "avg_launch_ms": avg_leaf}
+    # every DPF launch of the expansion: all its ChaCha8 blocks over their time
+    dpf_ms = sum(v[1] for k, v in zones.items() if k.startswith("dpf.")) / nprof
+    dpf_rate = (tree_blocks + leaves) / (dpf_ms * 1e-3)
     # NTT: the forward batch of step 4 (2c^2 + 2c transforms of length N)
     nf = 2 * c * c + 2 * c
     f1 = zones.get("ntt.fwd_pass1", (1, 0.0))
@@ -679,7 +692,8 @@ def main() -> int:
         "roofline": roofline,
         "roofline_ntt": roof_ntt,
         "phases_ms": {k: v[1] / nprof for k, v in sorted(zones.items())},
-        "tree_prg_rate_Gblocks_per_s": tree_blocks / (tree_ms * 1e-3) / 1e9,
+        "dpf_chacha8_rate_Gblocks_per_s": dpf_rate / 1e9,
+        "dpf_ms_per_expansion": dpf_ms,
         "chacha8_ceiling_Gblocks_per_s": ceiling / 1e9,
         "hbm_copy_GBps": copy_bps / 1e9,
         "hbm_probe_GBps": {"copy": hbm["copy"] / 1e9, "read": hbm["read"] / 1e9,

@@ -432,7 +432,206 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   }
 }
 
-// Level 0: the frontier is the root.// Level 0: the frontier is the root.  Also moves DltSft from the reference
+// ---- both parties in one run: the last tree levels fused into the leaf level --
+//
+// With both parties' trees in one run, the correction words of the last
+// three tree levels can be computed from the alpha path alone: nodes off the
+// path are equal in the two parties, so their hashes cancel in the XOR of
+// the two parties' level shares, and CW_L = H(path node of party 0) ^
+// H(path node of party 1) ^ DltSft0_L ^ DltSft1_L, the value the fused
+// protocol reduces to (dpf.cpp:520-551).  alpha's digit at level L is where
+// DltSft0_L ^ DltSft1_L is nonzero ([digit_L(alpha) = k] Delta, Delta != 0).
+// The leaf level then expands those three levels and the leaves in registers
+// from the stored nodes of level levels - 4 (deep_leaf_kernel), and no wider
+// level is ever stored.  w = 1, ChaCha8 tree PRG.
+
+__device__ __forceinline__ Blk child1(Blk parent, Blk h, int c, Blk cw) {
+  Blk node = c ? bxor(h, parent) : h;   // w = 1: child 0 = h, child 1 = h ^ parent
+  if (parent.lo & 1ULL) node = bxor(node, cw);
+  return node;
+}
+
+// One thread per instance (trees Ti and Ti + half).  Levels L0 - 1 and below
+// are complete in cw; levels L0 .. levels - 1 still hold the seeded DltSft
+// shares and receive CW_L in the first tree's slot (0 in the second's, so
+// that mine ^ peer is CW_L for both).
+__global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict__ F,
+                                                           const Blk* __restrict__ H,
+                                                           Blk* __restrict__ cw,
+                                                           const Blk* __restrict__ dlt0,
+                                                           const Blk* __restrict__ dlt1,
+                                                           int half, int levels, int L0) {
+  const int Ti = blockIdx.x * kTpb + threadIdx.x;
+  if (Ti >= half) return;
+  const size_t ntrees = 2 * (size_t)half;
+  auto digit = [&](int L) {
+    const Blk a = dlt0[(size_t)Ti * levels + L], b = dlt1[(size_t)Ti * levels + L];
+    return ((a.lo ^ b.lo) | (a.hi ^ b.hi)) ? 0 : 1;
+  };
+  size_t j = 0;   // path node of level L0 - 1
+  for (int L = 0; L < L0 - 1; ++L) j = (j << 1) | (size_t)digit(L);
+  const size_t S = (size_t)1 << (L0 - 1);
+  const size_t i0 = (size_t)Ti * S + j, i1 = ((size_t)Ti + half) * S + j;
+  Blk n0 = F[i0], n1 = F[i1];
+  Blk h0 = H[i0], h1 = H[i1];
+  const Blk* c0 = &cw[(size_t)(L0 - 1) * ntrees + Ti];
+  Blk cwv = bxor(c0[0], c0[half]);
+  int c = digit(L0 - 1);
+  n0 = child1(n0, h0, c, cwv);
+  n1 = child1(n1, h1, c, cwv);
+  for (int L = L0; L < levels; ++L) {
+    tree_hash<kPrgChaCha8, 1>(n0, &h0, nullptr);
+    tree_hash<kPrgChaCha8, 1>(n1, &h1, nullptr);
+    Blk* m = &cw[(size_t)L * ntrees + Ti];
+    cwv = bxor(bxor(h0, h1), bxor(m[0], m[half]));
+    m[0] = cwv;
+    m[half] = Blk{0, 0};
+    if (L + 1 < levels) {
+      c = digit(L);
+      n0 = child1(n0, h0, c, cwv);
+      n1 = child1(n1, h1, c, cwv);
+    }
+  }
+}
+
+// The leaf level of both parties' run with the last three tree levels fused
+// in (w = 1, 8 output coefficients per thread).  Per landing slot a thread's
+// 8 consecutive leaves d0 .. d0 + 7 descend from one node of level
+// levels - 3; the thread reads that node's parent (level levels - 4, stored
+// with its hash), forms the node, and expands 1 + 2 + 4 tree hashes and 8
+// out-hashes in registers.  Every (tree, node) is expanded by exactly one
+// thread, so the work is the same as hashing the levels one by one, without
+// their frontiers.  The output layer's sums, tau bits and Beaver partials are
+// those of leaf_gather_kernel.
+__global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
+  constexpr int LPT = 8;
+  constexpr int TILE = kTpb * LPT;
+  __shared__ uint64_t red[kGatherMaxSlots][4][2];
+  __shared__ uint64_t red_at[kGatherMaxSlots];
+  const int bb = blockIdx.y, party = blockIdx.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t qt = blockIdx.x * TILE;
+  const uint32_t q0 = qt + LPT * threadIdx.x;
+  const size_t bin_mask = ((size_t)1 << a.bin_log) - 1;
+  const size_t rr0 = q0 & bin_mask;  // a multiple of LPT
+  const int levels = a.level;        // leaves are level `levels`
+  const int Ltop = levels - 4;       // stored level
+  const size_t Stop = (size_t)1 << Ltop;
+  const int nparts_log = levels - (kTpbLog + 3);
+  const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
+  const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
+  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
+                                     ((size_t)bb << a.N_log) + q0);
+  uint64_t A[LPT], mw[LPT];
+#pragma unroll
+  for (int e = 0; e < LPT; ++e) A[e] = mw[e] = 0;
+  int nadd = 0, phase = 0;
+  auto reduce_A = [&]() {
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
+    nadd = 0;
+  };
+  int widx = 0;
+  auto store_words = [&](int wi) {
+    Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
+#pragma unroll
+    for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{mw[e], mw[e + 1]};
+  };
+  const int nslots = for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h,
+                                                                             int ds, int inst) {
+    if ((slot >> 6) != widx) {
+      store_words(widx);
+      widx = slot >> 6;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) mw[e] = 0;
+    }
+    if (h != phase) {
+      reduce_A();
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) A[e] = 0;
+      phase = h;
+    } else if (nadd == 3) {
+      reduce_A();
+    }
+    ++nadd;
+    const int T = (party * a.sets + bb) * a.B + inst;
+    Blk cw[4][2];   // levels Ltop .. levels - 1 (w = 1: CW_0 and the fold are equal)
+#pragma unroll
+    for (int l = 0; l < 4; ++l) load_cw_uniform<1>(a, T, Ltop + l, cw[l]);
+    const size_t d0 = rr0 + ((size_t)ds << a.bin_log);  // a multiple of 8
+    const size_t gi = (size_t)T * Stop + (d0 >> 4);
+    const Blk n14 = child1(a.Fp[gi], a.Hp[gi], (int)((d0 >> 3) & 1), cw[0][0]);
+    Blk h14;
+    tree_hash<kPrgChaCha8, 1>(n14, &h14, nullptr);
+    Blk n15[2], h15[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      n15[c] = child1(n14, h14, c, cw[1][0]);
+      tree_hash<kPrgChaCha8, 1>(n15[c], &h15[c], nullptr);
+    }
+    Blk n16[4], h16[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      n16[i] = child1(n15[i >> 1], h15[i >> 1], i & 1, cw[2][0]);
+      tree_hash<kPrgChaCha8, 1>(n16[i], &h16[i], nullptr);
+    }
+    uint64_t sC = 0;
+    uint32_t sT = 0;
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) {
+      const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[3][0]);
+      const uint64_t C = freduce64(out_hash_lo64(leaf));
+      const uint32_t tau = (uint32_t)(leaf.lo & 1ULL);
+      A[e] += C;
+      mw[e] |= (uint64_t)tau << (slot & 63);
+      sC = fadd(sC, C);
+      sT += tau;
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      sC = fadd(sC, shfl_xor_u64(sC, m));
+      sT += (uint32_t)__shfl_xor((int)sT, m, 64);
+    }
+    if (lane == 0) {
+      red[slot][wave][0] = sC;
+      red[slot][wave][1] = sT;
+    }
+    if (threadIdx.x == 0)
+      red_at[slot] = ((uint64_t)T << nparts_log) +
+                     (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + 3));
+  });
+  store_words(widx);
+#pragma unroll
+  for (int e = 0; e < LPT; ++e) mw[e] = 0;
+  for (int wi = widx + 1; wi < a.mask_words; ++wi) store_words(wi);
+  reduce_A();
+  if (phase == 1) {  // gA holds the +C sums
+#pragma unroll
+    for (int e = 0; e < LPT; e += 2) {
+      const Blk p = gdst[e / 2];
+      gdst[e / 2] = Blk{fsub(p.lo, A[e]), fsub(p.hi, A[e + 1])};
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+  }
+  __syncthreads();
+  for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
+    uint64_t c = red[sl][0][0], tt = red[sl][0][1];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      c = fadd(c, red[sl][w][0]);
+      tt += red[sl][w][1];
+    }
+    uint64_t* dst = a.part + red_at[sl] * 2;
+    dst[0] = c;
+    dst[1] = tt;
+  }
+}
+
+// Level 0: the frontier is the root.  Also moves DltSft from the reference
 // layout [T][levels][nk] into the level-major share array [levels][T][nk]
 // and seeds level 0 with H(root) (dpf.cpp:502, :522-532).
 template <int W, int PRG>
@@ -518,7 +717,8 @@ template <int W, int PRG>
 void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
   constexpr int M = 1 << W, NK = M - 1;
   const int n = r.n, levels = n / W;
-  dpf_reserve(s, n, W, (size_t)r.ntrees, r.host_ex != nullptr);
+  const bool deep = dpf_deep_leaf(r);
+  dpf_reserve(s, n, W, (size_t)r.ntrees, r.host_ex != nullptr, deep);
 
   auto frontier = [&](int which, size_t S, Blk*& F, Blk*& H) {
     F = s.fh[which].as<Blk>();
@@ -579,6 +779,37 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     a.chunk_log = ilog2_exact((uint64_t)a.iters) + ilog2_exact((uint64_t)kTpb);
     // the last tree level keeps only its hashes; the leaf level rebuilds
     // those parents from the grandparents (still in the other buffer)
+    if (deep && i == levels - 3) {
+      // the parents' level (levels - 4) is the last one stored: path CWs of
+      // the three levels below it, then the fused leaf level
+      const int half = r.ntrees / 2;
+      const Blk* dlt1 = r.dltsft1 ? r.dltsft1 : r.dltsft + (size_t)half * levels;
+      {
+        ProfZone pz(r.prof, st, "dpf.path_cw");
+        hipLaunchKernelGGL(dpf_path_cw_kernel, dim3((unsigned)((half + kTpb - 1) / kTpb)),
+                           dim3(kTpb), 0, st, F, H, s.cw.as<Blk>(), r.dltsft, dlt1, half, levels,
+                           levels - 3);
+        hip_check(hipGetLastError(), "launch dpf_path_cw_kernel");
+      }
+      a.part = r.partials;
+      a.gA = r.g_acc;
+      a.gA_party_stride = r.g_party_stride;
+      a.gmask = r.g_mask;
+      a.mask_words = r.mask_words;
+      a.t = r.t;
+      a.B = r.B;
+      a.N_log = r.N_log;
+      a.bin_log = n - 1;
+      a.sets = r.sets;
+      a.parties = r.parties;
+      a.level = levels;
+      ProfZone pz(r.prof, st, "dpf.leaf_deep");
+      const dim3 grid((unsigned)(((size_t)1 << a.N_log) / ((size_t)kTpb * 8)), (unsigned)a.sets,
+                      (unsigned)a.parties);
+      hipLaunchKernelGGL(deep_leaf_kernel, grid, dim3(kTpb), 0, st, a);
+      hip_check(hipGetLastError(), "launch deep_leaf_kernel");
+      break;
+    }
     const bool rebuild = i == levels && levels >= 2;
     if (rebuild) {
       Blk *Fg, *Hg;
@@ -647,11 +878,25 @@ int dpf_leaf_nparts(int n, int w, int iters_max) {
   return (int)(S / ((size_t)level_iters(S, iters_max) * kTpb));
 }
 
-void dpf_reserve(DpfScratch& s, int n, int w, size_t ntrees, bool host_exchange) {
+bool dpf_deep_leaf(int n, int w, int prg, int mode, bool host_exchange, int parties) {
+  return w == 1 && prg == kPrgChaCha8 && mode == kLeafGather && !host_exchange && parties == 2 &&
+         n >= 5 && dpf_gather_lpt(n) == 8;
+}
+
+bool dpf_deep_leaf(const DpfRun& r) {
+  return dpf_deep_leaf(r.n, r.w, r.prg, r.mode, r.host_ex != nullptr, r.parties) && !r.cw_out;
+}
+
+size_t dpf_frontier_nodes(int n, int w, bool deep) {
+  // the widest stored level: the last parent level (2^(w(levels-1)) nodes
+  // per tree), or level levels - 4 when the leaf level expands the rest
+  return deep ? (size_t)1 << (n - 4) : (size_t)1 << (w * (n / w - 1));
+}
+
+void dpf_reserve(DpfScratch& s, int n, int w, size_t ntrees, bool host_exchange, bool deep) {
   const int nk = (1 << w) - 1, levels = n / w;
-  // The widest stored level is the last parent level (2^(w(levels-1)) nodes
-  // per tree), each node with its nk hashes.
-  const size_t maxS = (size_t)1 << (w * (levels - 1));
+  // each stored node with its nk hashes
+  const size_t maxS = dpf_frontier_nodes(n, w, deep);
   const size_t need = ntrees * maxS * (size_t)(1 + nk) * sizeof(Blk);
   s.fh[0].ensure(need, "hipMalloc dpf frontier");
   s.fh[1].ensure(need, "hipMalloc dpf frontier");

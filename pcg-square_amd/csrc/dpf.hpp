@@ -84,9 +84,19 @@ int dpf_leaf_nparts(int n, int w, int iters_max);
 int dpf_gather_lpt(int n);
 int dpf_gather_nparts(int n);
 
+// Whether a run expands its last three tree levels inside the leaf level
+// (both parties in one run, w = 1, ChaCha8, fused gather with 8 coefficients
+// per thread): the correction words of those levels come from the alpha
+// path, and only levels up to levels - 4 are stored.
+bool dpf_deep_leaf(const DpfRun& run);
+bool dpf_deep_leaf(int n, int w, int prg, int mode, bool host_exchange, int parties);
+// Nodes per tree of the widest stored frontier level.
+size_t dpf_frontier_nodes(int n, int w, bool deep);
+
 // Allocates everything dpf_expand needs for `ntrees` trees of domain 2^n and
 // arity 2^w, so that a later dpf_expand of that shape allocates nothing.
-void dpf_reserve(DpfScratch& scratch, int n, int w, size_t ntrees, bool host_exchange);
+void dpf_reserve(DpfScratch& scratch, int n, int w, size_t ntrees, bool host_exchange,
+                 bool deep = false);
 
 // Runs the whole level loop on `stream`.  Synchronizes with the host only in
 // host-exchange mode (once per level, like the reference).

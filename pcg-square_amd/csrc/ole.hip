@@ -284,10 +284,17 @@ int leaf_nparts(const OleDims& d) {
   return fused_gather(d) ? dpf_gather_nparts(d.D_log) : dpf_leaf_nparts(d.D_log, d.w, 8);
 }
 
+// Whether the plan's DPF runs expand the last three tree levels inside the
+// leaf level (dpf.hpp, dpf_deep_leaf): only smaller frontiers are stored.
+bool plan_deep(const OleDims& d, int parties) {
+  return dpf_deep_leaf(d.D_log, d.w, d.prg, fused_gather(d) ? kLeafGather : kLeafConvert,
+                       parties == 1, parties);
+}
+
 // parties: 2 when one plan runs both parties, 1 for one party.
 size_t plan_group_bytes(const OleDims& d, int gb, int parties) {
   const size_t trees = (size_t)parties * gb * d.B;
-  const size_t maxS = (size_t)1 << (d.w * (d.levels - 1));
+  const size_t maxS = dpf_frontier_nodes(d.D_log, d.w, plan_deep(d, parties));
   const size_t fh = trees * maxS * (1 + d.nk) * sizeof(Blk) * 2;
   const size_t leaf = fused_gather(d)
                           ? (size_t)gather_mask_words(d) * parties * gb * d.N * sizeof(uint64_t)
@@ -352,7 +359,7 @@ void plan_alloc(pcgsq_ole_plan* pl, size_t budget) {
   pl->out.ensure((size_t)4 * d.N * sizeof(uint64_t), "ole outputs");
   // the DPF frontier and the NTT scratch too, so that a run allocates nothing
   // (an allocation inside a run would stall other streams' work)
-  dpf_reserve(pl->dpf, d.D_log, d.w, gtrees, parties == 1);
+  dpf_reserve(pl->dpf, d.D_log, d.w, gtrees, parties == 1, plan_deep(d, parties));
   (void)pl->ctx->ntt.scratch(((size_t)2 * d.nblk + 2 * d.c) * d.N, 0);
   if (d.prg == PCGSQ_PRG_AES) {
     uint32_t t0[256];

@@ -339,7 +339,13 @@ def test_aes_ecb_fips_and_random(ctx, pq, golden):
     (1 << 14, 2, 8, 3, O.PRG_CHACHA8),
     (1 << 14, 1, 8, 4, O.PRG_AES),
     (1 << 15, 2, 32, 1, O.PRG_CHACHA8),     # 128 landing slots: two tau words
-    (1 << 15, 1, 64, 1, O.PRG_CHACHA8)])    # 256 landing slots: four tau words
+    (1 << 15, 1, 64, 1, O.PRG_CHACHA8),     # 256 landing slots: four tau words
+    # bins of >= 2048, w = 1, ChaCha8: the last three tree levels expand inside
+    # the leaf level, their CWs from the alpha path
+    (1 << 14, 2, 8, 1, O.PRG_CHACHA8),      # 12-level DPFs, 8 levels stored
+    (1 << 15, 3, 16, 1, O.PRG_CHACHA8),
+    (1 << 16, 2, 4, 1, O.PRG_CHACHA8),      # bins of 2^14
+    (1 << 14, 2, 8, 1, O.PRG_AES)])         # AES keeps the level-by-level path
 def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
     seed = 11
     with pq.OlePlan(ctx, N, c, t, w, prg, pub_seed=3) as pl:
@@ -356,11 +362,12 @@ def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
         assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("N,t", [(1 << 13, 16), (1 << 12, 16)])
+@pytest.mark.parametrize("N,t", [(1 << 13, 16), (1 << 12, 16), (1 << 15, 16)])
 def test_ole_one_block_per_group(ctx, pq, N, t):
     """A budget too small for two DPF blocks runs the c^2 blocks one group at
-    a time (fused gather at 2^13, the unfused path at 2^12): the same g and
-    outputs as the oracle."""
+    a time (fused gather at 2^13, the unfused path at 2^12, the leaf level
+    with the last tree levels fused in at 2^15): the same g and outputs as the
+    oracle."""
     c, seed = 2, 5
     assert pq.ole_plan_estimate(N, c, t, 1, O.PRG_CHACHA8, 1) < pq.ole_plan_estimate(N, c, t)
     with pq.OlePlan(ctx, N, c, t, 1, O.PRG_CHACHA8, pub_seed=3, hbm_budget=1) as pl:
