@@ -442,7 +442,9 @@ def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     """Served PCG-OLE: Poisson arrivals (rate = 75% of the measured
     one-at-a-time capacity), N uniform over {2^16, 2^18, 2^20, 2^22}, c=4,
     t=16, a fresh seed per request; a 4-slot continuous-batching server
-    against one request per launch (1 slot) on the same arrival trace."""
+    (GPU-filling requests of 2^20 and 2^22 one at a time in arrival order,
+    smaller ones alongside, skipping ahead bounded at 100 ms) against one
+    request per launch (1 slot) on the same arrival trace."""
     from pcgsq import serving
     sizes = [1 << 16, 1 << 18, 1 << 20, 1 << 22]
     service = {}
@@ -462,17 +464,18 @@ def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     res = {"workload": "Poisson arrivals, N uniform over 2^16..2^22 (x4 steps), c=4, t=16, "
                        "fresh seed per request", "rate_per_s": rate, "duration_s": duration_s,
            "service_ms_alone": {f"2^{N.bit_length() - 1}": v * 1e3 for N, v in service.items()}}
-    # plans up front: two per shape for the 4-slot server (requests of one
-    # shape overlap on them), one per shape for one request per launch; the
-    # 4-slot server's work window lets one 2^22 request run beside smaller
-    # ones but starts the large requests in arrival order
-    window = (1 << 22) + (1 << 19)
-    res["window_N"] = window
-    for slots, per_shape, key in ((4, 2, "continuous_batching_4_slots"),
-                                  (1, 1, "one_request_per_launch")):
-        with pcgsq.Server(0, 0, slots, window=window if slots > 1 else 0) as srv:
+    # plans up front: for the 4-slot server two per small shape (requests of
+    # one shape overlap) and one per GPU-filling shape (2^20, 2^22: the
+    # exclusive class runs them one at a time, in arrival order), with an age
+    # bound of 100 ms on skipping ahead; one plan per shape for one request
+    # per launch
+    excl, age = 1 << 20, 0.100
+    res["policy"] = {"exclusive_N": excl, "age_bound_s": age}
+    for slots, key in ((4, "continuous_batching_4_slots"), (1, "one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots, exclusive_N=excl if slots > 1 else 0,
+                          age_bound_s=age if slots > 1 else 0.0) as srv:
             for N in sizes:
-                srv.reserve(N, 4, 16, per_shape)
+                srv.reserve(N, 4, 16, 2 if slots > 1 and N < excl else 1)
             recs, wall = serving.run_poisson(srv, rate, duration_s, sizes, 4, 16, seed=7)
             st, pl = srv.stats(), srv.plans()
         summ = serving.summarize(recs, wall)
@@ -484,11 +487,11 @@ def bench_served(pcgsq, duration_s: float = 8.0) -> dict:
     import numpy as np
     rng = np.random.default_rng(11)
     mix = [int(sizes[i]) for i in rng.integers(0, len(sizes), 64)]
-    for slots, per_shape, key in ((4, 2, "saturated_4_slots"),
-                                  (1, 1, "saturated_one_request_per_launch")):
-        with pcgsq.Server(0, 0, slots, window=window if slots > 1 else 0) as srv:
+    for slots, key in ((4, "saturated_4_slots"), (1, "saturated_one_request_per_launch")):
+        with pcgsq.Server(0, 0, slots, exclusive_N=excl if slots > 1 else 0,
+                          age_bound_s=age if slots > 1 else 0.0) as srv:
             for N in sizes:
-                srv.reserve(N, 4, 16, per_shape)
+                srv.reserve(N, 4, 16, 2 if slots > 1 and N < excl else 1)
             t0 = pcgsq.Server.clock()
             ids = [srv.submit(N, 4, 16, 5000 + k) for k, N in enumerate(mix)]
             for rid in ids:

@@ -509,6 +509,14 @@ int pcgsq_server_charge(pcgsq_server* s, const pcgsq_params* p, size_t* bytes);
  * large, so that large requests start in arrival order while smaller ones
  * fill the slots around them. */
 int pcgsq_server_set_window(pcgsq_server* s, uint64_t max_running_N);
+/* Admission policy (0 switches either part off, the default):
+ *   exclusive_N: requests with N >= exclusive_N (expansions that fill the
+ *     GPU on their own) run one at a time and start in arrival order among
+ *     themselves; smaller requests fill the other slots around them;
+ *   age_bound_s: a waiting request older than this that cannot start is
+ *     passed by no later request (admission stops at it until it starts), so
+ *     skipping ahead delays no request by more than the bound. */
+int pcgsq_server_set_policy(pcgsq_server* s, uint64_t exclusive_N, double age_bound_s);
 /* Plans in the pool, plans built and plans evicted since creation. */
 int pcgsq_server_plans(pcgsq_server* s, int* plans, long* built, long* evicted);
 /* The server's monotonic clock (seconds). */

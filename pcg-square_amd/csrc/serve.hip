@@ -133,6 +133,8 @@ struct pcgsq_server {
   bool stop = false;
   uint64_t next_id = 1;
   uint64_t window = 0;     // most ring dimension (sum of N) running at once; 0 = no limit
+  uint64_t exclusive_N = 0;  // requests with N >= this run one at a time, in arrival order
+  double age_bound = 0;      // seconds: a request waiting longer is passed by no later one
   long completed = 0, plans_built = 0, plans_evicted = 0;
 
   size_t plan_budget() const { return plan_cap; }
@@ -274,20 +276,32 @@ struct pcgsq_server {
     bool changed = false;
     std::vector<const Request*> passed;  // shapes passed over: later ones of a shape wait too
     uint64_t run_n = 0;                  // ring dimension of the running requests
+    bool excl_running = false;           // an exclusive-class request runs
     for (const auto& x : pool)
-      if (x->req) run_n += (uint64_t)x->req->p.N;
+      if (x->req) {
+        run_n += (uint64_t)x->req->p.N;
+        excl_running = excl_running || (exclusive_N && (uint64_t)x->req->p.N >= exclusive_N);
+      }
     uint64_t held_n = ~0ULL;  // smallest N held back by the window: no larger one passes it
+    bool excl_waiting = false;  // an exclusive-class request waits: no later one of the class starts
+    const double now = now_s();
     auto it = queue.begin();
     while (it != queue.end() && running() < max_running && !stop) {
       Request* r = *it;
       const uint64_t n = (uint64_t)r->p.N;
+      const bool excl = exclusive_N && n >= exclusive_N;
       bool wait = n >= held_n;
       for (const Request* q : passed) wait = wait || same_shape(*q, *r);
       if (!wait && window && run_n > 0 && run_n + n > window) {
         held_n = n;
         wait = true;
       }
+      if (!wait && excl && (excl_running || excl_waiting)) wait = true;
+      if (excl && wait) excl_waiting = true;
+      // an aged request that cannot start stops every later one from passing it
+      const bool aged = age_bound > 0 && now - r->t_submit > age_bound;
       if (wait) {
+        if (aged) break;
         ++it;
         continue;
       }
@@ -339,6 +353,8 @@ struct pcgsq_server {
         e = pool.back().get();
       }
       if (!e) {  // its shape's plans are busy: wait for one
+        if (aged) break;
+        if (excl) excl_waiting = true;
         passed.push_back(r);
         ++it;
         continue;
@@ -364,7 +380,10 @@ struct pcgsq_server {
       r->out = ob;
       it = queue.erase(it);
       start(r, e, k);
-      if (e->req) run_n += n;
+      if (e->req) {
+        run_n += n;
+        excl_running = excl_running || excl;
+      }
       changed = true;
     }
     return changed;
@@ -626,6 +645,19 @@ int pcgsq_server_set_window(pcgsq_server* s, uint64_t max_running_N) {
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->window = max_running_N;
+    }
+    s->cv.notify_all();
+  });
+}
+
+int pcgsq_server_set_policy(pcgsq_server* s, uint64_t exclusive_N, double age_bound_s) {
+  return guarded([&] {
+    require(s, "server_set_policy: null server");
+    require(age_bound_s >= 0, "server_set_policy: age bound must be >= 0");
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->exclusive_N = exclusive_N;
+      s->age_bound = age_bound_s;
     }
     s->cv.notify_all();
   });

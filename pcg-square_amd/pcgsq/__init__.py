@@ -182,6 +182,7 @@ def lib() -> C.CDLL:
     L.pcgsq_server_plans.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long),
                                      C.POINTER(C.c_long)]
     L.pcgsq_server_set_window.argtypes = [vp, C.c_uint64]
+    L.pcgsq_server_set_policy.argtypes = [vp, C.c_uint64, C.c_double]
     L.pcgsq_server_reserve.argtypes = [vp, C.POINTER(Params), C.c_uint64, C.c_int]
     L.pcgsq_dealer_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]
     L.pcgsq_dealer_u64.restype = C.c_uint64
@@ -1025,12 +1026,21 @@ class Server:
     (pcgsq_server_*): `slots` concurrent execution slots, admission by
     device-memory budget, outputs kept on the device until read/released."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4, window: int = 0):
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4, window: int = 0,
+                 exclusive_N: int = 0, age_bound_s: float = 0.0):
         h = vp()
         _check(lib().pcgsq_server_create(device, hbm_budget, slots, C.byref(h)))
         self.h = h
         if window:
             self.set_window(window)
+        if exclusive_N or age_bound_s:
+            self.set_policy(exclusive_N, age_bound_s)
+
+    def set_policy(self, exclusive_N: int = 0, age_bound_s: float = 0.0) -> None:
+        """Requests with N >= exclusive_N run one at a time, in arrival order
+        among themselves; a request waiting longer than age_bound_s is passed
+        by no later one.  0 switches either off."""
+        _check(lib().pcgsq_server_set_policy(self.h, exclusive_N, age_bound_s))
 
     def set_window(self, max_running_N: int) -> None:
         """Most ring dimension (sum of N) running at once; 0 = no limit.  A

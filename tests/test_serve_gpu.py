@@ -219,3 +219,71 @@ def test_served_outputs_match_precomputed_cpu_hashes(pq):
             got = [f"{h:016x}" for h in _digest(srv.outputs(rid, r["N"]))]
             assert got == r["fnv1a64"], f"N={r['N']} seed={r['seed']}: hashes differ from the CPU oracle"
             srv.release(rid)
+
+
+def test_served_exclusive_class_runs_large_requests_alone(pq):
+    """exclusive_N = 2^20: 2^20 and 2^22 requests never overlap one another
+    and start in arrival order even with two plans per shape, while 2^16
+    requests submitted after them run alongside; every result equals the same
+    request run alone."""
+    with pq.Server(0, 0, 4, exclusive_N=1 << 20) as srv:
+        for N in (1 << 16, 1 << 20, 1 << 22):
+            srv.reserve(N, 4, 16, 2)
+        heavy = [(1 << 20, 601), (1 << 22, 602), (1 << 20, 603), (1 << 22, 604)]
+        small = [(1 << 16, 610 + k) for k in range(4)]
+        hid = [srv.submit(N, 4, 16, s) for N, s in heavy]
+        sid = [srv.submit(N, 4, 16, s) for N, s in small]
+        for rid in hid + sid:
+            srv.wait(rid, 120.0)
+        th = [srv.timing(rid) for rid in hid]
+        for (_, s0, d0), (_, s1, _) in zip(th, th[1:]):
+            assert s1 >= d0 - 1e-3, "two exclusive-class requests overlapped"
+        ts = [srv.timing(rid) for rid in sid]
+        assert all(s < th[0][2] for _, s, _ in ts), "small requests did not run alongside"
+        outs = {rid: _digest(srv.outputs(rid, N)) for rid, (N, _) in zip(hid + sid, heavy + small)}
+        for rid in hid + sid:
+            srv.release(rid)
+    with pq.Server(0, 0, 1) as alone:
+        for rid, (N, seed) in zip(hid + sid, heavy + small):
+            r2 = alone.submit(N, 4, 16, seed)
+            alone.wait(r2, 120.0)
+            assert _digest(alone.outputs(r2, N)) == outs[rid], f"N={N} seed={seed}"
+            alone.release(r2)
+
+
+def test_served_age_bound_stops_skip_ahead(pq):
+    """A 2^22 request B that waits for the 2^22 request A before it (one plan
+    of that shape) is passed by 2^16 requests submitted after it only until B
+    is older than the age bound: every later request that starts before B
+    starts within the bound (plus the worker's polling), and none of those
+    submitted after B aged starts before B."""
+    bound = 0.010
+    with pq.Server(0, 0, 4, age_bound_s=bound) as srv:
+        srv.reserve(1 << 22, 4, 16, 1)
+        srv.reserve(1 << 16, 4, 16, 2)
+        a = srv.submit(1 << 22, 4, 16, 701)
+        b = srv.submit(1 << 22, 4, 16, 702)
+        smalls = []
+        t_end = pq.Server.clock() + 0.15
+        k = 0
+        while pq.Server.clock() < t_end:   # a stream of small requests behind B
+            smalls.append(srv.submit(1 << 16, 4, 16, 710 + k))
+            k += 1
+            import time
+            time.sleep(0.002)
+        for rid in [a, b] + smalls:
+            srv.wait(rid, 120.0)
+        tb_sub, tb_start, _ = srv.timing(b)
+        _, _, ta_done = srv.timing(a)
+        assert tb_start >= ta_done - 1e-3
+        early = 0
+        for rid in smalls:
+            sub, start, _ = srv.timing(rid)
+            if start < tb_start:
+                early += 1
+                assert start <= tb_sub + bound + 0.005, "a later request passed B after it aged"
+            if sub > tb_sub + bound:
+                assert start >= tb_start - 1e-3, "a request submitted after B aged started first"
+        assert early >= 1, "no small request ran while B waited within its bound"
+        for rid in [a, b] + smalls:
+            srv.release(rid)
