@@ -213,6 +213,24 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     const uint32_t L = sigma & ((1u << TMIN) - 1);
     const uint32_t Hh = sigma >> TMIN;
     const uint32_t base = (Hh << (TMIN + R)) | L;
+    // a storing first pass with sets of up to 8 elements: issue the set's
+    // four-step twiddle gathers now (8 VGPRs per element), so that their cache
+    // latency overlaps the gather and butterflies below
+    constexpr bool TW = (ROLE & kToHbm) && Tr::STORE == 1 && M <= 8;
+    Pair twa[TW ? M : 1], twb[TW ? M : 1];
+    if constexpr (TW) {
+      const uint32_t two_n_mask = (2u << d.N_log) - 1;
+      const uint32_t lo_mask = (1u << d.tw_lo_bits) - 1;
+      const uint32_t cc = c0 + c;
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        const uint32_t pos = base | ((uint32_t)w << TMIN);
+        const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
+        const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
+        twa[w] = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
+        twb[w] = ld_pair((gcptr)d.tw_hi, ex >> d.tw_lo_bits);
+      }
+    }
     uint64_t v[M];
     if constexpr (ROLE & kFromHbm) {
 #pragma unroll
@@ -241,11 +259,14 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
 #pragma unroll
       for (int w = 0; w < M; ++w) {
         // bound the twiddle loads in flight (each is 4 VGPRs)
-        if ((w & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+        if (!TW && (w & 3) == 0) __builtin_amdgcn_sched_barrier(0);
         const uint32_t pos = base | ((uint32_t)w << TMIN);
         const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
         uint64_t x = v[w];
-        if constexpr (Tr::STORE == 1) {
+        if constexpr (TW) {
+          x = shoup_lazy_dev(x, twa[w].w, twa[w].wq);
+          x = shoup_lazy_dev(x, twb[w].w, twb[w].wq);
+        } else if constexpr (Tr::STORE == 1) {
           const uint32_t cc = c0 + c;
           const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
           const Pair a = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
