@@ -297,9 +297,10 @@ def _two_party_threads(pcgsq, N, c, t, mode: str, rtt_us: float, reps: int) -> d
     one-party plan, exchanging through an in-memory queue pair; party 0
     waits rtt_us more per exchange (a network round trip: the reference's
     PCG_RTT_US, GPU_baseline/dpf_real_bench.cu:64-81).  mode: "dealer" (the
-    trusted-dealer simulation: the run starts at the DPF), "own" (each party
-    from its own seed, the 2PC preprocessing first; per-block exchanges, the
-    reference's serial arm), "own_batched" (the same in its batched arm)."""
+    trusted-dealer simulation: the run starts at the DPF; per-block exchanges,
+    the reference's serial arm), "dealer_batched" (the same in its batched
+    arm: one exchange per level for all blocks), "own" / "own_batched" (each
+    party from its own seed, the 2PC preprocessing first, serial / batched)."""
     import queue
     import threading
     inbox = (queue.Queue(), queue.Queue())
@@ -364,12 +365,14 @@ def bench_two_parties(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
     res = {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
                        "as two threads with a context and one-party plan each, exchanges "
                        "through a queue pair; both on this one GPU"}
-    res["dealer_serial"] = _two_party_threads(pcgsq, N, c, t, "dealer", 0, reps)
     for rtt in (0, 5, 50, 500):
-        for mode, key in (("own", "own_serial"), ("own_batched", "own_batched")):
+        for mode, key in (("dealer", "dealer_serial"), ("dealer_batched", "dealer_batched"),
+                          ("own", "own_serial"), ("own_batched", "own_batched")):
             r = _two_party_threads(pcgsq, N, c, t, mode, rtt, 2 if rtt >= 50 else reps)
             res.setdefault(key, {})[f"rtt_{rtt}us"] = r
-    res["as_os_processes_over_gloo"] = bench_two_processes(N, c, t, reps)
+    res["as_os_processes_over_gloo"] = {
+        mode: bench_two_processes(N, c, t, mode, reps)
+        for mode in ("dealer_batched", "own_batched", "own")}
     return res
 
 
@@ -384,9 +387,15 @@ dist.init_process_group("gloo", init_method="tcp://127.0.0.1:" + os.environ["MAS
                         rank=rank, world_size=2)
 ex = pcgsq.dist_exchange(1 - rank)
 times = []
+mode = os.environ["PCGSQ_MODE"]
 with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, rank) as op:
+    if mode.endswith("batched"):
+        op.set_exchange_mode(pcgsq.EXCHANGE_BATCHED)
     for k in range(reps + 1):   # the first round warms up (and runs the base OTs)
-        op.deal_own(1)
+        if mode.startswith("own"):
+            op.deal_own(1)
+        else:
+            op.deal(1)
         dist.barrier()
         t0 = time.perf_counter()
         op.run(ex)
@@ -400,11 +409,11 @@ dist.destroy_process_group()
 """
 
 
-def bench_two_processes(N: int, c: int, t: int, reps: int = 3) -> dict:
+def bench_two_processes(N: int, c: int, t: int, mode: str, reps: int = 3) -> dict:
     """The 2PC expansion as the reference deploys it: one OS process per
     party, every exchange over a network transport (torch.distributed gloo
     on 127.0.0.1, pcgsq.dist_exchange, in place of emp::NetIO), both
-    processes on this GPU."""
+    processes on this GPU.  mode as in _two_party_threads."""
     import socket
     import subprocess
     with socket.socket() as so:
@@ -415,7 +424,7 @@ def bench_two_processes(N: int, c: int, t: int, reps: int = 3) -> dict:
     for rank in (0, 1):
         env = dict(os.environ, RANK=str(rank), MASTER_PORT=str(port),
                    PCGSQ_PKG=os.path.join(here, "pcg-square_amd"),
-                   PCGSQ_CASE=f"{N},{c},{t},{reps}")
+                   PCGSQ_CASE=f"{N},{c},{t},{reps}", PCGSQ_MODE=mode)
         procs.append(subprocess.Popen([sys.executable, "-c", _PARTY_PROCESS], env=env,
                                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
     logs = []
@@ -431,10 +440,12 @@ def bench_two_processes(N: int, c: int, t: int, reps: int = 3) -> dict:
         return {"error": (logs[0] + logs[1])[-400:]}
     times = json.loads(res[0][len("RESULT "):])
     ms = 1e3 * sum(times) / len(times)
+    how = ("from a dealt session (trusted-dealer simulation)" if mode.startswith("dealer")
+           else "each from its own seed with the 2PC preprocessing")
+    arm = "batched" if mode.endswith("batched") else "serial"
     return {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 and party 1 "
-                        "as two OS processes, each from its own seed with the 2PC "
-                        "preprocessing, serial arm, exchanges over torch.distributed gloo (TCP, "
-                        "127.0.0.1); both on this one GPU",
+                        f"as two OS processes, {how}, {arm} arm, exchanges over "
+                        "torch.distributed gloo (TCP, 127.0.0.1); both on this one GPU",
             "ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
 
 

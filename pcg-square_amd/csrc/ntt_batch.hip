@@ -115,7 +115,7 @@ __device__ __noinline__ void tile_fn(const NttPassDesc* d, uint32_t tile) {
   uint32_t* dst = reinterpret_cast<uint32_t*>(&desc);
 #pragma unroll
   for (int k = 0; k < (int)(sizeof(NttPassDesc) / 4); ++k) dst[k] = src[k];
-  nttpass::pass_tile<K, A, B, C>(desc, batch_lds, 0, __builtin_amdgcn_readfirstlane(tile));
+  nttpass::pass_tile<K, A, B, C, false>(desc, batch_lds, 0, __builtin_amdgcn_readfirstlane(tile));
 }
 
 // the host sizes tiles by batch_rmax / batch_groups: every shape must agree

@@ -182,7 +182,7 @@ enum : int { kFromHbm = 1, kToHbm = 2, kCopy = 4 };
 
 // Group `S0, R` of a pass over a tile: for each set owned by this thread,
 // gather (from HBM or LDS), run the stages, scatter (to LDS or HBM).
-template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G>
+template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G, bool TWPRE>
 __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
                                           uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
                                           gcptr __restrict__ inb, gcptr __restrict__ in2b,
@@ -213,10 +213,10 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     const uint32_t L = sigma & ((1u << TMIN) - 1);
     const uint32_t Hh = sigma >> TMIN;
     const uint32_t base = (Hh << (TMIN + R)) | L;
-    // a storing first pass with sets of up to 8 elements: issue the set's
-    // four-step twiddle gathers now (8 VGPRs per element), so that their cache
-    // latency overlaps the gather and butterflies below
-    constexpr bool TW = (ROLE & kToHbm) && Tr::STORE == 1 && M <= 8;
+    // a storing first pass with sets of up to 8 elements (TWPRE): issue the
+    // set's four-step twiddle gathers now (8 VGPRs per element), so that their
+    // cache latency overlaps the gather and butterflies below
+    constexpr bool TW = TWPRE && (ROLE & kToHbm) && Tr::STORE == 1 && M <= 8;
     Pair twa[TW ? M : 1], twb[TW ? M : 1];
     if constexpr (TW) {
       const uint32_t two_n_mask = (2u << d.N_log) - 1;
@@ -306,7 +306,9 @@ struct Part {
 // One tile (C columns starting at column tile `ct` of the transform that
 // starts at word `base` of in, in2 and out).
 // Threads t >= Part::threads(C) idle but take part in the barriers.
-template <int K, int RA, int RB, int RC>
+// TWPRE: prefetch a first pass's twiddles (more VGPRs; off where the
+// registers of every shape add up, as in the persistent batch kernel).
+template <int K, int RA, int RB, int RC, bool TWPRE = true>
 __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __restrict__ lds,
                                           size_t base, uint32_t ct) {
   using P = Part<RA, RB, RC>;
@@ -323,35 +325,35 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every
     // element before any bit-reversed store lands
-    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
     __syncthreads();
     if (active)
-      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
   } else if constexpr (P::G == 2) {
     constexpr int SA = 0, SB = RA;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
     } else {
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
     }
   } else {
     constexpr int SA = 0, SB = RA, SC = RA + RB;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
     } else {
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
     }
   }
 }
