@@ -207,6 +207,8 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
             "ms": ms, "alg_bytes": info["alg_bytes"], "GBps": gbps,
             "hbm_frac": gbps * 1e9 / HBM_PEAK, "work_items": info["items"],
             "grid": info["grid"], "tuned": {"mode": shape["mode"],
+                                            "persistent_ms": shape.get("persistent_ms"),
+                                            "grouped_ms": shape.get("grouped_ms"),
                                             "wgs_per_cu": shape["wgs_per_cu"],
                                             "splits": {f"2^{k}": v for k, v in
                                                        shape["splits"].items()}},

@@ -2,8 +2,8 @@
 //
 // A batch holds `count` independent transforms of lengths 2^10..2^22, each in
 // its own slice of the batch buffers.  The host turns every pass of every
-// transform into tiles (the same tiles the per-length launches use, shaped
-// for 256 threads) and writes them into work lists.  One persistent kernel
+// transform into tiles (the tiles the per-length launches use, shaped for
+// 512 threads) and writes them into work lists.  One persistent kernel
 // drains the lists: a workgroup takes the next item with one atomic, waits
 // until every tile of the pass the item depends on has finished (per-pass
 // completion counters, a bounded spin), runs the tile and counts it done.
@@ -45,7 +45,11 @@
 namespace pcgsq {
 namespace {
 
-constexpr int kBatchThreads = 256;
+// 512 threads: a 2^10 column pass gets 8 columns per tile (64 contiguous
+// bytes of each row) and the per-item bookkeeping (claim, dependency check,
+// descriptor loads, store drain) is spread over twice the work of a
+// 256-thread tile; two such workgroups fill a CU's LDS.
+constexpr int kBatchThreads = 512;
 constexpr int kMaxQueues = 16;      // work lists (one per XCD)
 constexpr int kCtrLine = 32;        // counters of different lists sit 128 bytes apart
 constexpr int kCtrErr = kMaxQueues * kCtrLine;
@@ -72,11 +76,11 @@ struct BatchQueues {
   uint32_t n;     // lists in use (the device's XCD count)
 };
 
-// Register grouping per pass length, chosen so a tile is 256 threads wide:
-// the largest group and the number of groups of the shapes below.  The kernel
-// takes the VGPRs of its largest shape; 2^8 runs as 3+3+2 stages (two 4-stage
-// groups need 132 VGPRs, which leaves room for only 3 workgroups per CU where
-// the LDS holds 4; every shape now stays under 128).
+// Register grouping per pass length: the largest group and the number of
+// groups of the shapes below.  The kernel takes the VGPRs of its largest
+// shape, so every shape stays under 128 (four waves per SIMD): 2^8 runs as
+// 3+3+2 stages (two 4-stage groups need 132 VGPRs), and the first passes do
+// not prefetch their four-step twiddles.
 __host__ __device__ constexpr int batch_rmax(int nlog) {
   return (nlog == 6 || nlog == 8 || nlog == 9) ? 3 : 4;
 }

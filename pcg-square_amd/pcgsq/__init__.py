@@ -913,7 +913,7 @@ class NttBatch:
             self.tune(L, keep, best["wgs_per_cu"])
             if keep:
                 best["splits"][L] = keep
-        for w in (2, 3, 4):
+        for w in (1, 2, 3, 4):
             self.tune(0, 0, w)
             ms = timed()
             if log:
@@ -922,8 +922,10 @@ class NttBatch:
                 best["ms"], best["wgs_per_cu"] = ms, w
         self.tune(0, 0, best["wgs_per_cu"])
         best["mode"] = "persistent"
+        best["persistent_ms"] = best["ms"]
         self.set_mode(NTT_BATCH_GROUPED)
         ms = timed()
+        best["grouped_ms"] = ms
         if log:
             log(f"batch grouped by length: {ms:.3f} ms")
         if ms < best["ms"]:
