@@ -127,7 +127,8 @@ def cpu_baseline(N: int, c: int, t: int, threads: int, polymul_sample: int = 0) 
 NTT_BATCH_PROFILE = os.path.join(ROOT, "profiles", "ntt_batch_256_uneven_2e12_2e20.md")
 
 
-def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
+def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None,
+                    splits: dict | None = None, wgs_per_cu: int = 0) -> dict:
     """Batched NTT of uneven lengths: 256 transforms, lengths drawn uniformly
     from {2^12, 2^14, 2^16, 2^18, 2^20}, uniform coefficients in [0, P),
     forward then inverse of every transform in one replay of the batch's
@@ -136,7 +137,9 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
     by length over offset tables); against one launch per transform (the
     plain form) and against the same transforms regrouped by length into
     contiguous batches.  `mode` ("persistent" / "grouped") skips the tuner and
-    the comparisons: only that mode runs (the profiled command)."""
+    the comparisons: only that mode runs (the profiled command), with the
+    persistent launch's splits {L: n1_log} and workgroups per CU if given
+    (the shape the tuner picked)."""
     import numpy as np
     rng = np.random.default_rng(2024)
     logN = [int(12 + 2 * x) for x in rng.integers(0, 5, 256)]
@@ -146,7 +149,10 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
             shape = b.autotune()
         else:
             b.set_mode(pcgsq.NTT_BATCH_GROUPED if mode == "grouped" else pcgsq.NTT_BATCH_PERSISTENT)
-            shape = {"mode": mode, "wgs_per_cu": 0, "splits": {}}
+            for L, s1 in (splits or {}).items():
+                b.tune(L, s1, wgs_per_cu)
+            b.tune(0, 0, wgs_per_cu)
+            shape = {"mode": mode, "wgs_per_cu": wgs_per_cu, "splits": dict(splits or {})}
         b.run(graph=True)
         b.finish()
         ctx.profile(True)
@@ -159,7 +165,7 @@ def bench_ntt_batch(ctx, pcgsq, reps: int = 5, mode: str | None = None) -> dict:
         info = b.info()
         pa, _, _, _ = b.dev()
         if mode is not None:
-            return {"mode": mode, "ms": ms, "alg_bytes": info["alg_bytes"],
+            return {"mode": mode, "shape": shape, "ms": ms, "alg_bytes": info["alg_bytes"],
                     "GBps": info["alg_bytes"] / (ms * 1e-3) / 1e9, "replays": reps + 1}
 
         def plain():
@@ -532,6 +538,9 @@ def main() -> int:
     ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
                     help="batched NTT: run only this mode, untuned, with no comparisons "
                          "(for profiling it)")
+    ap.add_argument("--ntt-batch-splits", default="",
+                    help="with --ntt-batch-mode persistent: splits L:n1_log,... and "
+                         "workgroups per CU as wgs:K (e.g. 20:10,18:10,wgs:2)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
     unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "parties", "served"}
@@ -550,7 +559,11 @@ def main() -> int:
         with pcgsq.Context(0) as ctx:
             res["hbm_copy_GBps"] = ctx.probe_copy(1 << 30) / 1e9   # PMC calibration kernel
             if "ntt_batch" in parts:
-                res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq, mode=args.ntt_batch_mode)
+                sp = dict(tuple(int(v) if v != "wgs" else v for v in kv.split(":"))
+                          for kv in args.ntt_batch_splits.split(",") if kv)
+                wgs = sp.pop("wgs", 0)
+                res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq, mode=args.ntt_batch_mode,
+                                                     splits=sp, wgs_per_cu=wgs)
             if "dpf_batch" in parts:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:
