@@ -272,22 +272,25 @@ def bench_dpf_batch(ctx, pcgsq, ceiling_blocks: float, reps: int = 5) -> dict:
 
 
 def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
-    """Higher DPF arity (w = 1, 2, 4: up to 15 children per ChaCha8 block) at
-    N = 2^20, c = 4, t = 32 (DPF domain 2^16, divisible by every w; the
+    """Higher DPF arity (up to 15 children per ChaCha8 block) at N = 2^20,
+    c = 4: t = 32 (DPF domain 2^16) with w = 1, 2, 4 and t = 8 (domain 2^18)
+    with w = 1, 2, 3, the arities that divide the domain's bits (the
     headline's 2^17 admits only w = 1)."""
-    N, c, t = 1 << 20, 4, 32
-    out = {"workload": "pcg-ole expansion N=2^20, c=4, t=32, ChaCha8, both parties"}
-    for w in (1, 2, 4):
-        with pcgsq.OlePlan(ctx, N, c, t, w) as pl:
-            pl.deal(1)
-            pl.run()
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
+    N, c = 1 << 20, 4
+    out = {"workload": "pcg-ole expansion N=2^20, c=4, ChaCha8, both parties; "
+                       "t=32 (w1, w2, w4) and t=8 (t8_w1, t8_w2, t8_w3)"}
+    for t, ws, tag in ((32, (1, 2, 4), ""), (8, (1, 2, 3), "t8_")):
+        for w in ws:
+            with pcgsq.OlePlan(ctx, N, c, t, w) as pl:
+                pl.deal(1)
                 pl.run()
-            ctx.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / reps
-        out[f"w{w}"] = {"ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    pl.run()
+                ctx.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / reps
+            out[f"{tag}w{w}"] = {"ms": ms, "OLEs_per_s": N / (ms * 1e-3)}
     return out
 
 
