@@ -522,9 +522,18 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
   Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
                                      ((size_t)bb << a.N_log) + q0);
-  uint64_t A[LPT], mw[LPT];
+  // tau bits: cur holds the current half (32 slots) of the current 64-slot
+  // word; the low half goes out when the slots reach the high half, the rest
+  // when the word is done (slots come in increasing order)
+  uint64_t A[LPT];
+  uint32_t cur[LPT];
 #pragma unroll
-  for (int e = 0; e < LPT; ++e) A[e] = mw[e] = 0;
+  for (int e = 0; e < LPT; ++e) {
+    A[e] = 0;
+    cur[e] = 0;
+  }
+  int half = 0;
+  const int addr32 = (lane ^ 32) << 2;
   int nadd = 0, phase = 0;
   auto reduce_A = [&]() {
 #pragma unroll
@@ -532,18 +541,35 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
     nadd = 0;
   };
   int widx = 0;
-  auto store_words = [&](int wi) {
-    Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
+  auto store_half = [&](int wi, int hf) {  // cur as half hf of word wi
+    uint32_t* dst =
+        reinterpret_cast<uint32_t*>(a.gmask + (size_t)wi * word_stride + set_off + q0) + hf;
 #pragma unroll
-    for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{mw[e], mw[e + 1]};
+    for (int e = 0; e < LPT; ++e) dst[2 * e] = cur[e];
+  };
+  auto finish_word = [&](int wi) {  // the word's last bits; zeros for an untouched high half
+    if (half) {
+      store_half(wi, 1);
+    } else {
+      Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{cur[e], cur[e + 1]};
+    }
   };
   const int nslots = for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h,
                                                                              int ds, int inst) {
     if ((slot >> 6) != widx) {
-      store_words(widx);
+      finish_word(widx);
       widx = slot >> 6;
+      half = 0;
 #pragma unroll
-      for (int e = 0; e < LPT; ++e) mw[e] = 0;
+      for (int e = 0; e < LPT; ++e) cur[e] = 0;
+    }
+    if (!half && (slot & 32)) {
+      store_half(widx, 0);
+      half = 1;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) cur[e] = 0;
     }
     if (h != phase) {
       reduce_A();
@@ -577,23 +603,35 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
       n16[i] = child1(n15[i >> 1], h15[i >> 1], i & 1, cw[2][0]);
       tree_hash<kPrgChaCha8, 1>(n16[i], &h16[i], nullptr);
     }
-    uint64_t sC = 0;
+    uint64_t Cs[LPT];
+    uint32_t taus[LPT];
     uint32_t sT = 0;
 #pragma unroll
     for (int e = 0; e < LPT; ++e) {
       const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[3][0]);
-      const uint64_t C = freduce64(out_hash_lo64(leaf));
-      const uint32_t tau = (uint32_t)(leaf.lo & 1ULL);
-      A[e] += C;
-      mw[e] |= (uint64_t)tau << (slot & 63);
-      sC = fadd(sC, C);
-      sT += tau;
+      Cs[e] = freduce64(out_hash_lo64(leaf));
+      taus[e] = (uint32_t)leaf.lo & 1u;
+      A[e] += Cs[e];
+      sT += taus[e];
     }
 #pragma unroll
-    for (int m = 32; m > 0; m >>= 1) {
-      sC = fadd(sC, shfl_xor_u64(sC, m));
-      sT += (uint32_t)__shfl_xor((int)sT, m, 64);
-    }
+    for (int e = 0; e < LPT; ++e) cur[e] |= taus[e] << (slot & 31);
+    // sum of the 8 C (each < P) mod P: four at a time stay below 4P < 2^64
+    uint64_t sC = fadd(freduce64((Cs[0] + Cs[1]) + (Cs[2] + Cs[3])),
+                       freduce64((Cs[4] + Cs[5]) + (Cs[6] + Cs[7])));
+    static_assert(LPT == 8, "the pairwise sum above covers 8 leaves");
+    sC = fadd(sC, lane_xor_u64<32>(sC, addr32));
+    sT += lane_xor_u32<32>(sT, addr32);
+    sC = fadd(sC, lane_xor_u64<16>(sC, addr32));
+    sT += lane_xor_u32<16>(sT, addr32);
+    sC = fadd(sC, lane_xor_u64<8>(sC, addr32));
+    sT += lane_xor_u32<8>(sT, addr32);
+    sC = fadd(sC, lane_xor_u64<4>(sC, addr32));
+    sT += lane_xor_u32<4>(sT, addr32);
+    sC = fadd(sC, lane_xor_u64<2>(sC, addr32));
+    sT += lane_xor_u32<2>(sT, addr32);
+    sC = fadd(sC, lane_xor_u64<1>(sC, addr32));
+    sT += lane_xor_u32<1>(sT, addr32);
     if (lane == 0) {
       red[slot][wave][0] = sC;
       red[slot][wave][1] = sT;
@@ -602,10 +640,11 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
       red_at[slot] = ((uint64_t)T << nparts_log) +
                      (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + 3));
   });
-  store_words(widx);
+  finish_word(widx);
+  half = 0;
 #pragma unroll
-  for (int e = 0; e < LPT; ++e) mw[e] = 0;
-  for (int wi = widx + 1; wi < a.mask_words; ++wi) store_words(wi);
+  for (int e = 0; e < LPT; ++e) cur[e] = 0;
+  for (int wi = widx + 1; wi < a.mask_words; ++wi) finish_word(wi);
   reduce_A();
   if (phase == 1) {  // gA holds the +C sums
 #pragma unroll
