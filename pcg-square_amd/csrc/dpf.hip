@@ -506,10 +506,18 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
 __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   constexpr int LPT = 8;
   constexpr int TILE = kTpb * LPT;
-  __shared__ uint64_t red[kGatherMaxSlots][4][2];
+  // per-slot workgroup sums, by LDS atomics: the low and the high 32 bits of
+  // the threads' C sums (each below 2^40 and 2^38 over 256 threads) and the
+  // tau count
+  __shared__ unsigned long long red[kGatherMaxSlots][2];
+  __shared__ uint32_t red_t[kGatherMaxSlots];
   __shared__ uint64_t red_at[kGatherMaxSlots];
+  for (int i = threadIdx.x; i < kGatherMaxSlots; i += kTpb) {
+    red[i][0] = red[i][1] = 0;
+    red_t[i] = 0;
+  }
+  __syncthreads();
   const int bb = blockIdx.y, party = blockIdx.z;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t qt = blockIdx.x * TILE;
   const uint32_t q0 = qt + LPT * threadIdx.x;
   const size_t bin_mask = ((size_t)1 << a.bin_log) - 1;
@@ -522,22 +530,34 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
   Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
                                      ((size_t)bb << a.N_log) + q0);
+  // Sums per coefficient: a leaf's out-hash word x = 2^62 h + l (h < 4) goes
+  // in raw, l into A and h into Ah, so that A + Ah * (2^62 mod P) = sum of x
+  // mod P.  Three l (each below 2^62) on an A below P stay below 2^64: A is
+  // reduced every third slot.  Ah (at most 3 per slot, 256 slots) is folded
+  // in at the phase switch and at the end.
   // tau bits: cur holds the current half (32 slots) of the current 64-slot
   // word; the low half goes out when the slots reach the high half, the rest
-  // when the word is done (slots come in increasing order)
+  // when the word is done (slots come in increasing order).
   uint64_t A[LPT];
-  uint32_t cur[LPT];
+  uint32_t Ah[LPT], cur[LPT];
 #pragma unroll
   for (int e = 0; e < LPT; ++e) {
     A[e] = 0;
-    cur[e] = 0;
+    Ah[e] = cur[e] = 0;
   }
   int half = 0;
-  const int addr32 = (lane ^ 32) << 2;
   int nadd = 0, phase = 0;
   auto reduce_A = [&]() {
 #pragma unroll
     for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
+    nadd = 0;
+  };
+  auto fold_A = [&]() {  // A mod P with the high parts: Ah * kC62 < 2^38 < P
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) {
+      A[e] = fadd(freduce64(A[e]), (uint64_t)Ah[e] * kC62);
+      Ah[e] = 0;
+    }
     nadd = 0;
   };
   int widx = 0;
@@ -572,7 +592,7 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
       for (int e = 0; e < LPT; ++e) cur[e] = 0;
     }
     if (h != phase) {
-      reduce_A();
+      fold_A();
 #pragma unroll
       for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
 #pragma unroll
@@ -603,39 +623,31 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
       n16[i] = child1(n15[i >> 1], h15[i >> 1], i & 1, cw[2][0]);
       tree_hash<kPrgChaCha8, 1>(n16[i], &h16[i], nullptr);
     }
-    uint64_t Cs[LPT];
-    uint32_t taus[LPT];
-    uint32_t sT = 0;
+    uint64_t L[LPT];
+    uint32_t taus[LPT], sT = 0, sH = 0;
 #pragma unroll
     for (int e = 0; e < LPT; ++e) {
       const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[3][0]);
-      Cs[e] = freduce64(out_hash_lo64(leaf));
+      const uint64_t x = out_hash_lo64(leaf);
+      L[e] = x & ((1ULL << 62) - 1);
+      const uint32_t hx = (uint32_t)(x >> 62);
       taus[e] = (uint32_t)leaf.lo & 1u;
-      A[e] += Cs[e];
+      A[e] += L[e];
+      Ah[e] += hx;
+      sH += hx;
       sT += taus[e];
     }
 #pragma unroll
     for (int e = 0; e < LPT; ++e) cur[e] |= taus[e] << (slot & 31);
-    // sum of the 8 C (each < P) mod P: four at a time stay below 4P < 2^64
-    uint64_t sC = fadd(freduce64((Cs[0] + Cs[1]) + (Cs[2] + Cs[3])),
-                       freduce64((Cs[4] + Cs[5]) + (Cs[6] + Cs[7])));
-    static_assert(LPT == 8, "the pairwise sum above covers 8 leaves");
-    sC = fadd(sC, lane_xor_u64<32>(sC, addr32));
-    sT += lane_xor_u32<32>(sT, addr32);
-    sC = fadd(sC, lane_xor_u64<16>(sC, addr32));
-    sT += lane_xor_u32<16>(sT, addr32);
-    sC = fadd(sC, lane_xor_u64<8>(sC, addr32));
-    sT += lane_xor_u32<8>(sT, addr32);
-    sC = fadd(sC, lane_xor_u64<4>(sC, addr32));
-    sT += lane_xor_u32<4>(sT, addr32);
-    sC = fadd(sC, lane_xor_u64<2>(sC, addr32));
-    sT += lane_xor_u32<2>(sT, addr32);
-    sC = fadd(sC, lane_xor_u64<1>(sC, addr32));
-    sT += lane_xor_u32<1>(sT, addr32);
-    if (lane == 0) {
-      red[slot][wave][0] = sC;
-      red[slot][wave][1] = sT;
-    }
+    // this thread's sum of the 8 C mod P: four l at a time stay below 2^64,
+    // and sH <= 24
+    static_assert(LPT == 8, "the sum below covers 8 leaves");
+    const uint64_t sC = fadd(fadd(freduce64((L[0] + L[1]) + (L[2] + L[3])),
+                                  freduce64((L[4] + L[5]) + (L[6] + L[7]))),
+                             (uint64_t)sH * kC62);
+    atomicAdd(&red[slot][0], (unsigned long long)(uint32_t)sC);
+    atomicAdd(&red[slot][1], (unsigned long long)(sC >> 32));
+    atomicAdd(&red_t[slot], sT);
     if (threadIdx.x == 0)
       red_at[slot] = ((uint64_t)T << nparts_log) +
                      (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + 3));
@@ -645,7 +657,7 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
 #pragma unroll
   for (int e = 0; e < LPT; ++e) cur[e] = 0;
   for (int wi = widx + 1; wi < a.mask_words; ++wi) finish_word(wi);
-  reduce_A();
+  fold_A();
   if (phase == 1) {  // gA holds the +C sums
 #pragma unroll
     for (int e = 0; e < LPT; e += 2) {
@@ -658,15 +670,11 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   }
   __syncthreads();
   for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
-    uint64_t c = red[sl][0][0], tt = red[sl][0][1];
-#pragma unroll
-    for (int w = 1; w < 4; ++w) {
-      c = fadd(c, red[sl][w][0]);
-      tt += red[sl][w][1];
-    }
+    // low + 2^32 high, mod P (2^32 < P; both sums below 2^62 after freduce64)
+    const uint64_t c = fadd(freduce64(red[sl][0]), fmul(freduce64(red[sl][1]), 1ULL << 32));
     uint64_t* dst = a.part + red_at[sl] * 2;
     dst[0] = c;
-    dst[1] = tt;
+    dst[1] = red_t[sl];
   }
 }
 
