@@ -253,6 +253,167 @@ __device__ __forceinline__ void load_cw_uniform(const LevelArgs& a, int T, int l
   CW[NK] = fold;
 }
 
+// The output layer's state in the leaf kernels, per thread: sums of +-C over
+// the landing slots for its LPT output coefficients, their tau words, and the
+// per-slot workgroup sums for the Beaver step.
+//  * Sums: a leaf's out-hash word x = 2^62 h + l (h < 4) goes in raw, l into
+//    A and h into Ah, so that A + Ah * (2^62 mod P) = sum of x mod P.  Three
+//    l (each below 2^62) on an A below P stay below 2^64, so A is reduced
+//    every third slot; Ah (at most 3 per slot) is folded in at the phase
+//    switch and at the end.  The slots with h = 0 (+C) all come before those
+//    with h = 1 (-C): at the switch the +C sums go to gA, and at the end
+//    gA = (+C sums) - (-C sums).
+//  * tau words: cur holds the current half (32 slots) of the current 64-slot
+//    word; the low half goes out when the slots reach the high half, the rest
+//    when the word is done (slots come in increasing order, one at a time).
+//  * Slot sums: each thread's sum of its LPT C mod P and its tau count go
+//    into LDS by atomics: the low and high 32 bits of the C sums separately
+//    (below 2^40 and 2^38 over 256 threads), so the adds are exact.
+struct SlotSums {
+  unsigned long long c[kGatherMaxSlots][2];
+  uint32_t t[kGatherMaxSlots];
+  uint64_t at[kGatherMaxSlots];  // the partial each slot's sums go to
+};
+
+template <int LPT>
+struct LeafOut {
+  uint64_t A[LPT];
+  uint32_t Ah[LPT], cur[LPT];
+  int half = 0, nadd = 0, phase = 0, widx = 0;
+  Blk* gdst;          // this thread's LPT sums in gA
+  uint64_t* mask0;    // this thread's LPT words of tau word 0
+  size_t word_stride;
+
+  __device__ __forceinline__ LeafOut(Blk* g, uint64_t* m0, size_t ws)
+      : gdst(g), mask0(m0), word_stride(ws) {
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) {
+      A[e] = 0;
+      Ah[e] = cur[e] = 0;
+    }
+  }
+  __device__ __forceinline__ void reduce() {
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
+    nadd = 0;
+  }
+  __device__ __forceinline__ void fold() {  // Ah * kC62 < 2^38 < P
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) {
+      A[e] = fadd(freduce64(A[e]), (uint64_t)Ah[e] * kC62);
+      Ah[e] = 0;
+    }
+    nadd = 0;
+  }
+  __device__ __forceinline__ void store_half(int wi, int hf) {  // cur as half hf of word wi
+    uint32_t* dst = reinterpret_cast<uint32_t*>(mask0 + (size_t)wi * word_stride) + hf;
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) dst[2 * e] = cur[e];
+  }
+  __device__ __forceinline__ void finish_word(int wi) {  // zeros for an untouched high half
+    if (half) {
+      store_half(wi, 1);
+    } else {
+      uint64_t* dst = mask0 + (size_t)wi * word_stride;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) dst[e] = cur[e];
+    }
+  }
+  // Before slot `slot` (sign h): word and half transitions, the phase switch,
+  // the periodic reduction.
+  __device__ __forceinline__ void begin_slot(int slot, int h) {
+    if ((slot >> 6) != widx) {
+      finish_word(widx);
+      widx = slot >> 6;
+      half = 0;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) cur[e] = 0;
+    }
+    if (!half && (slot & 32)) {
+      store_half(widx, 0);
+      half = 1;
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) cur[e] = 0;
+    }
+    if (h != phase) {
+      fold();
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+#pragma unroll
+      for (int e = 0; e < LPT; ++e) A[e] = 0;
+      phase = h;
+    } else if (nadd == 3) {
+      reduce();
+    }
+    ++nadd;
+  }
+  // Leaf e of the slot: out-hash word x, tau bit; l goes to *L.
+  __device__ __forceinline__ void leaf(int e, uint64_t x, uint32_t tau, int slot, uint64_t* L,
+                                       uint32_t& sH, uint32_t& sT) {
+    *L = x & ((1ULL << 62) - 1);
+    const uint32_t hx = (uint32_t)(x >> 62);
+    A[e] += *L;
+    Ah[e] += hx;
+    sH += hx;
+    sT += tau;
+    cur[e] |= tau << (slot & 31);
+  }
+  // After the slot's leaves: this thread's C sum into the slot's LDS sums.
+  __device__ __forceinline__ void end_slot(SlotSums& r, int slot, const uint64_t (&L)[LPT],
+                                           uint32_t sH, uint32_t sT) {
+    uint64_t sC = (uint64_t)sH * kC62;  // sH <= 3 * LPT
+#pragma unroll
+    for (int g = 0; g < LPT; g += 4) {  // four l at a time stay below 2^64
+      uint64_t s4 = L[g];
+#pragma unroll
+      for (int j = 1; j < 4 && g + j < LPT; ++j) s4 += L[g + j];
+      sC = fadd(sC, freduce64(s4));
+    }
+    atomicAdd(&r.c[slot][0], (unsigned long long)(uint32_t)sC);
+    atomicAdd(&r.c[slot][1], (unsigned long long)(sC >> 32));
+    atomicAdd(&r.t[slot], sT);
+  }
+  // After the last slot: the remaining tau words and the sums.
+  __device__ __forceinline__ void finish(int mask_words) {
+    finish_word(widx);
+    half = 0;
+#pragma unroll
+    for (int e = 0; e < LPT; ++e) cur[e] = 0;
+    for (int wi = widx + 1; wi < mask_words; ++wi) finish_word(wi);
+    fold();
+    if (phase == 1) {  // gA holds the +C sums
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) {
+        const Blk p = gdst[e / 2];
+        gdst[e / 2] = Blk{fsub(p.lo, A[e]), fsub(p.hi, A[e + 1])};
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
+    }
+  }
+};
+
+__device__ __forceinline__ void slot_sums_clear(SlotSums& r) {
+  for (int i = threadIdx.x; i < kGatherMaxSlots; i += kTpb) {
+    r.c[i][0] = r.c[i][1] = 0;
+    r.t[i] = 0;
+  }
+  __syncthreads();
+}
+
+// After every thread's last slot (and a barrier): one partial per slot,
+// (sum of C mod P, tau count), to a.part.
+__device__ __forceinline__ void slot_sums_flush(const SlotSums& r, int nslots, uint64_t* part) {
+  for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
+    // low + 2^32 high, mod P (2^32 < P; both below 2^62 after freduce64)
+    const uint64_t c = fadd(freduce64(r.c[sl][0]), fmul(freduce64(r.c[sl][1]), 1ULL << 32));
+    uint64_t* dst = part + r.at[sl] * 2;
+    dst[0] = c;
+    dst[1] = r.t[sl];
+  }
+}
+
 // Leaf level with the output layer's gather fused in.  A workgroup owns
 // 256 * LPT consecutive output coefficients q of one (party, block) set, LPT
 // per thread, all in one bin, and walks the leaves that land on them
@@ -272,10 +433,9 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   constexpr int TILE = kTpb * LPT;
   constexpr int CPP = LPT >= M ? M : LPT;  // the thread's leaves per parent
   constexpr int PPT = LPT / CPP;           // the thread's parents per slot
-  __shared__ uint64_t red[kGatherMaxSlots][4][2];
-  __shared__ uint64_t red_at[kGatherMaxSlots];
+  __shared__ SlotSums red;
+  slot_sums_clear(red);
   const int bb = blockIdx.y, party = blockIdx.z;  // party: index within this launch
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t qt = blockIdx.x * TILE;
   const uint32_t q0 = qt + LPT * threadIdx.x;
   const size_t bin_mask = ((size_t)1 << a.bin_log) - 1;
@@ -284,53 +444,19 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
   const int nparts_log = a.S_log + W - (kTpbLog + ilog2_constexpr(LPT));
   const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
   const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
-  // A: raw sums of C, reduced every third slot (three values below P added to
-  // one below P stay below 2^64).  The slots with h = 0 (+C) all come before
-  // those with h = 1 (-C): at the switch the +C sums go to gA, and at the end
-  // gA = (+C sums) - (-C sums).
-  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
-                                     ((size_t)bb << a.N_log) + q0);
-  uint64_t A[LPT], mw[LPT];
-#pragma unroll
-  for (int e = 0; e < LPT; ++e) A[e] = mw[e] = 0;
-  int nadd = 0, phase = 0;
-  auto reduce_A = [&]() {
-#pragma unroll
-    for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
-    nadd = 0;
-  };
-  int widx = 0;
-  auto store_words = [&](int wi) {
-    Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
-#pragma unroll
-    for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{mw[e], mw[e + 1]};
-  };
+  LeafOut<LPT> out(reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
+                                          ((size_t)bb << a.N_log) + q0),
+                   a.gmask + set_off + q0, word_stride);
   const int nslots = for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h,
                                                                              int ds, int inst) {
-    if ((slot >> 6) != widx) {
-      store_words(widx);
-      widx = slot >> 6;
-#pragma unroll
-      for (int e = 0; e < LPT; ++e) mw[e] = 0;
-    }
-    if (h != phase) {
-      reduce_A();
-#pragma unroll
-      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
-#pragma unroll
-      for (int e = 0; e < LPT; ++e) A[e] = 0;
-      phase = h;
-    } else if (nadd == 3) {
-      reduce_A();
-    }
-    ++nadd;
+    out.begin_slot(slot, h);
     const int T = (party * a.sets + bb) * a.B + inst;
     Blk CW[NK + 1], CWg[NK + 1];
     load_cw_uniform<W>(a, T, a.level - 1, CW);
     load_cw_uniform<W>(a, T, a.level - 2, CWg);
     const size_t d0 = rr0 + ((size_t)ds << a.bin_log);  // a multiple of LPT
-    uint64_t sC = 0;
-    uint32_t sT = 0;
+    uint64_t L[LPT];
+    uint32_t sH = 0, sT = 0;
 #pragma unroll
     for (int pp = 0; pp < PPT; ++pp) {
       const size_t dp = d0 + (size_t)pp * CPP;
@@ -381,55 +507,18 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
           }
         }
         if (tp) leaf = bxor(leaf, cwc);
-        const uint64_t C = freduce64(out_hash_lo64(leaf));
-        const uint32_t tau = (uint32_t)(leaf.lo & 1ULL);
         const int q = pp * CPP + e;
-        A[q] += C;
-        mw[q] |= (uint64_t)tau << (slot & 63);
-        sC = fadd(sC, C);
-        sT += tau;
+        out.leaf(q, out_hash_lo64(leaf), (uint32_t)leaf.lo & 1u, slot, &L[q], sH, sT);
       }
     }
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) {
-      sC = fadd(sC, shfl_xor_u64(sC, m));
-      sT += (uint32_t)__shfl_xor((int)sT, m, 64);
-    }
-    if (lane == 0) {
-      red[slot][wave][0] = sC;
-      red[slot][wave][1] = sT;
-    }
+    out.end_slot(red, slot, L, sH, sT);
     if (threadIdx.x == 0)
-      red_at[slot] = ((uint64_t)T << nparts_log) +
+      red.at[slot] = ((uint64_t)T << nparts_log) +
                      (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + ilog2_constexpr(LPT)));
   });
-  store_words(widx);
-#pragma unroll
-  for (int e = 0; e < LPT; ++e) mw[e] = 0;
-  for (int wi = widx + 1; wi < a.mask_words; ++wi) store_words(wi);
-  reduce_A();
-  if (phase == 1) {  // gA holds the +C sums
-#pragma unroll
-    for (int e = 0; e < LPT; e += 2) {
-      const Blk p = gdst[e / 2];
-      gdst[e / 2] = Blk{fsub(p.lo, A[e]), fsub(p.hi, A[e + 1])};
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
-  }
+  out.finish(a.mask_words);
   __syncthreads();
-  for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
-    uint64_t c = red[sl][0][0], tt = red[sl][0][1];
-#pragma unroll
-    for (int w = 1; w < 4; ++w) {
-      c = fadd(c, red[sl][w][0]);
-      tt += red[sl][w][1];
-    }
-    uint64_t* dst = a.part + red_at[sl] * 2;
-    dst[0] = c;
-    dst[1] = tt;
-  }
+  slot_sums_flush(red, nslots, a.part);
 }
 
 // ---- both parties in one run: the last tree levels fused into the leaf level --
@@ -447,7 +536,10 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
 
 __device__ __forceinline__ Blk child1(Blk parent, Blk h, int c, Blk cw) {
   Blk node = c ? bxor(h, parent) : h;   // w = 1: child 0 = h, child 1 = h ^ parent
-  if (parent.lo & 1ULL) node = bxor(node, cw);
+  // the parent's t bit as one 32-bit mask for the four words of cw
+  const uint32_t m = 0u - ((uint32_t)parent.lo & 1u);
+  node.lo ^= ((uint64_t)((uint32_t)(cw.lo >> 32) & m) << 32) | ((uint32_t)cw.lo & m);
+  node.hi ^= ((uint64_t)((uint32_t)(cw.hi >> 32) & m) << 32) | ((uint32_t)cw.hi & m);
   return node;
 }
 
@@ -506,17 +598,8 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
 __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   constexpr int LPT = 8;
   constexpr int TILE = kTpb * LPT;
-  // per-slot workgroup sums, by LDS atomics: the low and the high 32 bits of
-  // the threads' C sums (each below 2^40 and 2^38 over 256 threads) and the
-  // tau count
-  __shared__ unsigned long long red[kGatherMaxSlots][2];
-  __shared__ uint32_t red_t[kGatherMaxSlots];
-  __shared__ uint64_t red_at[kGatherMaxSlots];
-  for (int i = threadIdx.x; i < kGatherMaxSlots; i += kTpb) {
-    red[i][0] = red[i][1] = 0;
-    red_t[i] = 0;
-  }
-  __syncthreads();
+  __shared__ SlotSums red;
+  slot_sums_clear(red);
   const int bb = blockIdx.y, party = blockIdx.z;
   const uint32_t qt = blockIdx.x * TILE;
   const uint32_t q0 = qt + LPT * threadIdx.x;
@@ -528,80 +611,12 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   const int nparts_log = levels - (kTpbLog + 3);
   const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
   const size_t word_stride = ((size_t)a.parties * a.sets) << a.N_log;
-  Blk* gdst = reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
-                                     ((size_t)bb << a.N_log) + q0);
-  // Sums per coefficient: a leaf's out-hash word x = 2^62 h + l (h < 4) goes
-  // in raw, l into A and h into Ah, so that A + Ah * (2^62 mod P) = sum of x
-  // mod P.  Three l (each below 2^62) on an A below P stay below 2^64: A is
-  // reduced every third slot.  Ah (at most 3 per slot, 256 slots) is folded
-  // in at the phase switch and at the end.
-  // tau bits: cur holds the current half (32 slots) of the current 64-slot
-  // word; the low half goes out when the slots reach the high half, the rest
-  // when the word is done (slots come in increasing order).
-  uint64_t A[LPT];
-  uint32_t Ah[LPT], cur[LPT];
-#pragma unroll
-  for (int e = 0; e < LPT; ++e) {
-    A[e] = 0;
-    Ah[e] = cur[e] = 0;
-  }
-  int half = 0;
-  int nadd = 0, phase = 0;
-  auto reduce_A = [&]() {
-#pragma unroll
-    for (int e = 0; e < LPT; ++e) A[e] = freduce64(A[e]);
-    nadd = 0;
-  };
-  auto fold_A = [&]() {  // A mod P with the high parts: Ah * kC62 < 2^38 < P
-#pragma unroll
-    for (int e = 0; e < LPT; ++e) {
-      A[e] = fadd(freduce64(A[e]), (uint64_t)Ah[e] * kC62);
-      Ah[e] = 0;
-    }
-    nadd = 0;
-  };
-  int widx = 0;
-  auto store_half = [&](int wi, int hf) {  // cur as half hf of word wi
-    uint32_t* dst =
-        reinterpret_cast<uint32_t*>(a.gmask + (size_t)wi * word_stride + set_off + q0) + hf;
-#pragma unroll
-    for (int e = 0; e < LPT; ++e) dst[2 * e] = cur[e];
-  };
-  auto finish_word = [&](int wi) {  // the word's last bits; zeros for an untouched high half
-    if (half) {
-      store_half(wi, 1);
-    } else {
-      Blk* dst = reinterpret_cast<Blk*>(a.gmask + (size_t)wi * word_stride + set_off + q0);
-#pragma unroll
-      for (int e = 0; e < LPT; e += 2) dst[e / 2] = Blk{cur[e], cur[e + 1]};
-    }
-  };
+  LeafOut<LPT> out(reinterpret_cast<Blk*>(a.gA + (size_t)party * a.gA_party_stride +
+                                          ((size_t)bb << a.N_log) + q0),
+                   a.gmask + set_off + q0, word_stride);
   const int nslots = for_each_landing(qt, a.N_log, a.bin_log, a.t, a.B, [&](int slot, int h,
                                                                              int ds, int inst) {
-    if ((slot >> 6) != widx) {
-      finish_word(widx);
-      widx = slot >> 6;
-      half = 0;
-#pragma unroll
-      for (int e = 0; e < LPT; ++e) cur[e] = 0;
-    }
-    if (!half && (slot & 32)) {
-      store_half(widx, 0);
-      half = 1;
-#pragma unroll
-      for (int e = 0; e < LPT; ++e) cur[e] = 0;
-    }
-    if (h != phase) {
-      fold_A();
-#pragma unroll
-      for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
-#pragma unroll
-      for (int e = 0; e < LPT; ++e) A[e] = 0;
-      phase = h;
-    } else if (nadd == 3) {
-      reduce_A();
-    }
-    ++nadd;
+    out.begin_slot(slot, h);
     const int T = (party * a.sets + bb) * a.B + inst;
     Blk cw[4][2];   // levels Ltop .. levels - 1 (w = 1: CW_0 and the fold are equal)
 #pragma unroll
@@ -624,58 +639,20 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
       tree_hash<kPrgChaCha8, 1>(n16[i], &h16[i], nullptr);
     }
     uint64_t L[LPT];
-    uint32_t taus[LPT], sT = 0, sH = 0;
+    uint32_t sH = 0, sT = 0;
 #pragma unroll
     for (int e = 0; e < LPT; ++e) {
       const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[3][0]);
-      const uint64_t x = out_hash_lo64(leaf);
-      L[e] = x & ((1ULL << 62) - 1);
-      const uint32_t hx = (uint32_t)(x >> 62);
-      taus[e] = (uint32_t)leaf.lo & 1u;
-      A[e] += L[e];
-      Ah[e] += hx;
-      sH += hx;
-      sT += taus[e];
+      out.leaf(e, out_hash_lo64(leaf), (uint32_t)leaf.lo & 1u, slot, &L[e], sH, sT);
     }
-#pragma unroll
-    for (int e = 0; e < LPT; ++e) cur[e] |= taus[e] << (slot & 31);
-    // this thread's sum of the 8 C mod P: four l at a time stay below 2^64,
-    // and sH <= 24
-    static_assert(LPT == 8, "the sum below covers 8 leaves");
-    const uint64_t sC = fadd(fadd(freduce64((L[0] + L[1]) + (L[2] + L[3])),
-                                  freduce64((L[4] + L[5]) + (L[6] + L[7]))),
-                             (uint64_t)sH * kC62);
-    atomicAdd(&red[slot][0], (unsigned long long)(uint32_t)sC);
-    atomicAdd(&red[slot][1], (unsigned long long)(sC >> 32));
-    atomicAdd(&red_t[slot], sT);
+    out.end_slot(red, slot, L, sH, sT);
     if (threadIdx.x == 0)
-      red_at[slot] = ((uint64_t)T << nparts_log) +
+      red.at[slot] = ((uint64_t)T << nparts_log) +
                      (((qt & bin_mask) + ((size_t)ds << a.bin_log)) >> (kTpbLog + 3));
   });
-  finish_word(widx);
-  half = 0;
-#pragma unroll
-  for (int e = 0; e < LPT; ++e) cur[e] = 0;
-  for (int wi = widx + 1; wi < a.mask_words; ++wi) finish_word(wi);
-  fold_A();
-  if (phase == 1) {  // gA holds the +C sums
-#pragma unroll
-    for (int e = 0; e < LPT; e += 2) {
-      const Blk p = gdst[e / 2];
-      gdst[e / 2] = Blk{fsub(p.lo, A[e]), fsub(p.hi, A[e + 1])};
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < LPT; e += 2) gdst[e / 2] = Blk{A[e], A[e + 1]};
-  }
+  out.finish(a.mask_words);
   __syncthreads();
-  for (int sl = threadIdx.x; sl < nslots; sl += kTpb) {
-    // low + 2^32 high, mod P (2^32 < P; both sums below 2^62 after freduce64)
-    const uint64_t c = fadd(freduce64(red[sl][0]), fmul(freduce64(red[sl][1]), 1ULL << 32));
-    uint64_t* dst = a.part + red_at[sl] * 2;
-    dst[0] = c;
-    dst[1] = red_t[sl];
-  }
+  slot_sums_flush(red, nslots, a.part);
 }
 
 // Level 0: the frontier is the root.  Also moves DltSft from the reference
@@ -729,9 +706,10 @@ void launch_root(hipStream_t st, const DpfRun& r, Blk* F0, Blk* H0, Blk* cw) {
 template <int W, int PRG, int MODE, bool REBUILD = false>
 void launch_level(hipStream_t st, LevelArgs a, Profiler* prof) {
   char zone[32];
-  snprintf(zone, sizeof zone, MODE == kModeHash ? "dpf.level.%02d"
-                              : MODE == kModeStore ? "dpf.leaf_store" : "dpf.leaf_convert",
-           a.level);
+  if (MODE == kModeHash)
+    snprintf(zone, sizeof zone, "dpf.level.%02d", a.level);
+  else
+    snprintf(zone, sizeof zone, "%s", MODE == kModeStore ? "dpf.leaf_store" : "dpf.leaf_convert");
   ProfZone pz(prof, st, zone);
   const size_t total = (size_t)a.ntrees << a.S_log;
   const size_t tiles = (total + kTpb - 1) / kTpb;
