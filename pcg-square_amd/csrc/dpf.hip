@@ -531,8 +531,10 @@ __global__ __launch_bounds__(kTpb) void leaf_gather_kernel(LevelArgs a) {
 // protocol reduces to (dpf.cpp:520-551).  alpha's digit at level L is where
 // DltSft0_L ^ DltSft1_L is nonzero ([digit_L(alpha) = k] Delta, Delta != 0).
 // The leaf level then expands those three levels and the leaves in registers
-// from the stored nodes of level levels - 4 (deep_leaf_kernel), and no wider
-// level is ever stored.  w = 1, ChaCha8 tree PRG.
+// from the stored hashes of level levels - 4 (deep_leaf_kernel), and no wider
+// level is ever stored.  Level levels - 4 stores only its hashes: its nodes
+// are rebuilt from level levels - 5's nodes and hashes by XORs.  w = 1,
+// ChaCha8 tree PRG.
 
 __device__ __forceinline__ Blk child1(Blk parent, Blk h, int c, Blk cw) {
   Blk node = c ? bxor(h, parent) : h;   // w = 1: child 0 = h, child 1 = h ^ parent
@@ -546,8 +548,10 @@ __device__ __forceinline__ Blk child1(Blk parent, Blk h, int c, Blk cw) {
 // One thread per instance (trees Ti and Ti + half).  Levels L0 - 1 and below
 // are complete in cw; levels L0 .. levels - 1 still hold the seeded DltSft
 // shares and receive CW_L in the first tree's slot (0 in the second's, so
-// that mine ^ peer is CW_L for both).
-__global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict__ F,
+// that mine ^ peer is CW_L for both).  Level L0 - 1 stored only its hashes
+// (H); its path node is rebuilt from level L0 - 2's node and hashes (Fg, Hg).
+__global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict__ Fg,
+                                                           const Blk* __restrict__ Hg,
                                                            const Blk* __restrict__ H,
                                                            Blk* __restrict__ cw,
                                                            const Blk* __restrict__ dlt0,
@@ -564,10 +568,14 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
   for (int L = 0; L < L0 - 1; ++L) j = (j << 1) | (size_t)digit(L);
   const size_t S = (size_t)1 << (L0 - 1);
   const size_t i0 = (size_t)Ti * S + j, i1 = ((size_t)Ti + half) * S + j;
-  Blk n0 = F[i0], n1 = F[i1];
+  const size_t g0 = i0 >> 1, g1 = i1 >> 1;  // their parents at level L0 - 2 (S/2 per tree)
+  const Blk* cg = &cw[(size_t)(L0 - 2) * ntrees + Ti];
+  Blk cwv = bxor(cg[0], cg[half]);
+  Blk n0 = child1(Fg[g0], Hg[g0], (int)(j & 1), cwv);
+  Blk n1 = child1(Fg[g1], Hg[g1], (int)(j & 1), cwv);
   Blk h0 = H[i0], h1 = H[i1];
   const Blk* c0 = &cw[(size_t)(L0 - 1) * ntrees + Ti];
-  Blk cwv = bxor(c0[0], c0[half]);
+  cwv = bxor(c0[0], c0[half]);
   int c = digit(L0 - 1);
   n0 = child1(n0, h0, c, cwv);
   n1 = child1(n1, h1, c, cwv);
@@ -589,9 +597,9 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
 // The leaf level of both parties' run with the last three tree levels fused
 // in (w = 1, 8 output coefficients per thread).  Per landing slot a thread's
 // 8 consecutive leaves d0 .. d0 + 7 descend from one node of level
-// levels - 3; the thread reads that node's parent (level levels - 4, stored
-// with its hash), forms the node, and expands 1 + 2 + 4 tree hashes and 8
-// out-hashes in registers.  Every (tree, node) is expanded by exactly one
+// levels - 3; the thread rebuilds that node's parent (level levels - 4) from
+// its stored hash and its own parent's node and hashes, forms the node, and
+// expands 1 + 2 + 4 tree hashes and 8 out-hashes in registers.  Every (tree, node) is expanded by exactly one
 // thread, so the work is the same as hashing the levels one by one, without
 // their frontiers.  The output layer's sums, tau bits and Beaver partials are
 // those of leaf_gather_kernel.
@@ -606,7 +614,7 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   const size_t bin_mask = ((size_t)1 << a.bin_log) - 1;
   const size_t rr0 = q0 & bin_mask;  // a multiple of LPT
   const int levels = a.level;        // leaves are level `levels`
-  const int Ltop = levels - 4;       // stored level
+  const int Ltop = levels - 4;       // the last stored level (its hashes; nodes one level up)
   const size_t Stop = (size_t)1 << Ltop;
   const int nparts_log = levels - (kTpbLog + 3);
   const size_t set_off = ((size_t)party * a.sets + bb) << a.N_log;
@@ -618,31 +626,32 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
                                                                              int ds, int inst) {
     out.begin_slot(slot, h);
     const int T = (party * a.sets + bb) * a.B + inst;
-    Blk cw[4][2];   // levels Ltop .. levels - 1 (w = 1: CW_0 and the fold are equal)
+    Blk cw[5][2];   // levels Ltop - 1 .. levels - 1 (w = 1: CW_0 and the fold are equal)
 #pragma unroll
-    for (int l = 0; l < 4; ++l) load_cw_uniform<1>(a, T, Ltop + l, cw[l]);
+    for (int l = 0; l < 5; ++l) load_cw_uniform<1>(a, T, Ltop - 1 + l, cw[l]);
     const size_t d0 = rr0 + ((size_t)ds << a.bin_log);  // a multiple of 8
-    const size_t gi = (size_t)T * Stop + (d0 >> 4);
-    const Blk n14 = child1(a.Fp[gi], a.Hp[gi], (int)((d0 >> 3) & 1), cw[0][0]);
+    const size_t gi = (size_t)T * Stop + (d0 >> 4);      // level Ltop
+    const Blk n13 = child1(a.Fg[gi >> 1], a.Hg[gi >> 1], (int)(gi & 1), cw[0][0]);
+    const Blk n14 = child1(n13, a.Hp[gi], (int)((d0 >> 3) & 1), cw[1][0]);
     Blk h14;
     tree_hash<kPrgChaCha8, 1>(n14, &h14, nullptr);
     Blk n15[2], h15[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      n15[c] = child1(n14, h14, c, cw[1][0]);
+      n15[c] = child1(n14, h14, c, cw[2][0]);
       tree_hash<kPrgChaCha8, 1>(n15[c], &h15[c], nullptr);
     }
     Blk n16[4], h16[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      n16[i] = child1(n15[i >> 1], h15[i >> 1], i & 1, cw[2][0]);
+      n16[i] = child1(n15[i >> 1], h15[i >> 1], i & 1, cw[3][0]);
       tree_hash<kPrgChaCha8, 1>(n16[i], &h16[i], nullptr);
     }
     uint64_t L[LPT];
     uint32_t sH = 0, sT = 0;
 #pragma unroll
     for (int e = 0; e < LPT; ++e) {
-      const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[3][0]);
+      const Blk leaf = child1(n16[e >> 1], h16[e >> 1], e & 1, cw[4][0]);
       out.leaf(e, out_hash_lo64(leaf), (uint32_t)leaf.lo & 1u, slot, &L[e], sH, sT);
     }
     out.end_slot(red, slot, L, sH, sT);
@@ -805,17 +814,22 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     // the last tree level keeps only its hashes; the leaf level rebuilds
     // those parents from the grandparents (still in the other buffer)
     if (deep && i == levels - 3) {
-      // the parents' level (levels - 4) is the last one stored: path CWs of
-      // the three levels below it, then the fused leaf level
+      // the parents' level (levels - 4) is the last one stored, hashes only:
+      // path CWs of the three levels below it, then the fused leaf level
       const int half = r.ntrees / 2;
       const Blk* dlt1 = r.dltsft1 ? r.dltsft1 : r.dltsft + (size_t)half * levels;
+      Blk *Fg, *Hg;  // level levels - 5: nodes and hashes (the parents' parents)
+      frontier(cur ^ 1, S >> W, Fg, Hg);
       {
         ProfZone pz(r.prof, st, "dpf.path_cw");
         hipLaunchKernelGGL(dpf_path_cw_kernel, dim3((unsigned)((half + kTpb - 1) / kTpb)),
-                           dim3(kTpb), 0, st, F, H, s.cw.as<Blk>(), r.dltsft, dlt1, half, levels,
-                           levels - 3);
+                           dim3(kTpb), 0, st, Fg, Hg, H, s.cw.as<Blk>(), r.dltsft, dlt1, half,
+                           levels, levels - 3);
         hip_check(hipGetLastError(), "launch dpf_path_cw_kernel");
       }
+      a.Fp = nullptr;
+      a.Fg = Fg;
+      a.Hg = Hg;
       a.part = r.partials;
       a.gA = r.g_acc;
       a.gA_party_stride = r.g_party_stride;
@@ -846,7 +860,9 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     if (i < levels) {
       Blk *Fn, *Hn;
       frontier(cur ^ 1, S * M, Fn, Hn);
-      a.Fc = i == levels - 1 ? nullptr : Fn;
+      // the last tree level, and in the deep run the last stored level, keep
+      // only their hashes (the next level rebuilds their nodes by XORs)
+      a.Fc = (i == levels - 1 || (deep && i == levels - 4)) ? nullptr : Fn;
       a.Hc = Hn;
       launch_level<W, PRG, kModeHash>(st, a, r.prof);
       cur ^= 1;
