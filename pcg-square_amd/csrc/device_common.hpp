@@ -213,36 +213,6 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
-// Lane exchange with a compile-time XOR mask and no per-call address
-// arithmetic: masks below 32 through ds_swizzle's bit mode (source lane =
-// lane ^ M inside each half-wave), 32 through ds_bpermute with the caller's
-// precomputed byte address (lane ^ 32) * 4.
-template <int M>
-__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t v, int addr32) {
-  static_assert(M >= 1 && M <= 32 && (M & (M - 1)) == 0, "lane_xor_u32: mask is a power of 2");
-  if constexpr (M == 32)
-    return (uint32_t)__builtin_amdgcn_ds_bpermute(addr32, (int)v);
-  else
-    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
-}
-template <int M>
-__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v, int addr32) {
-  const uint32_t lo = lane_xor_u32<M>((uint32_t)v, addr32);
-  const uint32_t hi = lane_xor_u32<M>((uint32_t)(v >> 32), addr32);
-  return (uint64_t)lo | ((uint64_t)hi << 32);
-}
-
-// XOR-reduce across aligned groups of `width` lanes (power of 2, <= 64).
-__device__ __forceinline__ uint64_t wave_xor(uint64_t v, int width) {
-  for (int m = width >> 1; m > 0; m >>= 1) v ^= shfl_xor_u64(v, m);
-  return v;
-}
-
-__device__ __forceinline__ uint64_t wave_fadd(uint64_t v, int width) {
-  for (int m = width >> 1; m > 0; m >>= 1) v = fadd(v, shfl_xor_u64(v, m));
-  return v;
-}
-
 // Workgroups of 256 threads.  Reduce per-thread values over "segments": a
 // segment is the set of threads of one workgroup that belong to one tree (S = 2^S_log parents per tree;
 // trees are aligned to S threads, workgroups to 256).  Word i combines with
