@@ -71,6 +71,9 @@ struct LevelArgs {
   size_t gA_party_stride;
   uint64_t* gmask;
   int mask_words, t, B, N_log, bin_log, sets, parties;
+  // deep run: the subtree kernel's outputs are stored subtree-major (see
+  // dpf_subtree_kernel); sub_n = nodes of level Ls + 1 (all trees)
+  size_t sub_n;
 };
 
 enum Mode : int { kModeHash = 0, kModeStore = 1, kModeConvert = 2 };
@@ -548,10 +551,9 @@ __device__ __forceinline__ Blk child1(Blk parent, Blk h, int c, Blk cw) {
 // One thread per instance (trees Ti and Ti + half).  Levels L0 - 1 and below
 // are complete in cw; levels L0 .. levels - 1 still hold the seeded DltSft
 // shares and receive CW_L in the first tree's slot (0 in the second's, so
-// that mine ^ peer is CW_L for both).  Level L0 - 1 stored only its hashes
-// (H); its path node is rebuilt from level L0 - 2's node and hashes (Fg, Hg).
-__global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict__ Fg,
-                                                           const Blk* __restrict__ Hg,
+// that mine ^ peer is CW_L for both).  Level L0 - 1 is stored in full (nodes
+// F, hashes H).
+__global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict__ F,
                                                            const Blk* __restrict__ H,
                                                            Blk* __restrict__ cw,
                                                            const Blk* __restrict__ dlt0,
@@ -568,14 +570,10 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
   for (int L = 0; L < L0 - 1; ++L) j = (j << 1) | (size_t)digit(L);
   const size_t S = (size_t)1 << (L0 - 1);
   const size_t i0 = (size_t)Ti * S + j, i1 = ((size_t)Ti + half) * S + j;
-  const size_t g0 = i0 >> 1, g1 = i1 >> 1;  // their parents at level L0 - 2 (S/2 per tree)
-  const Blk* cg = &cw[(size_t)(L0 - 2) * ntrees + Ti];
-  Blk cwv = bxor(cg[0], cg[half]);
-  Blk n0 = child1(Fg[g0], Hg[g0], (int)(j & 1), cwv);
-  Blk n1 = child1(Fg[g1], Hg[g1], (int)(j & 1), cwv);
+  Blk n0 = F[i0], n1 = F[i1];
   Blk h0 = H[i0], h1 = H[i1];
   const Blk* c0 = &cw[(size_t)(L0 - 1) * ntrees + Ti];
-  cwv = bxor(c0[0], c0[half]);
+  Blk cwv = bxor(c0[0], c0[half]);
   int c = digit(L0 - 1);
   n0 = child1(n0, h0, c, cwv);
   n1 = child1(n1, h1, c, cwv);
@@ -594,14 +592,75 @@ __global__ __launch_bounds__(kTpb) void dpf_path_cw_kernel(const Blk* __restrict
   }
 }
 
+// Tree levels Ls + 1 .. Ls + 5 of both parties' run in one launch, their CWs
+// known from the alpha path (no level sums).  A thread owns one node of level
+// Ls + 1: it forms it from its stored parent (level Ls: a.Fp, a.Hp), then
+// hashes its subtree depth-first in registers, 1 + 2 + 4 + 8 + 16 ChaCha8
+// blocks, and stores level Ls + 4 in full (a.Fc, a.Hc) and the hashes of
+// level Ls + 5 (a.Hg).  The stores are subtree-major, so that a wave's lanes
+// write consecutive words: node s of the subtree of level-(Ls+1) node f
+// (global index f * 2^d + s at depth d) sits at s * sub_n + f.  a.S_log =
+// Ls.  UNIFORM: a workgroup's nodes lie in one tree (2^(Ls+1) >= 256), so
+// the CWs are scalar loads.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kTpb) void dpf_subtree_kernel(LevelArgs a) {
+  const size_t f = (size_t)blockIdx.x * kTpb + threadIdx.x;  // level Ls + 1, global index
+  const int L1 = a.S_log + 1;
+  if (!UNIFORM && f >= ((size_t)a.ntrees << L1)) return;
+  const int T = (int)(f >> L1);
+  Blk cw[5][2];  // levels Ls .. Ls + 4 (w = 1: CW_0 and the fold are equal)
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    if constexpr (UNIFORM)
+      load_cw_uniform<1>(a, T, a.S_log + l, cw[l]);
+    else
+      load_cw<1>(a, T, a.S_log + l, cw[l]);
+  }
+  Blk* __restrict__ F4 = a.Fc;
+  Blk* __restrict__ H4 = a.Hc;
+  Blk* __restrict__ H5 = const_cast<Blk*>(a.Hg);
+  const Blk n1 = child1(a.Fp[f >> 1], a.Hp[f >> 1], (int)(f & 1), cw[0][0]);
+  Blk h1;
+  tree_hash<kPrgChaCha8, 1>(n1, &h1, nullptr);
+#pragma unroll 1
+  for (int x = 0; x < 2; ++x) {
+    const Blk n2 = child1(n1, h1, x, cw[1][0]);
+    Blk h2;
+    tree_hash<kPrgChaCha8, 1>(n2, &h2, nullptr);
+#pragma unroll 1
+    for (int y = 0; y < 2; ++y) {
+      const Blk n3 = child1(n2, h2, y, cw[2][0]);
+      Blk h3;
+      tree_hash<kPrgChaCha8, 1>(n3, &h3, nullptr);
+#pragma unroll
+      for (int z = 0; z < 2; ++z) {
+        const Blk n4 = child1(n3, h3, z, cw[3][0]);
+        Blk h4;
+        tree_hash<kPrgChaCha8, 1>(n4, &h4, nullptr);
+        const size_t s4 = ((size_t)x << 2) | ((size_t)y << 1) | (size_t)z;
+        F4[s4 * a.sub_n + f] = n4;
+        H4[s4 * a.sub_n + f] = h4;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const Blk n5 = child1(n4, h4, u, cw[4][0]);
+          Blk h5;
+          tree_hash<kPrgChaCha8, 1>(n5, &h5, nullptr);
+          H5[((s4 << 1) | (size_t)u) * a.sub_n + f] = h5;
+        }
+      }
+    }
+  }
+}
+
 // The leaf level of both parties' run with the last three tree levels fused
 // in (w = 1, 8 output coefficients per thread).  Per landing slot a thread's
 // 8 consecutive leaves d0 .. d0 + 7 descend from one node of level
 // levels - 3; the thread rebuilds that node's parent (level levels - 4) from
-// its stored hash and its own parent's node and hashes, forms the node, and
-// expands 1 + 2 + 4 tree hashes and 8 out-hashes in registers.  Every (tree, node) is expanded by exactly one
-// thread, so the work is the same as hashing the levels one by one, without
-// their frontiers.  The output layer's sums, tau bits and Beaver partials are
+// its stored hash and its own parent's node and hashes (both written by
+// dpf_subtree_kernel, subtree-major), forms the node, and expands 1 + 2 + 4
+// tree hashes and 8 out-hashes in registers.  Every (tree, node) is expanded
+// by exactly one thread, so the work is the same as hashing the levels one by
+// one, without their frontiers.  The output layer's sums, tau bits and Beaver partials are
 // those of leaf_gather_kernel.
 __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
   constexpr int LPT = 8;
@@ -630,9 +689,13 @@ __global__ __launch_bounds__(kTpb) void deep_leaf_kernel(LevelArgs a) {
 #pragma unroll
     for (int l = 0; l < 5; ++l) load_cw_uniform<1>(a, T, Ltop - 1 + l, cw[l]);
     const size_t d0 = rr0 + ((size_t)ds << a.bin_log);  // a multiple of 8
+    // level Ltop node gi and its parent, stored subtree-major under their
+    // level Ltop - 4 ancestor (dpf_subtree_kernel)
     const size_t gi = (size_t)T * Stop + (d0 >> 4);      // level Ltop
-    const Blk n13 = child1(a.Fg[gi >> 1], a.Hg[gi >> 1], (int)(gi & 1), cw[0][0]);
-    const Blk n14 = child1(n13, a.Hp[gi], (int)((d0 >> 3) & 1), cw[1][0]);
+    const size_t gp = gi >> 1;                           // level Ltop - 1
+    const size_t ip = (gp & 7) * a.sub_n + (gp >> 3), ih = (gi & 15) * a.sub_n + (gi >> 4);
+    const Blk n13 = child1(a.Fg[ip], a.Hg[ip], (int)(gi & 1), cw[0][0]);
+    const Blk n14 = child1(n13, a.Hp[ih], (int)((d0 >> 3) & 1), cw[1][0]);
     Blk h14;
     tree_hash<kPrgChaCha8, 1>(n14, &h14, nullptr);
     Blk n15[2], h15[2];
@@ -813,23 +876,48 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     a.chunk_log = ilog2_exact((uint64_t)a.iters) + ilog2_exact((uint64_t)kTpb);
     // the last tree level keeps only its hashes; the leaf level rebuilds
     // those parents from the grandparents (still in the other buffer)
-    if (deep && i == levels - 3) {
-      // the parents' level (levels - 4) is the last one stored, hashes only:
-      // path CWs of the three levels below it, then the fused leaf level
+    if (deep && i == levels - 8) {
+      // Level Ls = levels - 9 is the last one the level kernels build (their
+      // sums give CW_0 .. CW_Ls).  The CWs of the levels below come from the
+      // alpha path; levels Ls + 1 .. levels - 4 are expanded in one launch
+      // (level levels - 5 stored in full, levels - 4 hashes only), then the
+      // leaf level with the last three tree levels fused in.
+      const int Ls = levels - 9;
       const int half = r.ntrees / 2;
       const Blk* dlt1 = r.dltsft1 ? r.dltsft1 : r.dltsft + (size_t)half * levels;
-      Blk *Fg, *Hg;  // level levels - 5: nodes and hashes (the parents' parents)
-      frontier(cur ^ 1, S >> W, Fg, Hg);
       {
         ProfZone pz(r.prof, st, "dpf.path_cw");
         hipLaunchKernelGGL(dpf_path_cw_kernel, dim3((unsigned)((half + kTpb - 1) / kTpb)),
-                           dim3(kTpb), 0, st, Fg, Hg, H, s.cw.as<Blk>(), r.dltsft, dlt1, half,
-                           levels, levels - 3);
+                           dim3(kTpb), 0, st, F, H, s.cw.as<Blk>(), r.dltsft, dlt1, half,
+                           levels, Ls + 1);
         hip_check(hipGetLastError(), "launch dpf_path_cw_kernel");
       }
+      // level Ls + 4 in full into the other buffer; level Ls + 5's hashes
+      // into this one, behind level Ls (2^4 times wider, so no overlap)
+      Blk *F4, *H4, *F5, *H5;
+      frontier(cur ^ 1, S << (W * 4), F4, H4);
+      frontier(cur, S << (W * 5), F5, H5);
+      (void)F5;
+      a.Fc = F4;
+      a.Hc = H4;
+      a.Hg = H5;
+      a.sub_n = (size_t)r.ntrees << (Ls + 1);
+      {
+        ProfZone pz(r.prof, st, "dpf.subtree");
+        const size_t threads = (size_t)r.ntrees << (Ls + 1);
+        const dim3 grid((unsigned)((threads + kTpb - 1) / kTpb));
+        if (Ls + 1 >= kTpbLog)
+          hipLaunchKernelGGL(dpf_subtree_kernel<true>, grid, dim3(kTpb), 0, st, a);
+        else
+          hipLaunchKernelGGL(dpf_subtree_kernel<false>, grid, dim3(kTpb), 0, st, a);
+        hip_check(hipGetLastError(), "launch dpf_subtree_kernel");
+      }
       a.Fp = nullptr;
-      a.Fg = Fg;
-      a.Hg = Hg;
+      a.Hp = H5;
+      a.Fg = F4;
+      a.Hg = H4;
+      a.Fc = nullptr;
+      a.Hc = nullptr;
       a.part = r.partials;
       a.gA = r.g_acc;
       a.gA_party_stride = r.g_party_stride;
@@ -860,9 +948,9 @@ void expand_t(hipStream_t st, const DpfRun& r, DpfScratch& s) {
     if (i < levels) {
       Blk *Fn, *Hn;
       frontier(cur ^ 1, S * M, Fn, Hn);
-      // the last tree level, and in the deep run the last stored level, keep
-      // only their hashes (the next level rebuilds their nodes by XORs)
-      a.Fc = (i == levels - 1 || (deep && i == levels - 4)) ? nullptr : Fn;
+      // the last tree level keeps only its hashes (the leaf level rebuilds
+      // its nodes by XORs)
+      a.Fc = i == levels - 1 ? nullptr : Fn;
       a.Hc = Hn;
       launch_level<W, PRG, kModeHash>(st, a, r.prof);
       cur ^= 1;

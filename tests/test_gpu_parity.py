@@ -345,6 +345,7 @@ def test_aes_ecb_fips_and_random(ctx, pq, golden):
     (1 << 14, 2, 8, 1, O.PRG_CHACHA8),      # 12-level DPFs, 8 levels stored
     (1 << 15, 3, 16, 1, O.PRG_CHACHA8),
     (1 << 16, 2, 4, 1, O.PRG_CHACHA8),      # bins of 2^14
+    (1 << 16, 1, 32, 1, O.PRG_CHACHA8),     # 128 landing slots: two tau words
     (1 << 14, 2, 8, 1, O.PRG_AES)])         # AES keeps the level-by-level path
 def test_ole_matches_oracle(ctx, pq, N, c, t, w, prg):
     seed = 11
