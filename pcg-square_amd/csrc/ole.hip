@@ -173,7 +173,40 @@ __global__ void build_e_kernel(uint64_t* e, const uint64_t* pos, const uint64_t*
   e[(size_t)poly * N + pos[i]] = pay[i];
 }
 
-// X_s = sum_i A_i E_{s,i};  Z_s = sum_i A_i sum_j A_j G_{s,ij}  (NTT domain)
+// X_s = sum_i A_i E_{s,i};  Z_s = sum_i A_i sum_j A_j G_{s,ij}  (NTT domain).
+// C = c known at compile time: every load of a coefficient is issued before
+// the products (independent loads in flight), and A is loaded once for both
+// parties.
+template <int C>
+__global__ void step4_dot_c_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ G,
+                                   const uint64_t* __restrict__ E, uint64_t* __restrict__ out,
+                                   int N, int s0, int ns) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)N) return;
+  uint64_t a[C];
+#pragma unroll
+  for (int i = 0; i < C; ++i) a[i] = A[(size_t)i * N + q];
+  for (int s = s0; s < s0 + ns; ++s) {
+    uint64_t e[C], g[C * C];
+#pragma unroll
+    for (int i = 0; i < C; ++i) e[i] = E[((size_t)s * C + i) * N + q];
+#pragma unroll
+    for (int k = 0; k < C * C; ++k) g[k] = G[((size_t)s * C * C + k) * N + q];
+    uint64_t x = 0, z = 0;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      x = fadd(x, fmul(a[i], e[i]));
+      uint64_t inner = 0;
+#pragma unroll
+      for (int j = 0; j < C; ++j) inner = fadd(inner, fmul(a[j], g[i * C + j]));
+      z = fadd(z, fmul(a[i], inner));
+    }
+    out[((size_t)2 * s) * N + q] = x;
+    out[((size_t)2 * s + 1) * N + q] = z;
+  }
+}
+
+// The same for any c (one party per grid row).
 __global__ void step4_dot_kernel(const uint64_t* __restrict__ A, const uint64_t* __restrict__ G,
                                  const uint64_t* __restrict__ E, uint64_t* __restrict__ out,
                                  int c, int N, int s0) {
@@ -193,6 +226,26 @@ __global__ void step4_dot_kernel(const uint64_t* __restrict__ A, const uint64_t*
   }
   out[((size_t)2 * s) * N + q] = x;
   out[((size_t)2 * s + 1) * N + q] = z;
+}
+
+void launch_step4_dot(hipStream_t st, const uint64_t* A, const uint64_t* G, const uint64_t* E,
+                      uint64_t* out, int c, int N, int s0, int ns) {
+  const dim3 grid((unsigned)((N + 255) / 256));
+  switch (c) {
+    case 2:
+      hipLaunchKernelGGL(step4_dot_c_kernel<2>, grid, dim3(256), 0, st, A, G, E, out, N, s0, ns);
+      break;
+    case 3:
+      hipLaunchKernelGGL(step4_dot_c_kernel<3>, grid, dim3(256), 0, st, A, G, E, out, N, s0, ns);
+      break;
+    case 4:
+      hipLaunchKernelGGL(step4_dot_c_kernel<4>, grid, dim3(256), 0, st, A, G, E, out, N, s0, ns);
+      break;
+    default:
+      hipLaunchKernelGGL(step4_dot_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)ns),
+                         dim3(256), 0, st, A, G, E, out, c, N, s0);
+  }
+  hip_check(hipGetLastError(), "launch step4_dot");
 }
 
 __global__ void zero_kernel(uint64_t* p, size_t n) {
@@ -551,9 +604,7 @@ void plan_step4(pcgsq_ole_plan* pl) {
   pl->ctx->ntt.prof = &pl->ctx->prof;
   pl->ctx->ntt.forward(st, G, 2 * d.nblk + 2 * d.c, d.N, kNttFourStep);
   ProfZone pzd(&pl->ctx->prof, st, "ole.step4_dot");
-  hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 2), dim3(256), 0, st,
-                     pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N, 0);
-  hip_check(hipGetLastError(), "launch step4_dot");
+  launch_step4_dot(st, pl->ahat.as<uint64_t>(), G, E, pl->out.as<uint64_t>(), d.c, d.N, 0, 2);
   pl->ctx->ntt.inverse(st, pl->out.as<uint64_t>(), 4, d.N, kNttFourStep);
 }
 
@@ -796,11 +847,9 @@ void plan_run_party(pcgsq_ole_plan* pl, pcgsq_ole_exchange_fn fn, void* user) {
   pl->ctx->ntt.prof = &pl->ctx->prof;
   pl->ctx->ntt.forward(st, G, d.nblk, d.N, kNttFourStep);
   pl->ctx->ntt.forward(st, E, d.c, d.N, kNttFourStep);
-  hipLaunchKernelGGL(step4_dot_kernel, dim3((d.N + 255) / 256, 1), dim3(256), 0, st,
-                     pl->ahat.as<uint64_t>(), pl->fw.as<uint64_t>(),
-                     pl->fw.as<uint64_t>() + (size_t)2 * d.nblk * d.N, pl->out.as<uint64_t>(),
-                     d.c, d.N, p);
-  hip_check(hipGetLastError(), "launch step4_dot");
+  launch_step4_dot(st, pl->ahat.as<uint64_t>(), pl->fw.as<uint64_t>(),
+                   pl->fw.as<uint64_t>() + (size_t)2 * d.nblk * d.N, pl->out.as<uint64_t>(), d.c,
+                   d.N, p, 1);
   pl->ctx->ntt.inverse(st, pl->out.as<uint64_t>() + (size_t)2 * p * d.N, 2, d.N, kNttFourStep);
   hip_check(hipStreamSynchronize(st), "ole party run");
 }
