@@ -34,10 +34,11 @@ struct NttPassDesc {
   const uint64_t* boff;      // word offset of each transform of the launch (in, in2 and out
                              // alike); nullptr: back to back, transform b at b * 2^N_log
   const uint64_t* tab;       // stage table [n][2]
-  const uint64_t* tw_lo;     // four-step twiddle psi^(+-e), two-level tables [.][2]
-  const uint64_t* tw_hi;
+  const uint64_t* tw;        // first pass of a four-step transform: its twiddle, one
+                             // (w, w') pair per element, at the element's word offset in
+                             // the pass's output [N][2] (so a tile's twiddle loads coalesce
+                             // as its stores do)
   uint64_t scale_w, scale_wq;  // whole-transform inverse: N^-1
-  int tw_lo_bits;
   int c_log;                 // C = 2^c_log columns per tile
   int in_ps_log, in_cs_log, out_ps_log, out_cs_log;
   int N_log;                 // whole transform length (batch stride, twiddle modulus 2N)
@@ -120,7 +121,12 @@ __device__ __forceinline__ uint32_t lds_at(uint32_t c, uint32_t pos, uint32_t co
 
 // r butterfly stages [S0, S0 + R) on one set held in registers.  Hh is the
 // set's block index (its butterfly blocks in stage S0 + q are (Hh << q) | j).
-template <bool INV, int S0, int R>
+// CYC: the table is a cyclic sub-transform's, whose block 0 of every stage
+// has the twiddle 1 (entry 2^s); in the group that starts at stage 0 the
+// block index is known at compile time, and those butterflies skip the
+// multiply (about one butterfly per element of a length-2^n cyclic pass:
+// 2^n - 1 of its n 2^(n-1)).
+template <bool INV, bool CYC, int S0, int R>
 __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_t Hh,
                                                   gcptr __restrict__ tab) {
   constexpr int M = 1 << R;
@@ -135,9 +141,14 @@ __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_
       for (int w = 0; w < M; ++w) {
         if ((w >> bit) & 1) continue;
         const int w2 = w | (1 << bit);
-        const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
         const uint64_t U = red2p(v[w]);
-        const uint64_t T = shoup_lazy_dev(v[w2], t.w, t.wq);
+        uint64_t T;
+        if (CYC && S0 == 0 && (w >> (R - q)) == 0) {
+          T = red2p(v[w2]);  // twiddle 1: [0, 4P) -> [0, 2P), as a Shoup product leaves it
+        } else {
+          const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
+          T = shoup_lazy_dev(v[w2], t.w, t.wq);
+        }
         v[w] = U + T;
         v[w2] = U - T + kTwoP;
       }
@@ -151,8 +162,52 @@ __device__ __forceinline__ void group_butterflies(uint64_t (&v)[1 << R], uint32_
       for (int w = 0; w < M; ++w) {
         if ((w >> bit) & 1) continue;
         const int w2 = w | (1 << bit);
-        const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
         const uint64_t U = v[w], V = v[w2];  // both < 2P
+        v[w] = red2p(U + V);
+        if (CYC && S0 == 0 && (w >> (R - q)) == 0) {
+          v[w2] = red2p(U - V + kTwoP);  // twiddle 1
+        } else {
+          const Pair t = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)(w >> (R - q))));
+          v[w2] = shoup_lazy_dev(U - V + kTwoP, t.w, t.wq);
+        }
+      }
+    }
+  }
+}
+
+// The same stages with the set's 2^R - 1 stage twiddles already in registers
+// (stage q, block j at tw[2^q - 1 + j]): for a group after the first, whose
+// twiddles depend on the set, all of them are loaded before the set's gather
+// so that one wait covers their latency instead of one per stage.
+template <bool INV, int R>
+__device__ __forceinline__ void group_butterflies_pre(uint64_t (&v)[1 << R],
+                                                      const Pair (&tw)[(1 << R) - 1]) {
+  constexpr int M = 1 << R;
+  if constexpr (!INV) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int bit = R - 1 - q;
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        if ((w >> bit) & 1) continue;
+        const int w2 = w | (1 << bit);
+        const Pair t = tw[(1 << q) - 1 + (w >> (R - q))];
+        const uint64_t U = red2p(v[w]);
+        const uint64_t T = shoup_lazy_dev(v[w2], t.w, t.wq);
+        v[w] = U + T;
+        v[w2] = U - T + kTwoP;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+      const int bit = R - 1 - q;
+#pragma unroll
+      for (int w = 0; w < M; ++w) {
+        if ((w >> bit) & 1) continue;
+        const int w2 = w | (1 << bit);
+        const Pair t = tw[(1 << q) - 1 + (w >> (R - q))];
+        const uint64_t U = v[w], V = v[w2];
         v[w] = red2p(U + V);
         v[w2] = shoup_lazy_dev(U - V + kTwoP, t.w, t.wq);
       }
@@ -171,6 +226,8 @@ struct KindTraits {
                                : (K == kInvWhole || K == kInvWholeProd)             ? 2
                                                                                     : 0;
   static constexpr bool TW_SWAP = K == kInvRows || K == kInvRowsProd;
+  // the four-step row passes are cyclic sub-transforms (twiddle-1 blocks)
+  static constexpr bool CYC = K == kFwdRows || K == kInvRows || K == kInvRowsProd;
   // lanes along the columns (unit column stride in HBM) or along the positions
   static constexpr bool LOAD_CFAST = K == kFwdCols || K == kInvRows || K == kInvRowsProd ||
                                      K == kInvCols;
@@ -186,7 +243,7 @@ template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G, bool TWPRE>
 __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
                                           uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
                                           gcptr __restrict__ inb, gcptr __restrict__ in2b,
-                                          gptr __restrict__ outb) {
+                                          gptr __restrict__ outb, gcptr __restrict__ twb) {
   constexpr int M = 1 << R;
   constexpr int TMIN = NLOG - S0 - R;         // log2 of the set stride
   constexpr int SLOG = NLOG - R;              // log2 of sets per column
@@ -213,22 +270,28 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     const uint32_t L = sigma & ((1u << TMIN) - 1);
     const uint32_t Hh = sigma >> TMIN;
     const uint32_t base = (Hh << (TMIN + R)) | L;
+    // a later group (TWPRE): the set's stage twiddles, loaded before its gather
+    constexpr bool STW = TWPRE && S0 > 0 && !(ROLE & kCopy);
+    Pair stw[STW ? M - 1 : 1];
+    if constexpr (STW) {
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int j = 0; j < (1 << q); ++j)
+          stw[(1 << q) - 1 + j] =
+              ld_pair((gcptr)d.tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)j));
+    }
     // a storing first pass with sets of up to 8 elements (TWPRE): issue the
-    // set's four-step twiddle gathers now (8 VGPRs per element), so that their
-    // cache latency overlaps the gather and butterflies below
+    // set's twiddle loads now (4 VGPRs per element), so that their latency
+    // overlaps the gather and butterflies below
     constexpr bool TW = TWPRE && (ROLE & kToHbm) && Tr::STORE == 1 && M <= 8;
-    Pair twa[TW ? M : 1], twb[TW ? M : 1];
+    Pair twa[TW ? M : 1];
     if constexpr (TW) {
-      const uint32_t two_n_mask = (2u << d.N_log) - 1;
-      const uint32_t lo_mask = (1u << d.tw_lo_bits) - 1;
-      const uint32_t cc = c0 + c;
 #pragma unroll
       for (int w = 0; w < M; ++w) {
         const uint32_t pos = base | ((uint32_t)w << TMIN);
         const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
-        const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
-        twa[w] = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
-        twb[w] = ld_pair((gcptr)d.tw_hi, ex >> d.tw_lo_bits);
+        twa[w] = ld_pair(twb, (p << d.out_ps_log) + (c << d.out_cs_log));
       }
     }
     uint64_t v[M];
@@ -252,34 +315,29 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
       for (int w = 0; w < M; ++w)
         v[w] = lds[lds_at<NLOG>(c, base | ((uint32_t)w << TMIN), col_stride)];
     }
-    if constexpr (!(ROLE & kCopy)) group_butterflies<Tr::INV, S0, R>(v, Hh, (gcptr)d.tab);
+    if constexpr (STW) group_butterflies_pre<Tr::INV, R>(v, stw);
+    else if constexpr (!(ROLE & kCopy)) group_butterflies<Tr::INV, Tr::CYC, S0, R>(v, Hh, (gcptr)d.tab);
     if constexpr (ROLE & kToHbm) {
-      const uint32_t two_n_mask = (2u << d.N_log) - 1;
-      const uint32_t lo_mask = (1u << d.tw_lo_bits) - 1;
 #pragma unroll
       for (int w = 0; w < M; ++w) {
         // bound the twiddle loads in flight (each is 4 VGPRs)
         if (!TW && (w & 3) == 0) __builtin_amdgcn_sched_barrier(0);
         const uint32_t pos = base | ((uint32_t)w << TMIN);
         const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
+        const uint32_t off = (p << d.out_ps_log) + (c << d.out_cs_log);
         uint64_t x = v[w];
         if constexpr (TW) {
           x = shoup_lazy_dev(x, twa[w].w, twa[w].wq);
-          x = shoup_lazy_dev(x, twb[w].w, twb[w].wq);
         } else if constexpr (Tr::STORE == 1) {
-          const uint32_t cc = c0 + c;
-          const uint32_t ex = (Tr::TW_SWAP ? p * (2 * cc + 1) : cc * (2 * p + 1)) & two_n_mask;
-          const Pair a = ld_pair((gcptr)d.tw_lo, ex & lo_mask);
-          const Pair b = ld_pair((gcptr)d.tw_hi, ex >> d.tw_lo_bits);
+          const Pair a = ld_pair(twb, off);
           x = shoup_lazy_dev(x, a.w, a.wq);
-          x = shoup_lazy_dev(x, b.w, b.wq);
         } else if constexpr (Tr::STORE == 2) {
           x = shoup_lazy_dev(x, d.scale_w, d.scale_wq);
         }
         // a first pass hands its [0, 2P) values to the second as they are
         // (forward stages take [0, 4P), inverse stages [0, 2P)); only a
         // transform's final pass reduces to [0, P)
-        outb[(p << d.out_ps_log) + (c << d.out_cs_log)] = Tr::STORE == 1 ? x : red_full(x);
+        outb[off] = Tr::STORE == 1 ? x : red_full(x);
       }
     } else if constexpr (G == 3) {
       const uint32_t sw = c * col_stride + lds_lin(base);
@@ -322,38 +380,41 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
   gcptr in2b = KindTraits<K>::PROD ? (gcptr)(d.in2 + base + ((size_t)c0 << d.in_cs_log))
                                    : nullptr;
   gptr outb = (gptr)(d.out + base + ((size_t)c0 << d.out_cs_log));
+  // the twiddle table is per length (shared by every transform of the launch)
+  gcptr twb = KindTraits<K>::STORE == 1 ? (gcptr)(d.tw + ((size_t)c0 << d.out_cs_log) * 2)
+                                        : nullptr;
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every
     // element before any bit-reversed store lands
-    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+    if (active) run_group<K, NLOG, 0, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
     __syncthreads();
     if (active)
-      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      run_group<K, NLOG, 0, RA, RMAX, kToHbm | kCopy, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
   } else if constexpr (P::G == 2) {
     constexpr int SA = 0, SB = RA;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
     } else {
-      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
     }
   } else {
     constexpr int SA = 0, SB = RA, SC = RA + RB;
     if constexpr (!KindTraits<K>::INV) {
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
     } else {
-      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SC, RC, RMAX, kFromHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SB, RB, RMAX, 0, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
       __syncthreads();
-      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb);
+      if (active) run_group<K, NLOG, SA, RA, RMAX, kToHbm, P::G, TWPRE>(d, lds, t, T, C, c0, inb, in2b, outb, twb);
     }
   }
 }

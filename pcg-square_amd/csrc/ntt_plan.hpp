@@ -33,13 +33,25 @@ inline void upload(DevBuf& buf, const std::vector<uint64_t>& v, const char* what
             what);
 }
 
+// Fill the four-step twiddle tables of a split N = 2^n1_log * 2^n2_log, one
+// (w, floor(w 2^64 / P)) pair per element, indexed by the word offset at which
+// the first pass stores that element (ntt.hip):
+//   fwd[p 2^n2_log + c] = psi^(c (2p + 1)),       p < 2^n1_log, c < 2^n2_log
+//   inv[k 2^n2_log + p] = psi^-(p (2k + 1)) N^-1, k < 2^n1_log, p < 2^n2_log
+// from the two-level tables lo[e mod 2^lo_bits], hi[e >> lo_bits] of psi^(+-e)
+// (the inverse high table carries N^-1).  Runs on a stream of its own and
+// returns when the tables are written.
+void ntt_build_twiddles(uint64_t* fwd, uint64_t* inv, const uint64_t* lo, const uint64_t* hi,
+                        const uint64_t* loi, const uint64_t* hii, int n1_log, int n2_log,
+                        int lo_bits);
+
 struct NttPlan {
-  int N = 0, L = 0, n1_log = 0, n2_log = 0, tw_lo_bits = 0;
+  int N = 0, L = 0, n1_log = 0, n2_log = 0;
   int variant = 0;           // bit 0: alternative stage grouping of the n1 passes, bit 1: n2
   int cols1 = 0, cols2 = 0;  // columns per tile of the n1 / n2 passes (0 = automatic)
   uint64_t psi = 0, ninv = 0, ninv_q = 0;
   DevBuf n1_neg_fwd, n1_neg_inv, n2_cyc_fwd, n2_cyc_inv;
-  DevBuf tw_fwd_lo, tw_fwd_hi, tw_inv_lo, tw_inv_hi;
+  DevBuf tw_fwd, tw_inv;  // four-step twiddles, [N][2] each (ntt_build_twiddles)
   bool naive_ready = false;
   DevBuf nv_w, nv_winv, nv_twf, nv_twi;
 
@@ -79,23 +91,30 @@ struct NttPlan {
     }
     upload(n2_cyc_fwd, with_companions(g), "ntt tables");
     upload(n2_cyc_inv, with_companions(gi), "ntt tables");
-    // two-level psi^(+-e), e < 2N; the inverse high table carries N^-1
-    tw_lo_bits = (L + 2) / 2;
-    const int hi_bits = L + 1 - tw_lo_bits;
-    std::vector<uint64_t> lo(1 << tw_lo_bits), hi(1 << hi_bits), loi(1 << tw_lo_bits),
+    // two-level psi^(+-e), e < 2N (the inverse high table carries N^-1), from
+    // which the device fills the per-element tables
+    const int lo_bits = (L + 2) / 2;
+    const int hi_bits = L + 1 - lo_bits;
+    std::vector<uint64_t> lo(1 << lo_bits), hi(1 << hi_bits), loi(1 << lo_bits),
         hii(1 << hi_bits);
     for (size_t x = 0; x < lo.size(); ++x) {
       lo[x] = hpow(psi, x);
       loi[x] = hpow(psi_inv, x);
     }
     for (size_t y = 0; y < hi.size(); ++y) {
-      hi[y] = hpow(psi, y << tw_lo_bits);
-      hii[y] = hmul(hpow(psi_inv, y << tw_lo_bits), ninv);
+      hi[y] = hpow(psi, y << lo_bits);
+      hii[y] = hmul(hpow(psi_inv, y << lo_bits), ninv);
     }
-    upload(tw_fwd_lo, with_companions(lo), "ntt twiddles");
-    upload(tw_fwd_hi, with_companions(hi), "ntt twiddles");
-    upload(tw_inv_lo, with_companions(loi), "ntt twiddles");
-    upload(tw_inv_hi, with_companions(hii), "ntt twiddles");
+    DevBuf dlo, dhi, dloi, dhii;
+    upload(dlo, with_companions(lo), "ntt twiddles");
+    upload(dhi, with_companions(hi), "ntt twiddles");
+    upload(dloi, with_companions(loi), "ntt twiddles");
+    upload(dhii, with_companions(hii), "ntt twiddles");
+    tw_fwd.ensure((size_t)N * 16, "ntt twiddles");
+    tw_inv.ensure((size_t)N * 16, "ntt twiddles");
+    ntt_build_twiddles(tw_fwd.as<uint64_t>(), tw_inv.as<uint64_t>(), dlo.as<uint64_t>(),
+                       dhi.as<uint64_t>(), dloi.as<uint64_t>(), dhii.as<uint64_t>(), n1_log,
+                       n2_log, lo_bits);
   }
 
   void ensure_naive() {
@@ -155,7 +174,6 @@ inline PassSpec plan_pass(const NttPlan& pl, int phase, const uint64_t* in, cons
   d.in2 = in2;
   d.out = out;
   d.N_log = pl.L;
-  d.tw_lo_bits = pl.tw_lo_bits;
   d.scale_w = pl.ninv;
   d.scale_wq = pl.ninv_q;
   const bool v1 = pl.variant & 1, v2 = (pl.variant >> 1) & 1;
@@ -173,7 +191,7 @@ inline PassSpec plan_pass(const NttPlan& pl, int phase, const uint64_t* in, cons
       p.kind = kFwdCols; p.nlog = pl.n1_log; p.variant = v1; p.cols = pl.cols1;
       p.ncols = 1 << pl.n2_log;
       d.tab = pl.n1_neg_fwd.as<uint64_t>();
-      d.tw_lo = pl.tw_fwd_lo.as<uint64_t>(); d.tw_hi = pl.tw_fwd_hi.as<uint64_t>();
+      d.tw = pl.tw_fwd.as<uint64_t>();
       d.in_ps_log = pl.n2_log; d.in_cs_log = 0; d.out_ps_log = pl.n2_log; d.out_cs_log = 0;
       break;
     case kPhFwd2:  // rows k1, length-n2 cyclic over i2, written transposed (natural k)
@@ -186,7 +204,7 @@ inline PassSpec plan_pass(const NttPlan& pl, int phase, const uint64_t* in, cons
       p.kind = in2 ? kInvRowsProd : kInvRows; p.nlog = pl.n2_log; p.variant = v2;
       p.cols = pl.cols2; p.ncols = 1 << pl.n1_log;
       d.tab = pl.n2_cyc_inv.as<uint64_t>();
-      d.tw_lo = pl.tw_inv_lo.as<uint64_t>(); d.tw_hi = pl.tw_inv_hi.as<uint64_t>();
+      d.tw = pl.tw_inv.as<uint64_t>();
       d.in_ps_log = pl.n1_log; d.in_cs_log = 0; d.out_ps_log = 0; d.out_cs_log = pl.n2_log;
       break;
     default:  // columns i2, inverse negacyclic over k1
