@@ -270,8 +270,10 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     const uint32_t L = sigma & ((1u << TMIN) - 1);
     const uint32_t Hh = sigma >> TMIN;
     const uint32_t base = (Hh << (TMIN + R)) | L;
-    // a later group (TWPRE): the set's stage twiddles, loaded before its gather
-    constexpr bool STW = TWPRE && S0 > 0 && !(ROLE & kCopy);
+    // a later group: the set's stage twiddles, loaded before its gather (in
+    // the batch kernel, whose registers are those of its largest shape, for
+    // groups of up to 3 stages: 7 pairs)
+    constexpr bool STW = S0 > 0 && !(ROLE & kCopy) && (TWPRE || R <= 3);
     Pair stw[STW ? M - 1 : 1];
     if constexpr (STW) {
 #pragma unroll
