@@ -91,16 +91,6 @@ __global__ void pointwise_kernel(uint64_t* out, const uint64_t* a, const uint64_
 
 unsigned grid_for(size_t n) { return (unsigned)((n + kEwThreads - 1) / kEwThreads); }
 
-// floor(w 2^64 / P) for w < P without a division: floor(2^128 / P) is
-// 2^66 + 16 c with c = 3 2^25 - 1, so q = 4w + floor(16 c w / 2^64) is at most
-// one below the companion, and w 2^64 - q P (exact mod 2^64: it is < 2P)
-// tells which.
-__device__ __forceinline__ uint64_t shoup_companion_dev(uint64_t w) {
-  uint64_t q = 4 * w + mulhi64(w, 16 * kC62);
-  if (0ULL - q * kP >= kP) ++q;
-  return q;
-}
-
 // Per-element four-step twiddles (ntt_plan.hpp), one thread per element:
 // both tables' entry idx = a 2^n2_log + b holds psi^(+-e) with e = b (2a + 1)
 // mod 2N, a product of one entry of each two-level table.
@@ -113,12 +103,8 @@ __global__ void build_twiddles_kernel(uint64_t* fwd, uint64_t* inv, const uint64
   const uint32_t a = idx >> n2_log, b = idx & ((1u << n2_log) - 1);
   const uint32_t e = (b * (2 * a + 1)) & ((2u << L) - 1);
   const uint32_t el = e & ((1u << lo_bits) - 1), eh = e >> lo_bits;
-  const uint64_t wf = shoup(lo[2 * el], hi[2 * eh], hi[2 * eh + 1]);
-  const uint64_t wi = shoup(loi[2 * el], hii[2 * eh], hii[2 * eh + 1]);
-  fwd[2 * (size_t)idx] = wf;
-  fwd[2 * (size_t)idx + 1] = shoup_companion_dev(wf);
-  inv[2 * (size_t)idx] = wi;
-  inv[2 * (size_t)idx + 1] = shoup_companion_dev(wi);
+  fwd[idx] = shoup(lo[2 * el], hi[2 * eh], hi[2 * eh + 1]);
+  inv[idx] = shoup(loi[2 * el], hii[2 * eh], hii[2 * eh + 1]);
 }
 
 }  // namespace

@@ -34,10 +34,10 @@ struct NttPassDesc {
   const uint64_t* boff;      // word offset of each transform of the launch (in, in2 and out
                              // alike); nullptr: back to back, transform b at b * 2^N_log
   const uint64_t* tab;       // stage table [n][2]
-  const uint64_t* tw;        // first pass of a four-step transform: its twiddle, one
-                             // (w, w') pair per element, at the element's word offset in
-                             // the pass's output [N][2] (so a tile's twiddle loads coalesce
-                             // as its stores do)
+  const uint64_t* tw;        // first pass of a four-step transform: its twiddle w, one word
+                             // per element, at the element's word offset in the pass's
+                             // output [N] (so a tile's twiddle loads coalesce as its stores
+                             // do); the Shoup constant is formed from w (shoup_w_approx)
   uint64_t scale_w, scale_wq;  // whole-transform inverse: N^-1
   int c_log;                 // C = 2^c_log columns per tile
   int in_ps_log, in_cs_log, out_ps_log, out_cs_log;
@@ -67,6 +67,15 @@ __device__ __forceinline__ uint64_t red2p(uint64_t x) { return x >= kTwoP ? x - 
 __device__ __forceinline__ uint64_t red_full(uint64_t x) {
   x = x >= kTwoP ? x - kTwoP : x;
   return x >= kP ? x - kP : x;
+}
+
+// floor(w 2^64 / P) or one less, for w < P: floor(2^128 / P) = 2^66 + 16 c
+// with c = 3 2^25 - 1 gives 4w + floor(16 c w / 2^64), at most one below.  A
+// Shoup product x w - floor(x q / 2^64) P with this q in place of the exact
+// constant lies in [0, 3P) for any x < 2^64 (the quotient is at most two
+// below floor(x w / P) instead of one).
+__device__ __forceinline__ uint64_t shoup_w_approx(uint64_t w) {
+  return 4 * w + mulhi64(w, 16 * kC62);
 }
 
 __device__ __forceinline__ uint32_t brv(uint32_t x, int bits) {
@@ -284,16 +293,16 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
               ld_pair((gcptr)d.tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)j));
     }
     // a storing first pass with sets of up to 8 elements (TWPRE): issue the
-    // set's twiddle loads now (4 VGPRs per element), so that their latency
+    // set's twiddle loads now (2 VGPRs per element), so that their latency
     // overlaps the gather and butterflies below
     constexpr bool TW = TWPRE && (ROLE & kToHbm) && Tr::STORE == 1 && M <= 8;
-    Pair twa[TW ? M : 1];
+    uint64_t twa[TW ? M : 1];
     if constexpr (TW) {
 #pragma unroll
       for (int w = 0; w < M; ++w) {
         const uint32_t pos = base | ((uint32_t)w << TMIN);
         const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
-        twa[w] = ld_pair(twb, (p << d.out_ps_log) + (c << d.out_cs_log));
+        twa[w] = twb[(p << d.out_ps_log) + (c << d.out_cs_log)];
       }
     }
     uint64_t v[M];
@@ -328,11 +337,13 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
         const uint32_t p = Tr::OUT_BRV ? brv(pos, NLOG) : pos;
         const uint32_t off = (p << d.out_ps_log) + (c << d.out_cs_log);
         uint64_t x = v[w];
-        if constexpr (TW) {
-          x = shoup_lazy_dev(x, twa[w].w, twa[w].wq);
-        } else if constexpr (Tr::STORE == 1) {
-          const Pair a = ld_pair(twb, off);
-          x = shoup_lazy_dev(x, a.w, a.wq);
+        if constexpr (Tr::STORE == 1) {
+          // the four-step twiddle; with its approximate Shoup constant the
+          // product lies in [0, 3P): a forward second pass takes it as it
+          // is ([0, 4P)), an inverse one needs [0, 2P)
+          const uint64_t tw = TW ? twa[TW ? w : 0] : twb[off];
+          x = shoup_lazy_dev(x, tw, shoup_w_approx(tw));
+          if constexpr (Tr::INV) x = red2p(x);
         } else if constexpr (Tr::STORE == 2) {
           x = shoup_lazy_dev(x, d.scale_w, d.scale_wq);
         }
@@ -383,7 +394,7 @@ __device__ __forceinline__ void pass_tile(const NttPassDesc& d, uint64_t* __rest
                                    : nullptr;
   gptr outb = (gptr)(d.out + base + ((size_t)c0 << d.out_cs_log));
   // the twiddle table is per length (shared by every transform of the launch)
-  gcptr twb = KindTraits<K>::STORE == 1 ? (gcptr)(d.tw + ((size_t)c0 << d.out_cs_log) * 2)
+  gcptr twb = KindTraits<K>::STORE == 1 ? (gcptr)(d.tw + ((size_t)c0 << d.out_cs_log))
                                         : nullptr;
   if constexpr (P::G == 1) {
     // one group: stage through LDS so that an in-place pass reads every

@@ -34,8 +34,8 @@ inline void upload(DevBuf& buf, const std::vector<uint64_t>& v, const char* what
 }
 
 // Fill the four-step twiddle tables of a split N = 2^n1_log * 2^n2_log, one
-// (w, floor(w 2^64 / P)) pair per element, indexed by the word offset at which
-// the first pass stores that element (ntt.hip):
+// word w < P per element, indexed by the word offset at which the first pass
+// stores that element (ntt.hip):
 //   fwd[p 2^n2_log + c] = psi^(c (2p + 1)),       p < 2^n1_log, c < 2^n2_log
 //   inv[k 2^n2_log + p] = psi^-(p (2k + 1)) N^-1, k < 2^n1_log, p < 2^n2_log
 // from the two-level tables lo[e mod 2^lo_bits], hi[e >> lo_bits] of psi^(+-e)
@@ -51,7 +51,7 @@ struct NttPlan {
   int cols1 = 0, cols2 = 0;  // columns per tile of the n1 / n2 passes (0 = automatic)
   uint64_t psi = 0, ninv = 0, ninv_q = 0;
   DevBuf n1_neg_fwd, n1_neg_inv, n2_cyc_fwd, n2_cyc_inv;
-  DevBuf tw_fwd, tw_inv;  // four-step twiddles, [N][2] each (ntt_build_twiddles)
+  DevBuf tw_fwd, tw_inv;  // four-step twiddles, [N] each (ntt_build_twiddles)
   bool naive_ready = false;
   DevBuf nv_w, nv_winv, nv_twf, nv_twi;
 
@@ -110,8 +110,8 @@ struct NttPlan {
     upload(dhi, with_companions(hi), "ntt twiddles");
     upload(dloi, with_companions(loi), "ntt twiddles");
     upload(dhii, with_companions(hii), "ntt twiddles");
-    tw_fwd.ensure((size_t)N * 16, "ntt twiddles");
-    tw_inv.ensure((size_t)N * 16, "ntt twiddles");
+    tw_fwd.ensure((size_t)N * 8, "ntt twiddles");
+    tw_inv.ensure((size_t)N * 8, "ntt twiddles");
     ntt_build_twiddles(tw_fwd.as<uint64_t>(), tw_inv.as<uint64_t>(), dlo.as<uint64_t>(),
                        dhi.as<uint64_t>(), dloi.as<uint64_t>(), dhii.as<uint64_t>(), n1_log,
                        n2_log, lo_bits);
