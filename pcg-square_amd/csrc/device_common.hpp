@@ -118,6 +118,13 @@ __device__ __forceinline__ uint64_t shoup_lazy_dev(uint64_t x, uint64_t w, uint6
   return ((uint64_t)hi << 32) | (uint32_t)lo;
 }
 
+// floor(w 2^64 / P) or one less, for w < P: floor(2^128 / P) = 2^66 + 16 c
+// with c = 3 2^25 - 1 gives 4w + floor(16 c w / 2^64), at most one below.  A
+// Shoup product (shoup_lazy) with this constant in place of the exact one lies
+// in [0, 3P) for any x < 2^64 (the quotient is at most two below
+// floor(x w / P) instead of one).
+PCGSQ_HD uint64_t shoup_w_approx(uint64_t w) { return 4 * w + mulhi64(w, 16 * kC62); }
+
 PCGSQ_HD uint64_t shoup(uint64_t x, uint64_t w, uint64_t wq) {
   uint64_t r = shoup_lazy(x, w, wq);
   return r >= kP ? r - kP : r;

@@ -69,15 +69,6 @@ __device__ __forceinline__ uint64_t red_full(uint64_t x) {
   return x >= kP ? x - kP : x;
 }
 
-// floor(w 2^64 / P) or one less, for w < P: floor(2^128 / P) = 2^66 + 16 c
-// with c = 3 2^25 - 1 gives 4w + floor(16 c w / 2^64), at most one below.  A
-// Shoup product x w - floor(x q / 2^64) P with this q in place of the exact
-// constant lies in [0, 3P) for any x < 2^64 (the quotient is at most two
-// below floor(x w / P) instead of one).
-__device__ __forceinline__ uint64_t shoup_w_approx(uint64_t w) {
-  return 4 * w + mulhi64(w, 16 * kC62);
-}
-
 __device__ __forceinline__ uint32_t brv(uint32_t x, int bits) {
   return bits ? __brev(x) >> (32 - bits) : 0u;
 }

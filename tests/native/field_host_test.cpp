@@ -36,6 +36,12 @@ int main(int argc, char** argv) {
     if (s != (uint64_t)(((u128)x * b) % kP)) { if (fails++ < 5) printf("shoup\n"); }
     const uint64_t sl = shoup_lazy(x, b, wq);
     if (sl >= 2 * kP || sl % kP != s) { if (fails++ < 5) printf("shoup_lazy range\n"); }
+    // the NTT first passes' constant formed from w alone: wq or wq - 1, and a
+    // product with it in [0, 3P)
+    const uint64_t wa = shoup_w_approx(b);
+    if (wa != wq && wa + 1 != wq) { if (fails++ < 5) printf("shoup_w_approx\n"); }
+    const uint64_t sa = shoup_lazy(x, b, wa);
+    if (sa >= 3 * kP || sa % kP != s) { if (fails++ < 5) printf("shoup approx range\n"); }
     const uint64_t y = rnd();
     if (freduce64(y) != y % kP) { if (fails++ < 5) printf("freduce64\n"); }
     if (mulhi64(y, x) != (uint64_t)(((u128)y * x) >> 64)) { if (fails++ < 5) printf("mulhi64\n"); }
