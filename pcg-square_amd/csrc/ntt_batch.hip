@@ -407,19 +407,41 @@ struct pcgsq_ntt_batch {
         lds = std::max(lds, ntt_tile_lds_bytes(b.nlog, C, batch_groups(b.nlog)));
       }
     }
-    // one pass-major work list per XCD, longest transforms first within a pass
+    // One pass-major work list per XCD, longest transforms first within a
+    // pass.  In the first passes, which read their length's per-element
+    // twiddle table (the same for every transform of that length), the
+    // list's transforms of one length go tile-major: a group of tiles (those
+    // sharing 128-byte lines) of each such transform in turn, so that the
+    // group's slice of the table is read from HBM once per XCD, not once per
+    // transform.
     std::vector<BatchItem> it;
     queues = BatchQueues{};
     queues.n = (uint32_t)nq;
     for (int qi = 0; qi < nq; ++qi) {
       queues.beg[qi] = (uint32_t)it.size();
-      for (int p = 0; p < 4; ++p)
-        for (int x : order) {
-          if (xq[(size_t)x] != qi) continue;
-          const BatchPhase& b = ph[(size_t)x * 4 + p];
-          for (int t = 0; t < b.tiles; ++t)
-            it.push_back(BatchItem{(uint32_t)(x * 4 + p), (uint32_t)t});
+      for (int p = 0; p < 4; ++p) {
+        const bool first = p == kPhFwd1 || p == kPhInv1;
+        for (size_t i = 0; i < order.size();) {
+          // the run of this list's transforms of one length (order is sorted by length)
+          const int L = logN[(size_t)order[i]];
+          std::vector<int> xs;
+          size_t j = i;
+          for (; j < order.size() && logN[(size_t)order[j]] == L; ++j)
+            if (xq[(size_t)order[j]] == qi) xs.push_back(order[j]);
+          i = j;
+          if (xs.empty()) continue;
+          const BatchPhase& b0 = ph[(size_t)xs[0] * 4 + p];
+          const int grp = (first && b0.kind >= 0 && b0.kind != kFwdWhole && b0.kind != kInvWhole)
+                              ? std::max(1, 16 >> b0.d.c_log)
+                              : b0.tiles;
+          for (int g0 = 0; g0 < b0.tiles; g0 += std::max(1, grp))
+            for (int x : xs) {
+              const BatchPhase& b = ph[(size_t)x * 4 + p];
+              for (int t = g0; t < std::min(b.tiles, g0 + std::max(1, grp)); ++t)
+                it.push_back(BatchItem{(uint32_t)(x * 4 + p), (uint32_t)t});
+            }
         }
+      }
       queues.end[qi] = (uint32_t)it.size();
     }
     nitems = (uint32_t)it.size();
