@@ -690,6 +690,21 @@ def main() -> int:
                 "frac": ntt_bytes_per_pass / (ntt_avg * 1e-3) / HBM_PEAK,
                 "traffic": pmc_traffic(traffic_file, "ntt_fwd_pass_hbm_bytes_per_launch"),
                 "avg_launch_ms": ntt_avg}
+    # output layer after Beaver (gather_fix_kernel): per coefficient of each
+    # party's c^2 blocks, g read and written and one tau word per 64 landing
+    # slots read (the +-CW of the set bits added in)
+    roof_conv = None
+    if "out.gather_fix" in zones:
+        n_gf, ms_gf = zones["out.gather_fix"]
+        gf_avg = ms_gf / n_gf
+        conv_bytes = 2 * c * c * N * (16 + 8 * ((4 * t + 63) // 64))
+        roof_conv = {"bound": "hbm",
+                     "kernel": "gather_fix_kernel (output layer: the +-CW terms of the set tau "
+                               "bits into g, both parties, %d blocks, N=%d)" % (c * c, N),
+                     "achieved": conv_bytes / (gf_avg * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": conv_bytes / (gf_avg * 1e-3) / HBM_PEAK,
+                     "traffic": pmc_traffic(traffic_file, "gather_fix_hbm_bytes_per_launch"),
+                     "algorithmic_bytes": conv_bytes, "avg_launch_ms": gf_avg}
     out = {
         "metric": METRIC,
         "value": N * args.steps / dt,
@@ -723,6 +738,7 @@ def main() -> int:
             "and_gates_per_s": c * c * t * t * gmw_and_gates(N, t) / (deal_ms[True] * 1e-3)},
         "roofline": roofline,
         "roofline_ntt": roof_ntt,
+        "roofline_convert": roof_conv,
         "phases_ms": {k: v[1] / nprof for k, v in sorted(zones.items())},
         "dpf_chacha8_rate_Gblocks_per_s": dpf_rate / 1e9,
         "dpf_ms_per_expansion": dpf_ms,
