@@ -271,6 +271,60 @@ def bench_dpf_batch(ctx, pcgsq, ceiling_blocks: float, reps: int = 5) -> dict:
     return out
 
 
+# The reference's published poly-mul recipe (GPU_baseline/ntt_batch_bench.cu:
+# 47-48,69): N = 2^20, batch 16, a[j] = (0x9e3779b97f4a7c15 (j+1)) mod 2^64 mod P,
+# b[j] = (0xc2b2ae3d27d4eb4f (j+7)) mod 2^64 mod P; its XOR of all outputs
+# (GPU_baseline/results_l40s/ntt_4step.txt:7) and its ms per product
+# (device time between the input and output copies, divided by the batch).
+POLYMUL_XOR_2_20 = 0x3969340B039F5D5D
+POLYMUL_PUBLISHED_MS = {
+    "l40s_four_step_standalone_brev": 0.6889,   # GPU_baseline/final_l40s/b3_ntt.csv:95
+    "l40s_four_step_folded_brev": 0.5401,       # b3_ntt.csv:96
+    "l40s_naive": 1.6170,                       # b3_ntt.csv:97
+    "b200_four_step": 0.2479,                   # pcg_baseline_out_235933_prev/ntt_stages.csv:64
+    "b200_naive": 0.2671,                       # ntt_stages.csv:65
+}
+
+
+def bench_polymul_recipe(ctx, pcgsq, reps: int = 10) -> dict:
+    """The reference's batch-16 poly-mul at N = 2^20 on both backends, inputs
+    and output resident on the device as in its device-time figure: the
+    products' XOR checked against the reference's published checksum, and
+    ms per product beside the published L40S and B200 figures."""
+    import numpy as np
+    N, batch = 1 << 20, 16
+    j = np.arange(batch * N, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        a = (np.uint64(0x9E3779B97F4A7C15) * (j + np.uint64(1))) % np.uint64(pcgsq.P)
+        b = (np.uint64(0xC2B2AE3D27D4EB4F) * (j + np.uint64(7))) % np.uint64(pcgsq.P)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    dout = pcgsq.DeviceBuffer(ctx, a.nbytes)
+    res = {"workload": "reference poly-mul recipe (ntt_batch_bench.cu:47-48,69): N=2^20, "
+                       "batch 16, inputs and output on the device",
+           "published_ms_per_mul": POLYMUL_PUBLISHED_MS}
+    try:
+        for name, backend in (("four_step", pcgsq.NTT_FOURSTEP), ("naive", pcgsq.NTT_NAIVE)):
+            ctx.poly_mul_dev(dout.ptr, da.ptr, db.ptr, batch, N, backend)
+            ctx.synchronize()
+            xor = int(np.bitwise_xor.reduce(dout.to_host()))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.poly_mul_dev(dout.ptr, da.ptr, db.ptr, batch, N, backend)
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            res[name] = {"ms_per_batch": ms, "ms_per_mul": ms / batch,
+                         "xor": "%016x" % xor, "xor_matches_reference": xor == POLYMUL_XOR_2_20,
+                         "vs_l40s": POLYMUL_PUBLISHED_MS["l40s_four_step_folded_brev"
+                                                         if name == "four_step" else
+                                                         "l40s_naive"] / (ms / batch),
+                         "vs_b200": POLYMUL_PUBLISHED_MS["b200_four_step" if name == "four_step"
+                                                         else "b200_naive"] / (ms / batch)}
+    finally:
+        for x in (da, db, dout):
+            x.close()
+    return res
+
+
 def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
     """Higher DPF arity (up to 15 children per ChaCha8 block) at N = 2^20,
     c = 4: t = 32 (DPF domain 2^16) with w = 1, 2, 4 and t = 8 (domain 2^18)
@@ -534,8 +588,9 @@ def main() -> int:
     ap.add_argument("--c", type=int, default=4)
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--parts", default="expand,ntt_batch,dpf_batch,arity,parties,served",
-                    help="comma list of expand, ntt_batch, dpf_batch, arity, parties, served; "
+    ap.add_argument("--parts", default="expand,ntt_batch,polymul,dpf_batch,arity,parties,served",
+                    help="comma list of expand, ntt_batch, polymul, dpf_batch, arity, parties, "
+                         "served; "
                          "without expand only the named parts run and print (for profiling "
                          "one workload)")
     ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
@@ -546,7 +601,8 @@ def main() -> int:
                          "workgroups per CU as wgs:K (e.g. 20:10,18:10,wgs:2)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
-    unknown = parts - {"expand", "ntt_batch", "dpf_batch", "arity", "parties", "served"}
+    unknown = parts - {"expand", "ntt_batch", "polymul", "dpf_batch", "arity", "parties",
+                       "served"}
     if unknown:
         ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
@@ -567,6 +623,8 @@ def main() -> int:
                 wgs = sp.pop("wgs", 0)
                 res["batched_ntt"] = bench_ntt_batch(ctx, pcgsq, mode=args.ntt_batch_mode,
                                                      splits=sp, wgs_per_cu=wgs)
+            if "polymul" in parts:
+                res["polymul_recipe"] = bench_polymul_recipe(ctx, pcgsq)
             if "dpf_batch" in parts:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:
@@ -639,6 +697,7 @@ def main() -> int:
         hbm = ctx.probe_hbm(1 << 30)
         copy_bps = hbm["copy"]
         ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
+        polymul = bench_polymul_recipe(ctx, pcgsq) if "polymul" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
         arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
     two_parties = bench_two_parties(pcgsq, N, c, t) if "parties" in parts else None
@@ -747,6 +806,7 @@ def main() -> int:
         "hbm_probe_GBps": {"copy": hbm["copy"] / 1e9, "read": hbm["read"] / 1e9,
                            "write": hbm["write"] / 1e9, "copy_shape": hbm["copy_shape"]},
         "batched_ntt": ntt_batch,
+        "polymul_recipe": polymul,
         "batched_dpf": dpf_batch,
         "arity": arity,
         "two_party_processes": two_parties,
