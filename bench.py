@@ -325,6 +325,70 @@ def bench_polymul_recipe(ctx, pcgsq, reps: int = 10) -> dict:
     return res
 
 
+# The reference's one-block DPF + convert recipe (GPU_baseline/dpf_real_bench.cu:
+# 48-110: ChaCha8, w = 1, one party, closed-form roots and DltSft, the peer's
+# correction-word share stubbed to zero, host outputs) and its published L40S
+# figures (GPU_baseline/final_l40s/b1_shape.csv:45-46: expand, sums, scatter,
+# total ms) and checksum (GPU_baseline/results_l40s/plateau.txt:7, n=18 B=256).
+DPF_BLOCK_PUBLISHED = {
+    17: {"expand_ms": 4.850, "sums_ms": 0.844, "scatter_ms": 1.346, "total_ms": 7.040},
+    18: {"expand_ms": 9.571, "sums_ms": 1.562, "scatter_ms": 2.559, "total_ms": 13.692,
+         "checksum": "27cb4298e84744ed"},
+}
+
+
+def bench_dpf_block(ctx, pcgsq, reps: int = 5) -> dict:
+    """One DPF block (B = 256 instances, t = 16) of the reference's recipe
+    through the drop-in entry points (pcg_cuda::dpf_gpu_batch_full_eval_device,
+    dpf_out_sums, dpf_out_scatter_g): one party, a host exchange per level
+    returning a zero peer share (in Python here, a C++ lambda there), outputs
+    copied to the host as the reference's are.  Wall time per phase, beside the
+    published L40S figures; at n = 18 the reference's checksum."""
+    import numpy as np
+    B, t = 256, 16
+    res = {"workload": "reference DPF+convert recipe (dpf_real_bench.cu:48-110): one party, "
+                       "B=256, t=16, w=1, ChaCha8, host exchange per level (zero peer share), "
+                       "host outputs"}
+    for n in (17, 18):
+        roots = np.zeros((B, 2), dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            roots[:, 0] = np.uint64(0x9E3779B97F4A7C15) * (np.arange(B, dtype=np.uint64) +
+                                                           np.uint64(1))
+        roots[:, 1] = np.arange(B, dtype=np.uint64) + 1
+        i = np.arange(B * n, dtype=np.uint64)
+        dlt = np.stack([np.uint64(0x1234567) * (i + np.uint64(1)), i], axis=1)
+        CW = (np.uint64(0xABCDEF) * (np.arange(B, dtype=np.uint64) + np.uint64(1))) % \
+            np.uint64(pcgsq.P)
+        D, gN = 1 << n, t * (1 << (n - 1))
+        zero = lambda level, mine: np.zeros_like(mine)
+        g = np.zeros(gN, dtype=np.uint64)   # one host g for every run, as the reference's
+        tt = {"expand_ms": 0.0, "sums_ms": 0.0, "scatter_ms": 0.0}
+        for k in range(reps + 1):
+            t0 = time.perf_counter()
+            d = ctx.dpf_batch_full_eval_device(roots, dlt, n, 1, zero, party=0,
+                                               prg=pcgsq.PRG_CHACHA8)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            sc, st = ctx.dpf_out_sums(d, B, D)
+            t2 = time.perf_counter()
+            ctx.dpf_out_scatter_g(d, B, D, t, 0, CW, gN, out=g)
+            t3 = time.perf_counter()
+            if k:
+                tt["expand_ms"] += (t1 - t0) * 1e3 / reps
+                tt["sums_ms"] += (t2 - t1) * 1e3 / reps
+                tt["scatter_ms"] += (t3 - t2) * 1e3 / reps
+        cks = int(np.bitwise_xor.reduce(sc ^ st)) ^ int(np.bitwise_xor.reduce(g[::97]))
+        tot = sum(tt.values())
+        pub = DPF_BLOCK_PUBLISHED[n]
+        r = dict(tt, total_ms=tot, leaves=B * D, ns_per_leaf=tot * 1e6 / (B * D),
+                 checksum="%016x" % cks, published_l40s=pub,
+                 vs_l40s=pub["total_ms"] / tot)
+        if "checksum" in pub:
+            r["checksum_matches_reference"] = r["checksum"] == pub["checksum"]
+        res[f"n{n}"] = r
+    return res
+
+
 def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
     """Higher DPF arity (up to 15 children per ChaCha8 block) at N = 2^20,
     c = 4: t = 32 (DPF domain 2^16) with w = 1, 2, 4 and t = 8 (domain 2^18)
@@ -588,9 +652,10 @@ def main() -> int:
     ap.add_argument("--c", type=int, default=4)
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--parts", default="expand,ntt_batch,polymul,dpf_batch,arity,parties,served",
-                    help="comma list of expand, ntt_batch, polymul, dpf_batch, arity, parties, "
-                         "served; "
+    ap.add_argument("--parts",
+                    default="expand,ntt_batch,polymul,dpf_block,dpf_batch,arity,parties,served",
+                    help="comma list of expand, ntt_batch, polymul, dpf_block, dpf_batch, arity, "
+                         "parties, served; "
                          "without expand only the named parts run and print (for profiling "
                          "one workload)")
     ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
@@ -601,8 +666,8 @@ def main() -> int:
                          "workgroups per CU as wgs:K (e.g. 20:10,18:10,wgs:2)")
     args = ap.parse_args()
     parts = set(args.parts.split(","))
-    unknown = parts - {"expand", "ntt_batch", "polymul", "dpf_batch", "arity", "parties",
-                       "served"}
+    unknown = parts - {"expand", "ntt_batch", "polymul", "dpf_block", "dpf_batch", "arity",
+                       "parties", "served"}
     if unknown:
         ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
@@ -625,6 +690,8 @@ def main() -> int:
                                                      splits=sp, wgs_per_cu=wgs)
             if "polymul" in parts:
                 res["polymul_recipe"] = bench_polymul_recipe(ctx, pcgsq)
+            if "dpf_block" in parts:
+                res["dpf_block_recipe"] = bench_dpf_block(ctx, pcgsq)
             if "dpf_batch" in parts:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:
@@ -698,6 +765,7 @@ def main() -> int:
         copy_bps = hbm["copy"]
         ntt_batch = bench_ntt_batch(ctx, pcgsq) if "ntt_batch" in parts else None
         polymul = bench_polymul_recipe(ctx, pcgsq) if "polymul" in parts else None
+        dpf_block = bench_dpf_block(ctx, pcgsq) if "dpf_block" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
         arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
     two_parties = bench_two_parties(pcgsq, N, c, t) if "parties" in parts else None
@@ -807,6 +875,7 @@ def main() -> int:
                            "write": hbm["write"] / 1e9, "copy_shape": hbm["copy_shape"]},
         "batched_ntt": ntt_batch,
         "polymul_recipe": polymul,
+        "dpf_block_recipe": dpf_block,
         "batched_dpf": dpf_batch,
         "arity": arity,
         "two_party_processes": two_parties,

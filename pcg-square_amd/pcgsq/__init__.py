@@ -427,11 +427,16 @@ class Context:
         library's fixed per-GPU scratch array)."""
         _check(lib().pcgsq_dpf_out_sums_noalloc(self.h, d_leaves, B, D, P, d_sumC, d_sumT))
 
-    def dpf_out_scatter_g(self, d_leaves: int, B: int, D: int, t: int, party: int, CW, N: int):
+    def dpf_out_scatter_g(self, d_leaves: int, B: int, D: int, t: int, party: int, CW, N: int,
+                          out=None):
+        """g [N] on the host; `out` (optional) is a caller-owned uint64 array of N
+        words that receives it (reused across calls, it saves the page faults of
+        a fresh array)."""
         cw = _u64(CW).reshape(-1)
         if cw.size != B:
             raise ValueError(f"dpf_out_scatter_g: CW has {cw.size} words, expected B = {B}")
-        g = np.empty(N, dtype=np.uint64)
+        g = np.empty(N, dtype=np.uint64) if out is None else _out_array(out, N,
+                                                                         "dpf_out_scatter_g")
         _check(lib().pcgsq_dpf_out_scatter_g(self.h, d_leaves, B, D, t, party, P, _p(cw), _p(g),
                                              N))
         return g
