@@ -389,6 +389,60 @@ def bench_dpf_block(ctx, pcgsq, reps: int = 5) -> dict:
     return res
 
 
+# The reference's one-party end-to-end arms (GPU_baseline/e2e_full_bench.cu,
+# published in GPU_baseline/final_l40s/e2e_arms.csv:32-36: c=4, t=16, N=2^20, one
+# party, peer stubbed, PCG_RTT_US injected per exchange; wall ms).
+E2E_PUBLISHED_MS = {
+    "serial": {"5": 192.1137, "25.2": 198.5512, "2000": 839.7925},
+    "batched": {"5": 189.6091, "25.2": 190.3887},
+}
+
+
+def bench_one_party_stub(pcgsq, N: int, c: int, t: int, reps: int = 3) -> dict:
+    """One party's whole expansion on its own, as the reference's end-to-end
+    bench times it: a one-party plan dealt by the trusted dealer, every
+    exchange answered by a stub peer with zeros after the injected round trip
+    (a busy wait), the serial and the batched arm.  Timing numbers only: with a
+    zero peer the outputs are not the 2PC outputs (the reference's rows are
+    the same)."""
+    import numpy as np
+    res = {"workload": f"N=2^{N.bit_length() - 1}, c={c}, t={t}, ChaCha8: party 0 alone, "
+                       "stub peer (zeros), injected round trip per exchange",
+           "published_l40s_ms": E2E_PUBLISHED_MS}
+    nx = [0]
+
+    def stub(rtt_us):
+        def exchange(phase, level, batch, send, nrecv):
+            nx[0] += 1
+            _spin_us(rtt_us)
+            return np.zeros(nrecv, dtype=np.uint8) if nrecv else None
+        return exchange
+
+    with pcgsq.Context(0) as cx, pcgsq.OleParty(cx, N, c, t, 0) as op:
+        for arm, mode in (("serial", pcgsq.EXCHANGE_PER_BLOCK),
+                          ("batched", pcgsq.EXCHANGE_BATCHED)):
+            op.set_exchange_mode(mode)
+            for rtt in (5.0, 25.2, 2000.0):
+                if rtt == 2000.0 and arm == "batched":
+                    continue
+                n_rep = 1 if rtt >= 1000 else reps
+                times = []
+                for k in range(n_rep + 1):
+                    op.deal(1)
+                    nx[0] = 0
+                    t0 = time.perf_counter()
+                    op.run(stub(rtt))
+                    if k:
+                        times.append(time.perf_counter() - t0)
+                ms = 1e3 * sum(times) / len(times)
+                key = ("%g" % rtt)
+                pub = E2E_PUBLISHED_MS[arm].get(key)
+                res.setdefault(arm, {})[f"rtt_{key}us"] = {
+                    "ms": ms, "exchanges": nx[0], "published_l40s_ms": pub,
+                    "vs_l40s": pub / ms if pub else None}
+    return res
+
+
 def bench_arity(ctx, pcgsq, reps: int = 5) -> dict:
     """Higher DPF arity (up to 15 children per ChaCha8 block) at N = 2^20,
     c = 4: t = 32 (DPF domain 2^16) with w = 1, 2, 4 and t = 8 (domain 2^18)
@@ -653,9 +707,10 @@ def main() -> int:
     ap.add_argument("--t", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--parts",
-                    default="expand,ntt_batch,polymul,dpf_block,dpf_batch,arity,parties,served",
+                    default="expand,ntt_batch,polymul,dpf_block,dpf_batch,arity,one_party,parties,"
+                            "served",
                     help="comma list of expand, ntt_batch, polymul, dpf_block, dpf_batch, arity, "
-                         "parties, served; "
+                         "one_party, parties, served; "
                          "without expand only the named parts run and print (for profiling "
                          "one workload)")
     ap.add_argument("--ntt-batch-mode", choices=("persistent", "grouped"), default=None,
@@ -667,7 +722,7 @@ def main() -> int:
     args = ap.parse_args()
     parts = set(args.parts.split(","))
     unknown = parts - {"expand", "ntt_batch", "polymul", "dpf_block", "dpf_batch", "arity",
-                       "parties", "served"}
+                       "one_party", "parties", "served"}
     if unknown:
         ap.error(f"unknown parts: {sorted(unknown)}")
     if args.gpus != 1:
@@ -696,6 +751,8 @@ def main() -> int:
                 res["batched_dpf"] = bench_dpf_batch(ctx, pcgsq, ctx.probe_chacha8())
             if "arity" in parts:
                 res["arity"] = bench_arity(ctx, pcgsq)
+        if "one_party" in parts:
+            res["one_party_stub_peer"] = bench_one_party_stub(pcgsq, N, c, t)
         if "parties" in parts:
             res["two_party_processes"] = bench_two_parties(pcgsq, N, c, t)
         if "served" in parts:
@@ -768,6 +825,7 @@ def main() -> int:
         dpf_block = bench_dpf_block(ctx, pcgsq) if "dpf_block" in parts else None
         dpf_batch = bench_dpf_batch(ctx, pcgsq, ceiling) if "dpf_batch" in parts else None
         arity = bench_arity(ctx, pcgsq) if "arity" in parts else None
+    one_party = bench_one_party_stub(pcgsq, N, c, t) if "one_party" in parts else None
     two_parties = bench_two_parties(pcgsq, N, c, t) if "parties" in parts else None
     served = bench_served(pcgsq) if "served" in parts else None
 
@@ -878,6 +936,7 @@ def main() -> int:
         "dpf_block_recipe": dpf_block,
         "batched_dpf": dpf_batch,
         "arity": arity,
+        "one_party_stub_peer": one_party,
         "two_party_processes": two_parties,
         "served": served,
     }
