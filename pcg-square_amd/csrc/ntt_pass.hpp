@@ -215,6 +215,35 @@ __device__ __forceinline__ void group_butterflies_pre(uint64_t (&v)[1 << R],
   }
 }
 
+// The 2^R - 1 stage twiddles of a set of a group starting at stage S0
+// (stage q, block j at stw[2^q - 1 + j]), block index Hh.
+template <int S0, int R>
+__device__ __forceinline__ void load_stage_twiddles(Pair (&stw)[(1 << R) - 1],
+                                                    gcptr __restrict__ tab, uint32_t Hh) {
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int j = 0; j < (1 << q); ++j)
+      stw[(1 << q) - 1 + j] = ld_pair(tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)j));
+}
+
+// The same with a wave-uniform Hh, read through the scalar cache: the stage
+// table is written when the plan is made and never during a pass, so it is
+// read as constant memory (scalar loads, off the vector memory path).
+template <int S0, int R>
+__device__ __forceinline__ void load_stage_twiddles_uniform(Pair (&stw)[(1 << R) - 1],
+                                                            const uint64_t* tab, uint32_t Hh) {
+  typedef const __attribute__((address_space(4))) u64x2* cptr2;
+  const cptr2 t = (cptr2)(uintptr_t)tab;
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int j = 0; j < (1 << q); ++j) {
+      const u64x2 v = t[(1u << (S0 + q)) + ((Hh << q) | (uint32_t)j)];
+      stw[(1 << q) - 1 + j] = Pair{v.x, v.y};
+    }
+}
+
 template <int K>
 struct KindTraits {
   static constexpr bool INV = K == kInvRows || K == kInvRowsProd || K == kInvCols ||
@@ -276,12 +305,16 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     constexpr bool STW = S0 > 0 && !(ROLE & kCopy) && (TWPRE || R <= 3);
     Pair stw[STW ? M - 1 : 1];
     if constexpr (STW) {
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-#pragma unroll
-        for (int j = 0; j < (1 << q); ++j)
-          stw[(1 << q) - 1 + j] =
-              ld_pair((gcptr)d.tab, (1u << (S0 + q)) + ((Hh << q) | (uint32_t)j));
+      // A wave's 64 lanes hold sets of one block index when the lanes run
+      // along C columns with C 2^TMIN >= 64, or along the positions with
+      // 2^TMIN >= 64 (waves and tiles are aligned to 64 threads): the
+      // twiddles are then wave-uniform and come by scalar loads, off the
+      // vector memory path.
+      const bool uniform = cfast ? (C << TMIN) >= 64u : TMIN >= 6;
+      if (uniform)
+        load_stage_twiddles_uniform<S0, R>(stw, d.tab, __builtin_amdgcn_readfirstlane(Hh));
+      else
+        load_stage_twiddles<S0, R>(stw, (gcptr)d.tab, Hh);
     }
     // a storing first pass with sets of up to 8 elements (TWPRE): issue the
     // set's twiddle loads now (2 VGPRs per element), so that their latency
