@@ -266,24 +266,38 @@ struct KindTraits {
 // Role of a register group in the pass (kCopy: move the set, no stages).
 enum : int { kFromHbm = 1, kToHbm = 2, kCopy = 4 };
 
+// Shape facts of group `S0, R` of a pass.
+template <int K, int NLOG, int S0, int R, int ROLE, bool TWPRE>
+struct GroupTraits {
+  using Tr = KindTraits<K>;
+  static constexpr int TMIN = NLOG - S0 - R;  // log2 of the set stride
+  // lanes run along the columns when the HBM side of this group has unit
+  // column stride, else along the positions
+  static constexpr bool cfast = (ROLE & kFromHbm) ? Tr::LOAD_CFAST
+                                : (ROLE & kToHbm) ? Tr::STORE_CFAST
+                                                  : true;
+  // a later group: the set's stage twiddles, loaded before its gather (in
+  // the batch kernel, whose registers are those of its largest shape, for
+  // groups of up to 3 stages: 7 pairs)
+  static constexpr bool STW = S0 > 0 && !(ROLE & kCopy) && (TWPRE || R <= 3);
+};
+
 // Group `S0, R` of a pass over a tile: for each set owned by this thread,
 // gather (from HBM or LDS), run the stages, scatter (to LDS or HBM).
-template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G, bool TWPRE>
-__device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
-                                          uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
-                                          gcptr __restrict__ inb, gcptr __restrict__ in2b,
-                                          gptr __restrict__ outb, gcptr __restrict__ twb) {
+// UNI: the set's stage twiddles are wave-uniform and come by scalar loads.
+template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G, bool TWPRE, bool UNI>
+__device__ __forceinline__ void run_group_sets(const NttPassDesc& d, uint64_t* __restrict__ lds,
+                                               uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
+                                               gcptr __restrict__ inb, gcptr __restrict__ in2b,
+                                               gptr __restrict__ outb, gcptr __restrict__ twb) {
+  using GT = GroupTraits<K, NLOG, S0, R, ROLE, TWPRE>;
   constexpr int M = 1 << R;
-  constexpr int TMIN = NLOG - S0 - R;         // log2 of the set stride
+  constexpr int TMIN = GT::TMIN;
   constexpr int SLOG = NLOG - R;              // log2 of sets per column
   constexpr int PER = 1 << (RMAX - R);        // sets per thread
   using Tr = KindTraits<K>;
   const uint32_t col_stride = lds_col_stride(NLOG, (int)C, G);
-  // lanes run along the columns when the HBM side of this group has unit
-  // column stride, else along the positions
-  constexpr bool cfast = (ROLE & kFromHbm) ? Tr::LOAD_CFAST
-                         : (ROLE & kToHbm) ? Tr::STORE_CFAST
-                                           : true;
+  constexpr bool cfast = GT::cfast;
 #pragma unroll 1
   for (int k = 0; k < PER; ++k) {
     const uint32_t g = t + (uint32_t)k * T;
@@ -299,23 +313,12 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
     const uint32_t L = sigma & ((1u << TMIN) - 1);
     const uint32_t Hh = sigma >> TMIN;
     const uint32_t base = (Hh << (TMIN + R)) | L;
-    // a later group: the set's stage twiddles, loaded before its gather (in
-    // the batch kernel, whose registers are those of its largest shape, for
-    // groups of up to 3 stages: 7 pairs)
-    constexpr bool STW = S0 > 0 && !(ROLE & kCopy) && (TWPRE || R <= 3);
+    constexpr bool STW = GT::STW;
     Pair stw[STW ? M - 1 : 1];
-    if constexpr (STW) {
-      // A wave's 64 lanes hold sets of one block index when the lanes run
-      // along C columns with C 2^TMIN >= 64, or along the positions with
-      // 2^TMIN >= 64 (waves and tiles are aligned to 64 threads): the
-      // twiddles are then wave-uniform and come by scalar loads, off the
-      // vector memory path.
-      const bool uniform = cfast ? (C << TMIN) >= 64u : TMIN >= 6;
-      if (uniform)
-        load_stage_twiddles_uniform<S0, R>(stw, d.tab, __builtin_amdgcn_readfirstlane(Hh));
-      else
-        load_stage_twiddles<S0, R>(stw, (gcptr)d.tab, Hh);
-    }
+    if constexpr (STW && UNI)
+      load_stage_twiddles_uniform<S0, R>(stw, d.tab, __builtin_amdgcn_readfirstlane(Hh));
+    else if constexpr (STW)
+      load_stage_twiddles<S0, R>(stw, (gcptr)d.tab, Hh);
     // a storing first pass with sets of up to 8 elements (TWPRE): issue the
     // set's twiddle loads now (2 VGPRs per element), so that their latency
     // overlaps the gather and butterflies below
@@ -385,6 +388,37 @@ __device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __rest
       for (int w = 0; w < M; ++w)
         lds[lds_at<NLOG>(c, base | ((uint32_t)w << TMIN), col_stride)] = v[w];
     }
+  }
+}
+
+// A wave's 64 lanes hold sets of one block index when the lanes run along C
+// columns with C 2^TMIN >= 64, or along the positions with 2^TMIN >= 64
+// (waves and tiles are aligned to 64 threads): the stage twiddles are then
+// wave-uniform and come by scalar loads, off the vector memory path.  The
+// choice holds for the whole group, so it is made once, and each form of the
+// set loop has its own registers.  Not in the batch kernel (TWPRE off),
+// whose registers are those of its largest shape.
+template <int K, int NLOG, int S0, int R, int RMAX, int ROLE, int G, bool TWPRE>
+__device__ __forceinline__ void run_group(const NttPassDesc& d, uint64_t* __restrict__ lds,
+                                          uint32_t t, uint32_t T, uint32_t C, uint32_t c0,
+                                          gcptr __restrict__ inb, gcptr __restrict__ in2b,
+                                          gptr __restrict__ outb, gcptr __restrict__ twb) {
+  using GT = GroupTraits<K, NLOG, S0, R, ROLE, TWPRE>;
+  constexpr bool ANY = GT::STW && TWPRE;
+  constexpr bool ALWAYS = ANY && GT::TMIN >= 6;
+  if constexpr (ALWAYS) {
+    run_group_sets<K, NLOG, S0, R, RMAX, ROLE, G, TWPRE, true>(d, lds, t, T, C, c0, inb, in2b,
+                                                               outb, twb);
+  } else if constexpr (ANY && GT::cfast) {
+    if ((C << GT::TMIN) >= 64u)
+      run_group_sets<K, NLOG, S0, R, RMAX, ROLE, G, TWPRE, true>(d, lds, t, T, C, c0, inb, in2b,
+                                                                 outb, twb);
+    else
+      run_group_sets<K, NLOG, S0, R, RMAX, ROLE, G, TWPRE, false>(d, lds, t, T, C, c0, inb, in2b,
+                                                                  outb, twb);
+  } else {
+    run_group_sets<K, NLOG, S0, R, RMAX, ROLE, G, TWPRE, false>(d, lds, t, T, C, c0, inb, in2b,
+                                                                outb, twb);
   }
 }
 
