@@ -3,7 +3,7 @@
 // A batch holds `count` independent transforms of lengths 2^10..2^22, each in
 // its own slice of the batch buffers.  The host turns every pass of every
 // transform into tiles (the tiles the per-length launches use, shaped for
-// 512 threads) and writes them into work lists.  One persistent kernel
+// 1024 threads) and writes them into work lists.  One persistent kernel
 // drains the lists: a workgroup takes the next item with one atomic, waits
 // until every tile of the pass the item depends on has finished (per-pass
 // completion counters, a bounded spin), runs the tile and counts it done.
@@ -45,11 +45,13 @@
 namespace pcgsq {
 namespace {
 
-// 512 threads: a 2^10 column pass gets 8 columns per tile (64 contiguous
-// bytes of each row) and the per-item bookkeeping (claim, dependency check,
-// descriptor loads, store drain) is spread over twice the work of a
-// 256-thread tile; two such workgroups fill a CU's LDS.
-constexpr int kBatchThreads = 512;
+// 1024 threads: a 2^10 column pass gets 16 columns per tile, whole 128-byte
+// lines of each row (with 8 columns the two halves of a line were read by
+// different tiles, not always from L2: 1.22x the algorithmic bytes against
+// 1.05x now), and the per-item bookkeeping (claim, dependency check,
+// descriptor loads, store drain) is spread over 1024 threads' work; one such
+// workgroup fills a CU's LDS (16 waves, as two 512-thread ones did).
+constexpr int kBatchThreads = 1024;
 constexpr int kMaxQueues = 16;      // work lists (one per XCD)
 constexpr int kCtrLine = 32;        // counters of different lists sit 128 bytes apart
 constexpr int kCtrErr = kMaxQueues * kCtrLine;
