@@ -1,10 +1,12 @@
 """Autotuner for the NTT pass shapes.
 
-For each transform length N it times forward + inverse of a device-resident
-batch for candidate shapes -- one pass or a column/row split (n1 = 2^n1_log),
-the register grouping of each pass (variant bits) and the sub-transforms per
-workgroup tile of each pass (cols1, cols2) -- with the library's own
-per-launch event timing, and keeps the fastest.  Shapes never change results:
+For each transform length N it times candidate shapes -- one pass or a
+column/row split (n1 = 2^n1_log), the register grouping of each pass (variant
+bits) and the sub-transforms per workgroup tile of each pass (cols1, cols2) --
+on the transforms of one PCG-OLE expansion at c = 4: 2c^2 + 2c = 40 forward
+transforms of a device-resident batch and 4 inverse ones (tile widths that
+fill the GPU at 40 transforms are not those that fill it at 16).  It uses the
+library's own per-launch event timing and keeps the fastest.  Shapes never change results:
 every candidate is checked bit-exact against the default shape before it is
 timed.  Splits and groupings are searched exhaustively, tile widths by
 coordinate descent from the best of those.
@@ -12,7 +14,8 @@ coordinate descent from the best of those.
     python pcg-square_amd/pcgsq/autotune.py --out tuning.json
 
 The cache maps the device's gfx name -> {N: [n1_log, variant, cols1, cols2,
-us_per_transform]} and is applied by pcgsq.Context(apply_tuning=True).
+us per forward transform of that mix]} and is applied by
+pcgsq.Context(apply_tuning=True).
 """
 from __future__ import annotations
 
@@ -52,28 +55,29 @@ def variants(L: int, s: int):
     return bits
 
 
-def time_shape(ctx, buf, batch, N, iters=3) -> float:
-    """Average device ms of forward + inverse on `buf` (batch x N)."""
+def time_shape(ctx, buf, batch, inv_batch, N, iters=10) -> float:
+    """Average device ms of `batch` forward transforms on `buf` (batch x N)
+    and `inv_batch` inverse ones, over enough replays that widths a few
+    percent apart order the same way from run to run."""
     ctx.ntt_forward_dev(buf.ptr, batch, N)
-    ctx.ntt_inverse_dev(buf.ptr, batch, N)
+    ctx.ntt_inverse_dev(buf.ptr, inv_batch, N)
     ctx.synchronize()
     ctx.profile(True)
     for _ in range(iters):
         ctx.ntt_forward_dev(buf.ptr, batch, N)
-        ctx.ntt_inverse_dev(buf.ptr, batch, N)
+        ctx.ntt_inverse_dev(buf.ptr, inv_batch, N)
     ctx.synchronize()
     z = ctx.profile_read()
     ctx.profile(False)
     return sum(v[1] for k, v in z.items() if k.startswith("ntt.")) / iters
 
 
-def tune(ctx, Ns, elems=1 << 24, log=print) -> dict:
+def tune(ctx, Ns, batch=40, inv_batch=4, log=print) -> dict:
     import pcgsq
     res = {}
     rng = np.random.default_rng(1)
     for N in Ns:
         L = N.bit_length() - 1
-        batch = max(1, elems // N)
         a = rng.integers(0, pcgsq.P, (batch, N), dtype=np.uint64)
         ctx.ntt_tune(N)                      # default shape gives the reference result
         ref = ctx.ntt_forward(a[:1])
@@ -86,9 +90,9 @@ def tune(ctx, Ns, elems=1 << 24, log=print) -> dict:
             ctx.ntt_tune(N, *shape)
             if not np.array_equal(ctx.ntt_forward(a[:1]), ref):
                 raise RuntimeError(f"shape {shape} changed the result at N={N}")
-            us = time_shape(ctx, buf, batch, N) * 1e3 / batch
+            us = time_shape(ctx, buf, batch, inv_batch, N) * 1e3 / batch
             log(f"N=2^{L} n1_log={shape[0]} variant={shape[1]} cols={shape[2]},{shape[3]}: "
-                f"{us:.2f} us/transform (fwd+inv)")
+                f"{us:.2f} us per forward transform ({inv_batch} inverse per {batch})")
             seen[shape] = us
             return us
 
