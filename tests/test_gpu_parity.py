@@ -227,6 +227,41 @@ def test_ntt_every_tunable_shape_matches_oracle(ctx, pq, logN):
         ctx.ntt_tune(N)
 
 
+def test_autotune_picks_and_applies_shapes(ctx, pq, tmp_path):
+    """The tuner (which checks every candidate bit-exact before timing it)
+    returns a shape per length; a cache written from its result and applied
+    to a fresh context sets those shapes, which give the oracle's transform."""
+    import json
+
+    from pcgsq import autotune as at
+    Ns = [1 << 12, 1 << 14]
+    logs = []
+    res = at.tune(ctx, Ns, batch=4, inv_batch=1, log=logs.append)
+    try:
+        assert sorted(res) == sorted(str(N) for N in Ns)
+        for N in Ns:
+            s, v, c1, c2, us = res[str(N)]
+            L = N.bit_length() - 1
+            assert s in at.splits(N) and v in at.variants(L, s)
+            assert c1 in (0, 1, 2, 4, 8, 16) and c2 in (0, 1, 2, 4, 8, 16) and us > 0
+        assert any(line.startswith("best N=2^14") for line in logs)
+        path = tmp_path / "tuning.json"
+        key = ctx.device_name().split(" (")[-1].split(",")[0]
+        path.write_text(json.dumps({key: res}))
+        with pq.Context(0, apply_tuning=False) as c2x:
+            assert at.apply(c2x, str(path)) == len(Ns)
+            rng = np.random.default_rng(31)
+            for N in Ns:
+                a = rng.integers(0, P, (2, N), dtype=np.uint64)
+                fa = c2x.ntt_forward(a)
+                for k in range(2):
+                    assert np.array_equal(fa[k], O.ntt_forward(a[k]))
+                assert np.array_equal(c2x.ntt_inverse(fa), a)
+    finally:
+        for N in Ns:
+            ctx.ntt_tune(N)
+
+
 @pytest.mark.parametrize("backend", [0, 1])
 @pytest.mark.parametrize("logN", [1, 2, 6, 9, 12, 15, 17])
 def test_poly_mul_matches_oracle(ctx, pq, backend, logN):
