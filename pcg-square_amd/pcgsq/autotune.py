@@ -137,12 +137,14 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_CACHE)
     ap.add_argument("--logN", default="10,12,13,14,16,18,20,22")
+    ap.add_argument("--batch", type=int, default=40, help="forward transforms timed per shape")
+    ap.add_argument("--inv-batch", type=int, default=4, help="inverse transforms timed per shape")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     import pcgsq
     Ns = [1 << int(x) for x in a.logN.split(",")]
     with pcgsq.Context(0, apply_tuning=False) as ctx:
-        res = tune(ctx, Ns)
+        res = tune(ctx, Ns, batch=a.batch, inv_batch=min(a.inv_batch, a.batch))
         key = ctx.device_name().split(" (")[-1].split(",")[0]   # gfx arch string
     cache = {}
     if os.path.exists(a.out):
