@@ -171,6 +171,27 @@ __global__ __launch_bounds__(kTpb) void dpf_level_kernel(LevelArgs a) {
         hx = bxor(hx, h[k]);
       }
       const bool t = (parent.lo & 1ULL) != 0;
+      if constexpr (MODE == kModeHash && W == 1) {
+        // both children hashed before either is stored: the two 16-byte
+        // nodes (and hashes) of a parent go out back to back, so that a
+        // wave's stores cover whole lines instead of every other 16 bytes
+        Blk node[2], hh[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          node[c] = c == 0 ? h[0] : bxor(hx, parent);
+          if (t) node[c] = bxor(node[c], CW[c]);
+          tree_hash<PRG, 1>(node[c], &hh[c], lds);
+          acc[0] ^= hh[c].lo;
+          acc[1] ^= hh[c].hi;
+        }
+        if (a.Fc) {
+          a.Fc[2 * f] = node[0];
+          a.Fc[2 * f + 1] = node[1];
+        }
+        a.Hc[2 * f] = hh[0];
+        a.Hc[2 * f + 1] = hh[1];
+        continue;
+      }
 #pragma unroll
       for (int c = 0; c < M; ++c) {
         Blk node = (c < NK) ? h[c < NK ? c : 0] : bxor(hx, parent);
